@@ -1,0 +1,190 @@
+/*
+ * Deterministic single-precision exp, log and pow for host and device.
+ *
+ * The CPU reference of libcraw calls glibc's expf (NLM weights, temporal
+ * alpha, chroma bilateral), logf (AWB histogram) and powf (ACES gamut
+ * compression). The GPU's own libraries round those differently, and a one
+ * ulp difference in a weight moves a 16-bit re-encode or a histogram bin.
+ * These functions are one source compiled into both the HIP kernels and
+ * the C oracle, so that the two agree bit for bit by construction.
+ *
+ * Method: the argument is taken to double, the function evaluated there to
+ * about 2^-52 relative, and the result rounded once to float. On CDNA4 a
+ * double FMA issues at the same rate as a float one, so this costs the
+ * GPU about what a float polynomial would. Because the double result is
+ * within ~2^-52 of the true value, the float result is the correctly
+ * rounded one except for arguments within that distance of a rounding
+ * midpoint, which is also where glibc's (nearly correctly rounded)
+ * versions would be the ones to differ.
+ *
+ *   exp: Cody-Waite reduction x = k ln2 + r, |r| <= ln2/2, with ln2 split
+ *        into a 32-bit head and a tail (the fdlibm split), degree-12
+ *        Taylor polynomial of exp(r) in Horner form with FMA, scaled by
+ *        2^k through the exponent field.
+ *   log: x = m 2^e with m in [sqrt(1/2), sqrt(2)), log m = 2 atanh(s),
+ *        s = (m - 1) / (m + 1), odd series to s^19.
+ *   pow: exp(y log x) for x >= 0, both halves in double.
+ *
+ * Every operation is an IEEE double add, multiply, divide or fused
+ * multiply-add, or an exact integer manipulation, so host and device
+ * produce identical bits. The translation units that include this must
+ * be built without floating-point contraction (-ffp-contract=off) so that
+ * the compiler fuses nothing on its own, and the host with FMA support
+ * (-mfma), so that fma() is the hardware instruction rather than a libm
+ * emulation (both are correctly rounded; the flag is for speed).
+ */
+#ifndef CRAW_DETMATH_H
+#define CRAW_DETMATH_H
+
+#include <stdint.h>
+#include <math.h>
+
+#if defined(__HIPCC__) || defined(__HIP_DEVICE_COMPILE__)
+#define CRAW_DM_FN __host__ __device__ static inline
+#else
+#define CRAW_DM_FN static inline
+#endif
+
+CRAW_DM_FN uint64_t craw_dm_bits(double x)
+{
+    uint64_t u;
+    __builtin_memcpy(&u, &x, sizeof(u));
+    return u;
+}
+
+CRAW_DM_FN double craw_dm_from_bits(uint64_t u)
+{
+    double x;
+    __builtin_memcpy(&x, &u, sizeof(x));
+    return x;
+}
+
+/* exp(x) in double, for |x| <= 700. */
+CRAW_DM_FN double craw_dm_exp_d(double x)
+{
+    const double inv_ln2 = 0x1.71547652b82fep+0;
+    const double ln2_hi  = 0x1.62e42fee00000p-1;
+    const double ln2_lo  = 0x1.a39ef35793c76p-33;
+    double k = __builtin_rint(x * inv_ln2);
+    double r = __builtin_fma(-k, ln2_hi, x);
+    double p;
+    int64_t ki;
+
+    r = __builtin_fma(-k, ln2_lo, r);
+
+    /* sum_{n=0}^{12} r^n / n!, coefficients rounded to double */
+    p = 0x1.1eed8eff8d898p-29;
+    p = __builtin_fma(p, r, 0x1.ae64567f544e4p-26);
+    p = __builtin_fma(p, r, 0x1.27e4fb7789f5cp-22);
+    p = __builtin_fma(p, r, 0x1.71de3a556c734p-19);
+    p = __builtin_fma(p, r, 0x1.a01a01a01a01ap-16);
+    p = __builtin_fma(p, r, 0x1.a01a01a01a01ap-13);
+    p = __builtin_fma(p, r, 0x1.6c16c16c16c17p-10);
+    p = __builtin_fma(p, r, 0x1.1111111111111p-7);
+    p = __builtin_fma(p, r, 0x1.5555555555555p-5);
+    p = __builtin_fma(p, r, 0x1.5555555555555p-3);
+    p = __builtin_fma(p, r, 0x1.0000000000000p-1);
+    p = __builtin_fma(p, r, 1.0);
+    p = __builtin_fma(p, r, 1.0);
+
+    /* p in [0.70, 1.42]; 2^k is a normal double for |k| <= 1010. */
+    ki = (int64_t)k;
+    return p * craw_dm_from_bits((uint64_t)(ki + 1023) << 52);
+}
+
+/* log(x) in double, for finite x > 0 (double normal). */
+CRAW_DM_FN double craw_dm_log_d(double x)
+{
+    const double ln2_hi = 0x1.62e42fee00000p-1;
+    const double ln2_lo = 0x1.a39ef35793c76p-33;
+    uint64_t u = craw_dm_bits(x);
+    int64_t e = (int64_t)((u >> 52) & 0x7ff) - 1023;
+    double m = craw_dm_from_bits((u & 0x000fffffffffffffull) | 0x3ff0000000000000ull);
+    double f, s, z, rr, s2, logm, ed;
+
+    if (m > 0x1.6a09e667f3bcdp+0) {
+        m = m * 0.5;
+        e = e + 1;
+    }
+
+    f = m - 1.0;                  /* exact: m is within a factor 2 of 1 */
+    s = f / (2.0 + f);
+    z = s * s;
+
+    /* sum_{n>=1} z^n / (2n + 1) up to z^9 */
+    rr = 0x1.af286bca1af28p-5;
+    rr = __builtin_fma(rr, z, 0x1.e1e1e1e1e1e1ep-5);
+    rr = __builtin_fma(rr, z, 0x1.1111111111111p-4);
+    rr = __builtin_fma(rr, z, 0x1.3b13b13b13b14p-4);
+    rr = __builtin_fma(rr, z, 0x1.745d1745d1746p-4);
+    rr = __builtin_fma(rr, z, 0x1.c71c71c71c71cp-4);
+    rr = __builtin_fma(rr, z, 0x1.2492492492492p-3);
+    rr = __builtin_fma(rr, z, 0x1.999999999999ap-3);
+    rr = __builtin_fma(rr, z, 0x1.5555555555555p-2);
+    rr = rr * z;
+
+    s2 = s + s;
+    logm = __builtin_fma(s2, rr, s2);
+
+    ed = (double)e;
+    return ed * ln2_hi + __builtin_fma(ed, ln2_lo, logm);
+}
+
+/* expf with the reference's semantics: NaN in, NaN out; overflow to
+ * +inf and underflow through the subnormals to +0 follow from the single
+ * rounding of the double result. */
+CRAW_DM_FN float craw_dm_expf(float x)
+{
+    double xd;
+
+    if (x != x)
+        return x + x;
+
+    xd = (double)x;
+    xd = xd < -200.0 ? -200.0 : xd;
+    xd = xd > 200.0 ? 200.0 : xd;
+
+    return (float)craw_dm_exp_d(xd);
+}
+
+/* logf: NaN for NaN and negatives, -inf at zero, +inf at +inf. Float
+ * subnormals are normal doubles, so they need no special path. */
+CRAW_DM_FN float craw_dm_logf(float x)
+{
+    if (x != x)
+        return x + x;
+    if (x < 0.0f)
+        return __builtin_nanf("");
+    if (x == 0.0f)
+        return -__builtin_inff();
+    if (x == __builtin_inff())
+        return x;
+
+    return (float)craw_dm_log_d((double)x);
+}
+
+/* powf for x >= 0 (the only bases the path raises): 0^y = 0 and inf^y =
+ * inf for y > 0, 1^y = 1, NaN for a negative or NaN base. */
+CRAW_DM_FN float craw_dm_powf(float x, float y)
+{
+    double t;
+
+    if (x != x || y != y)
+        return x + y;
+    if (x < 0.0f)
+        return __builtin_nanf("");
+    if (x == 1.0f || y == 0.0f)
+        return 1.0f;
+    if (x == 0.0f)
+        return y > 0.0f ? 0.0f : __builtin_inff();
+    if (x == __builtin_inff())
+        return y > 0.0f ? x : 0.0f;
+
+    t = (double)y * craw_dm_log_d((double)x);
+    t = t < -700.0 ? -700.0 : t;
+    t = t > 700.0 ? 700.0 : t;
+
+    return (float)craw_dm_exp_d(t);
+}
+
+#endif /* CRAW_DETMATH_H */

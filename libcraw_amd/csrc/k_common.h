@@ -1,0 +1,157 @@
+/*
+ * libcraw for MI355X: device-side helpers and the argument blocks the
+ * kernels are launched with.
+ *
+ * Every kernel in this library that produces path values is built without
+ * floating-point contraction and with correctly rounded float division
+ * and square root, and evaluates each expression in the order
+ * cyanreg/libcraw's CPU reference (lib/ref) writes it, so that its float
+ * results are the CPU reference's bit for bit (transcendentals through
+ * detmath.h).
+ */
+#ifndef CRAW_K_COMMON_H
+#define CRAW_K_COMMON_H
+
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+#include <stdint.h>
+
+#include "detmath.h"
+#include "hostmath.h"
+
+#define KDEV __device__ __forceinline__
+
+/* Mirror without repeating the edge sample (lib/ref/ref_debayer.c:20-33,
+ * lib/ref/ref_denoise.c:63-76): as many reflections as it takes, 0 for a
+ * dimension below 2. */
+KDEV int kmirror(int v, int n)
+{
+    if (n < 2)
+        return 0;
+    while (v < 0 || v >= n) {
+        if (v < 0)
+            v = -v;
+        if (v >= n)
+            v = 2 * n - 2 - v;
+    }
+    return v;
+}
+
+/* Per-site value out of four, by selects rather than an indexed load
+ * (the site varies per lane; an indexed kernel argument would go through
+ * scratch memory). */
+KDEV float ksel4(int s, float a0, float a1, float a2, float a3)
+{
+    float lo = (s & 1) ? a1 : a0;
+    float hi = (s & 1) ? a3 : a2;
+    return (s & 2) ? hi : lo;
+}
+
+/* Black/white normalisation, white balance and clip of one sample,
+ * DESIGN 4.1 as lib/ref/ref_debayer.c:42-63 computes it: divide by the
+ * range (white - black, formed on the host in float, exactly as the
+ * reference forms it), multiply by the channel's gain, clip at 1.0;
+ * below black is kept. */
+struct KPrep {
+    float black[4];
+    float range[4];
+    float wb[4];      /* per quad site: R, G, G, B gains */
+};
+
+KDEV float kprep(uint32_t raw, int s, const KPrep &p)
+{
+    float r = (float)raw * (1.0f / 65535.0f);
+    float t = (r - ksel4(s, p.black[0], p.black[1], p.black[2], p.black[3])) /
+              ksel4(s, p.range[0], p.range[1], p.range[2], p.range[3]);
+    t *= ksel4(s, p.wb[0], p.wb[1], p.wb[2], p.wb[3]);
+    return t < 1.0f ? t : 1.0f;
+}
+
+/* The colour stage's constants (lib/ref/ref_color.c:10-76): the frame's
+ * cam_to_xyz, the scale gain * 2^stops, XYZ to the output space, and the
+ * gamut compression. The two products are applied one after the other,
+ * as the reference applies them. */
+#define KCOLOR_CLAMP_NEGATIVE 1u
+#define KCOLOR_GAMUT_COMPRESS 2u
+
+struct KColor {
+    float cam[3][3];
+    float scale;
+    float out[3][3];
+    float thr[3], scl[3], pwr, inv_pwr;
+    uint32_t flags;
+};
+
+/* lib/gamut.c:66-92: ACES 1.3 RGC of one AP1 triple. */
+KDEV void kgamut(const KColor &c, float &r, float &g, float &b)
+{
+    float ach = r > (g > b ? g : b) ? r : (g > b ? g : b);
+    float v[3] = { r, g, b };
+    float mag;
+
+    if (ach == 0.0f)
+        return;
+    mag = fabsf(ach);
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        float d = (ach - v[i]) / mag;
+        float nd, p;
+
+        if (d < c.thr[i])
+            continue;
+        nd = (d - c.thr[i]) / c.scl[i];
+        p = craw_dm_powf(nd, c.pwr);
+        d = c.thr[i] + c.scl[i] * nd / craw_dm_powf(1.0f + p, c.inv_pwr);
+        v[i] = ach - d * mag;
+    }
+    r = v[0];
+    g = v[1];
+    b = v[2];
+}
+
+KDEV void kcolor(const KColor &c, float cr, float cg, float cb, float o[3])
+{
+    float x[3];
+
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+        x[r] = (c.cam[r][0] * cr + c.cam[r][1] * cg + c.cam[r][2] * cb) * c.scale;
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+        o[r] = c.out[r][0] * x[0] + c.out[r][1] * x[1] + c.out[r][2] * x[2];
+    if (c.flags & KCOLOR_GAMUT_COMPRESS)
+        kgamut(c, o[0], o[1], o[2]);
+    if (c.flags & KCOLOR_CLAMP_NEGATIVE) {
+#pragma unroll
+        for (int r = 0; r < 3; r++)
+            o[r] = o[r] > 0.0f ? o[r] : 0.0f;
+    }
+}
+
+/* Where a tile's finished camera RGB goes. */
+struct KOut {
+    void *dst;           /* RGBA16F or RGBA32F */
+    size_t pitch;        /* bytes */
+    int half;            /* 1: RGBA16F */
+    /* AWB histogram (NULL: none): count, weight, bright, each 128^2. */
+    uint32_t *hist;
+    float awb_wb0, awb_wb2;
+};
+
+KDEV void kstore_rgba(const KOut &o, int x, int y, const float v[3])
+{
+    char *row = (char *)o.dst + (size_t)y * o.pitch;
+
+    if (o.half) {
+        __half2 lo = __halves2half2(__float2half_rn(v[0]), __float2half_rn(v[1]));
+        __half2 hi = __halves2half2(__float2half_rn(v[2]), __float2half_rn(1.0f));
+        uint2 u;
+        __builtin_memcpy(&u.x, &lo, 4);
+        __builtin_memcpy(&u.y, &hi, 4);
+        ((uint2 *)row)[x] = u;
+    } else {
+        ((float4 *)row)[x] = make_float4(v[0], v[1], v[2], 1.0f);
+    }
+}
+
+#endif /* CRAW_K_COMMON_H */

@@ -1,0 +1,633 @@
+/*
+ * libcraw for MI355X: debayer, chroma denoise, AWB histogram and colour
+ * kernels.
+ *
+ * A workgroup develops one tile of the frame. The prepped mosaic (and, for
+ * RCD, each intermediate of its three passes) is staged in LDS over the
+ * tile plus the apron the later stencils reach, so a sample is read from
+ * HBM once and every stencil tap is an LDS read. Out-of-frame positions of
+ * an apron hold the value at the mirrored position, and every stage
+ * evaluates its stencil at the mirrored position's own neighbourhood:
+ * exactly the reference's "fetch through the mirror" semantics
+ * (lib/ref/ref_debayer.c:20-40) for every intermediate, at any border.
+ *
+ * The finished camera RGB of a tile stays in LDS for the AWB histogram
+ * (one sample per whole 8x8 block, tiles being multiples of 8) and for the
+ * colour stage, so without chroma denoise the whole chain is one kernel:
+ * 2 bytes in, 8 or 16 bytes out per pixel. With chroma denoise the
+ * debayer writes the camera RGB once (three float planes) and the 4x4
+ * block means, the bilateral runs on the quarter grid, and a last kernel
+ * puts the band back, builds the histogram and applies the colour stage.
+ */
+#include "k_common.h"
+#include "kernels.h"
+
+/* ------------------------------------------------------------ finish */
+
+/* AWB histogram of the tile's camera RGB (SoA in LDS) and the colour stage
+ * into the destination. Expects the LDS tile complete (after a barrier).
+ * Blocks are summed row by row as lib/awb.c:249-256 sums them. */
+template <int TW, int TH, int NT>
+KDEV void kfinish(const float (*rgbs)[TH][TW], int x0, int y0, int W, int H,
+                  const KColor &col, const KOut &out)
+{
+    const int tid = threadIdx.x;
+
+    if (out.hist) {
+        constexpr int NBX = TW / 8, NBY = TH / 8;
+
+        for (int b = tid; b < NBX * NBY; b += NT) {
+            int gx0 = x0 + (b % NBX) * 8, gy0 = y0 + (b / NBX) * 8;
+            int gx1, gy1;
+            float acc[3] = { 0.0f, 0.0f, 0.0f };
+            CRAWAwbSample s;
+
+            if (gx0 >= W || gy0 >= H)
+                continue;
+            gx1 = gx0 + 8 < W ? gx0 + 8 : W;
+            gy1 = gy0 + 8 < H ? gy0 + 8 : H;
+            for (int y = gy0; y < gy1; y++) {
+                for (int x = gx0; x < gx1; x++) {
+                    acc[0] += rgbs[0][y - y0][x - x0];
+                    acc[1] += rgbs[1][y - y0][x - x0];
+                    acc[2] += rgbs[2][y - y0][x - x0];
+                }
+            }
+            if (craw_awb_sample(acc, (float)((gy1 - gy0) * (gx1 - gx0)), out.awb_wb0,
+                                out.awb_wb2, &s)) {
+                atomicAdd(&out.hist[s.bin], 1u);
+                atomicAdd(&out.hist[16384 + s.bin], s.weight);
+                atomicAdd(&out.hist[32768 + s.bin], s.bright);
+            }
+        }
+    }
+
+#pragma unroll
+    for (int k = 0; k < TW * TH / NT; k++) {
+        int p = tid + k * NT, px = p % TW, py = p / TW;
+        int x = x0 + px, y = y0 + py;
+        float o[3];
+
+        if (x >= W || y >= H)
+            continue;
+        kcolor(col, rgbs[0][py][px], rgbs[1][py][px], rgbs[2][py][px], o);
+        kstore_rgba(out, x, y, o);
+    }
+}
+
+/* Camera RGB of the tile to the chroma stage's inputs: the three planes
+ * and the 4x4 block means of (R - G, B - G, G), summed row by row over the
+ * pixels inside the frame (lib/ref/ref_chroma.c:59-82). */
+template <int TW, int TH, int NT>
+KDEV void kstore_inter(const float (*rgbs)[TH][TW], int x0, int y0, int W, int H,
+                       const KInter &it)
+{
+    const int tid = threadIdx.x;
+    const size_t plane = (size_t)W * H;
+
+#pragma unroll
+    for (int k = 0; k < TW * TH / NT; k++) {
+        int p = tid + k * NT, px = p % TW, py = p / TW;
+        int x = x0 + px, y = y0 + py;
+
+        if (x >= W || y >= H)
+            continue;
+        it.rgb[(size_t)y * W + x] = rgbs[0][py][px];
+        it.rgb[plane + (size_t)y * W + x] = rgbs[1][py][px];
+        it.rgb[2 * plane + (size_t)y * W + x] = rgbs[2][py][px];
+    }
+
+    for (int b = tid; b < (TW / 4) * (TH / 4); b += NT) {
+        int qx = x0 / 4 + b % (TW / 4), qy = y0 / 4 + b / (TW / 4);
+        float sum[3] = { 0.0f, 0.0f, 0.0f }, n = 0.0f;
+
+        if (4 * qx >= W || 4 * qy >= H)
+            continue;
+        for (int j = 0; j < 4; j++) {
+            for (int i = 0; i < 4; i++) {
+                int x = 4 * qx + i, y = 4 * qy + j;
+                float r, g, bl;
+
+                if (x >= W || y >= H)
+                    continue;
+                r = rgbs[0][y - y0][x - x0];
+                g = rgbs[1][y - y0][x - x0];
+                bl = rgbs[2][y - y0][x - x0];
+                sum[0] += r - g;
+                sum[1] += bl - g;
+                sum[2] += g;
+                n += 1.0f;
+            }
+        }
+        {
+            size_t q = (size_t)qy * it.wq + qx, qplane = (size_t)it.wq * it.hq;
+            it.down[q] = sum[0] / n;
+            it.down[qplane + q] = sum[1] / n;
+            it.down[2 * qplane + q] = sum[2] / n;
+        }
+    }
+}
+
+/* -------------------------------------------------- bilinear and MHC */
+
+#define BL_TW 64
+#define BL_TH 16
+#define BL_A  2
+#define BL_NT 256
+
+template <int METHOD, bool INTER>
+__global__ __launch_bounds__(BL_NT) void k_debayer_simple(const uint16_t *__restrict__ src,
+                                                          int src_pitch, int W, int H,
+                                                          KPrep prep, KColor col, KOut out,
+                                                          KInter inter)
+{
+    constexpr int SW = BL_TW + 2 * BL_A, SH = BL_TH + 2 * BL_A;
+    __shared__ float vs[SH][SW];
+    __shared__ float rgbs[3][BL_TH][BL_TW];
+    const int tid = threadIdx.x;
+    const int x0 = blockIdx.x * BL_TW, y0 = blockIdx.y * BL_TH;
+    const int xe = min(x0 + BL_TW, W), ye = min(y0 + BL_TH, H);
+    const int rw = xe - x0 + 2 * BL_A, rh = ye - y0 + 2 * BL_A;
+
+    for (int i = tid; i < rw * rh; i += BL_NT) {
+        int lx = i % rw, ly = i / rw;
+        int mx = kmirror(x0 - BL_A + lx, W), my = kmirror(y0 - BL_A + ly, H);
+        int s = (my & 1) * 2 + (mx & 1);
+
+        vs[ly][lx] = kprep(src[(size_t)my * src_pitch + mx], s, prep);
+    }
+    __syncthreads();
+
+#define V(dx, dy) vs[py + BL_A + (dy)][px + BL_A + (dx)]
+#pragma unroll
+    for (int k = 0; k < BL_TW * BL_TH / BL_NT; k++) {
+        int p = tid + k * BL_NT, px = p % BL_TW, py = p / BL_TW;
+        int x = x0 + px, y = y0 + py;
+        int s = (y & 1) * 2 + (x & 1);
+        float r, g, b;
+
+        if (x >= W || y >= H)
+            continue;
+        if (METHOD == 0) {
+            /* lib/ref/ref_debayer.c:65-96 */
+            float edge = 0.25f * (V(0, -1) + V(0, 1) + V(-1, 0) + V(1, 0));
+            float corner = 0.25f * (V(-1, -1) + V(1, -1) + V(-1, 1) + V(1, 1));
+            float horiz = 0.5f * (V(-1, 0) + V(1, 0));
+            float vert = 0.5f * (V(0, -1) + V(0, 1));
+
+            if (s == 0)      { r = V(0, 0); g = edge;    b = corner; }
+            else if (s == 1) { r = horiz;   g = V(0, 0); b = vert; }
+            else if (s == 2) { r = vert;    g = V(0, 0); b = horiz; }
+            else             { r = corner;  g = edge;    b = V(0, 0); }
+        } else {
+            /* lib/ref/ref_debayer.c:98-176, the four MHC kernels over 8 */
+            float c = V(0, 0);
+            float green = (4.0f * c + 2.0f * (V(0, -1) + V(0, 1) + V(-1, 0) + V(1, 0))
+                           - (V(0, -2) + V(0, 2) + V(-2, 0) + V(2, 0))) * 0.125f;
+            float diag = (6.0f * c + 2.0f * (V(-1, -1) + V(1, -1) + V(-1, 1) + V(1, 1))
+                          - 1.5f * (V(0, -2) + V(0, 2) + V(-2, 0) + V(2, 0))) * 0.125f;
+            float ch = (5.0f * c + 4.0f * (V(-1, 0) + V(1, 0))
+                        - (V(-1, -1) + V(1, -1) + V(-1, 1) + V(1, 1) + V(-2, 0) + V(2, 0))
+                        + 0.5f * (V(0, -2) + V(0, 2))) * 0.125f;
+            float cv = (5.0f * c + 4.0f * (V(0, -1) + V(0, 1))
+                        - (V(-1, -1) + V(1, -1) + V(-1, 1) + V(1, 1) + V(0, -2) + V(0, 2))
+                        + 0.5f * (V(-2, 0) + V(2, 0))) * 0.125f;
+
+            if (s == 0)      { r = c;    g = green; b = diag; }
+            else if (s == 1) { r = ch;   g = c;     b = cv; }
+            else if (s == 2) { r = cv;   g = c;     b = ch; }
+            else             { r = diag; g = green; b = c; }
+        }
+        rgbs[0][py][px] = r;
+        rgbs[1][py][px] = g;
+        rgbs[2][py][px] = b;
+    }
+#undef V
+    __syncthreads();
+
+    if (INTER)
+        kstore_inter<BL_TW, BL_TH, BL_NT>(rgbs, x0, y0, W, H, inter);
+    else
+        kfinish<BL_TW, BL_TH, BL_NT>(rgbs, x0, y0, W, H, col, out);
+}
+
+/* --------------------------------------------------------------- RCD */
+
+#define RC_TW 32
+#define RC_TH 16
+#define RC_NT 256
+#define RCD_EPS 1e-5f
+
+/* One LDS array of a tile stage with apron A: indexed by frame
+ * coordinates relative to the tile origin. */
+template <int A>
+struct RcStage {
+    static constexpr int W = RC_TW + 2 * A, H = RC_TH + 2 * A, N = W * H;
+};
+
+/* lib/ref/ref_debayer.c:195-202 */
+KDEV float rcd_ratio(float a, float b)
+{
+    a = a > 0.0f ? a : 0.0f;
+    b = b > 0.0f ? b : 0.0f;
+    return (a - b) / (a + b + RCD_EPS);
+}
+
+/* DESIGN 5.3 / lib/ref/ref_debayer.c:204-395. Stages and the apron each
+ * is needed over: V 5, low pass 4, gradients 3, wv 2, wp 1, green 2,
+ * colour estimate 1, output 0. Every stage computes at the mirrored
+ * position of each apron position, from the previous stage's values at
+ * that position's own (virtual) neighbourhood. */
+template <bool INTER>
+__global__ __launch_bounds__(RC_NT) void k_debayer_rcd(const uint16_t *__restrict__ src,
+                                                       int src_pitch, int W, int H, KPrep prep,
+                                                       KColor col, KOut out, KInter inter)
+{
+    typedef RcStage<5> S5;
+    typedef RcStage<4> S4;
+    typedef RcStage<3> S3;
+    typedef RcStage<2> S2;
+    typedef RcStage<1> S1;
+    __shared__ float sv[S5::N];
+    __shared__ float slp[S4::N];
+    /* gradients: vertical, horizontal, NE-SW, NW-SE; reused as the RGB
+     * tile once the weights are formed */
+    __shared__ float sg[4][S3::N];
+    __shared__ float swv[S2::N];
+    __shared__ float swp[S1::N];
+    __shared__ float sgp[S2::N];
+    __shared__ float scp[S1::N];
+    const int tid = threadIdx.x;
+    const int x0 = blockIdx.x * RC_TW, y0 = blockIdx.y * RC_TH;
+    const int xe = min(x0 + RC_TW, W), ye = min(y0 + RC_TH, H);
+    const int tw = xe - x0, th = ye - y0;
+
+    /* Position (X, Y) in frame coordinates of a stage with apron A. */
+#define AT(arr, A, X, Y) arr[((Y) - y0 + (A)) * RcStage<A>::W + ((X) - x0 + (A))]
+#define V(dx, dy) AT(sv, 5, mx + (dx), my + (dy))
+#define L(dx, dy) AT(slp, 4, mx + (dx), my + (dy))
+#define G(dx, dy) AT(sgp, 2, mx + (dx), my + (dy))
+#define C(dx, dy) AT(scp, 1, mx + (dx), my + (dy))
+
+    /* V: prep, apron 5 */
+    {
+        const int rw = tw + 10, rh = th + 10;
+        for (int i = tid; i < rw * rh; i += RC_NT) {
+            int lx = i % rw, ly = i / rw;
+            int mx = kmirror(x0 - 5 + lx, W), my = kmirror(y0 - 5 + ly, H);
+            int s = (my & 1) * 2 + (mx & 1);
+            sv[ly * S5::W + lx] = kprep(src[(size_t)my * src_pitch + mx], s, prep);
+        }
+    }
+    __syncthreads();
+
+    /* Pass A: low pass (apron 4) and the four gradients (apron 3). */
+    {
+        const int rw = tw + 8, rh = th + 8;
+        for (int i = tid; i < rw * rh; i += RC_NT) {
+            int lx = i % rw, ly = i / rw;
+            int vx = x0 - 4 + lx, vy = y0 - 4 + ly;
+            int mx = kmirror(vx, W), my = kmirror(vy, H);
+
+            slp[ly * S4::W + lx] =
+                (4.0f * V(0, 0)
+                 + 2.0f * (V(0, -1) + V(0, 1) + V(-1, 0) + V(1, 0))
+                 + V(-1, -1) + V(1, -1) + V(-1, 1) + V(1, 1)) * (1.0f / 16.0f);
+            if (lx >= 1 && ly >= 1 && lx < rw - 1 && ly < rh - 1) {
+                int j = (ly - 1) * S3::W + (lx - 1);
+                sg[0][j] = fabsf(V(0, -1) - V(0, 1))
+                         + fabsf(2.0f * V(0, 0) - V(0, -2) - V(0, 2));
+                sg[1][j] = fabsf(V(-1, 0) - V(1, 0))
+                         + fabsf(2.0f * V(0, 0) - V(-2, 0) - V(2, 0));
+                sg[2][j] = fabsf(V(-1, 1) - V(1, -1))
+                         + fabsf(2.0f * V(0, 0) - V(-2, 2) - V(2, -2));
+                sg[3][j] = fabsf(V(-1, -1) - V(1, 1))
+                         + fabsf(2.0f * V(0, 0) - V(-2, -2) - V(2, 2));
+            }
+        }
+    }
+    __syncthreads();
+
+    /* Pass A: direction weights, wv over apron 2 and wp over apron 1, from
+     * 3x3 means of the gradients summed row by row
+     * (lib/ref/ref_debayer.c:179-190). */
+    {
+        const int rw = tw + 4, rh = th + 4;
+        for (int i = tid; i < rw * rh; i += RC_NT) {
+            int lx = i % rw, ly = i / rw;
+            int vx = x0 - 2 + lx, vy = y0 - 2 + ly;
+            int mx = kmirror(vx, W), my = kmirror(vy, H);
+            float a0 = 0.0f, a1 = 0.0f;
+
+            for (int dy = -1; dy <= 1; dy++) {
+                for (int dx = -1; dx <= 1; dx++) {
+                    a0 += AT(sg[0], 3, mx + dx, my + dy);
+                    a1 += AT(sg[1], 3, mx + dx, my + dy);
+                }
+            }
+            {
+                float vsum = a0 * (1.0f / 9.0f), hsum = a1 * (1.0f / 9.0f);
+                swv[ly * S2::W + lx] = hsum / (hsum + vsum + RCD_EPS);
+            }
+            if (lx >= 1 && ly >= 1 && lx < rw - 1 && ly < rh - 1) {
+                float b0 = 0.0f, b1 = 0.0f;
+
+                for (int dy = -1; dy <= 1; dy++) {
+                    for (int dx = -1; dx <= 1; dx++) {
+                        b0 += AT(sg[2], 3, mx + dx, my + dy);
+                        b1 += AT(sg[3], 3, mx + dx, my + dy);
+                    }
+                }
+                {
+                    float ps = b0 * (1.0f / 9.0f), qs = b1 * (1.0f / 9.0f);
+                    swp[(ly - 1) * S1::W + (lx - 1)] = qs / (ps + qs + RCD_EPS);
+                }
+            }
+        }
+    }
+    __syncthreads();
+
+    /* Pass B: green, apron 2. */
+    {
+        const int rw = tw + 4, rh = th + 4;
+        for (int i = tid; i < rw * rh; i += RC_NT) {
+            int lx = i % rw, ly = i / rw;
+            int mx = kmirror(x0 - 2 + lx, W), my = kmirror(y0 - 2 + ly, H);
+            int s = (my & 1) * 2 + (mx & 1);
+            float gval;
+
+            if (s == 1 || s == 2) {
+                gval = V(0, 0);
+            } else {
+                float l00 = L(0, 0), lu = L(0, -2), ld = L(0, 2), ll = L(-2, 0), lr = L(2, 0);
+                float gv = 0.5f * (V(0, -1) * (1.0f + rcd_ratio(l00, lu))
+                                 + V(0, 1) * (1.0f + rcd_ratio(l00, ld)));
+                float gh = 0.5f * (V(-1, 0) * (1.0f + rcd_ratio(l00, ll))
+                                 + V(1, 0) * (1.0f + rcd_ratio(l00, lr)));
+                float w = AT(swv, 2, mx, my);
+                gval = w * gv + (1.0f - w) * gh;
+            }
+            sgp[ly * S2::W + lx] = gval;
+        }
+    }
+    __syncthreads();
+
+    /* Pass C, first half: colour estimate at the red and blue sites,
+     * apron 1. */
+    {
+        const int rw = tw + 2, rh = th + 2;
+        for (int i = tid; i < rw * rh; i += RC_NT) {
+            int lx = i % rw, ly = i / rw;
+            int mx = kmirror(x0 - 1 + lx, W), my = kmirror(y0 - 1 + ly, H);
+            int s = (my & 1) * 2 + (mx & 1);
+            float c = 0.0f;
+
+            if (s == 0 || s == 3) {
+                float g00 = G(0, 0);
+                float dsw = V(-1, 1) - G(-1, 1);
+                float dne = V(1, -1) - G(1, -1);
+                float dnw = V(-1, -1) - G(-1, -1);
+                float dse = V(1, 1) - G(1, 1);
+                float w = AT(swp, 1, mx, my);
+
+                c = w * (g00 + 0.5f * (dsw + dne)) + (1.0f - w) * (g00 + 0.5f * (dnw + dse));
+            }
+            scp[ly * S1::W + lx] = c;
+        }
+    }
+    __syncthreads();
+
+    /* Output: RGB of the tile's own pixels into the gradient space. */
+    {
+        float (*rgbs)[RC_TH][RC_TW] = (float (*)[RC_TH][RC_TW])&sg[0][0];
+
+#pragma unroll
+        for (int k = 0; k < RC_TW * RC_TH / RC_NT; k++) {
+            int p = tid + k * RC_NT, px = p % RC_TW, py = p / RC_TW;
+            int mx = x0 + px, my = y0 + py;
+            int s = (my & 1) * 2 + (mx & 1);
+            float r, g, b;
+
+            if (mx >= W || my >= H)
+                continue;
+            if (s == 0 || s == 3) {
+                float c = C(0, 0);
+                g = G(0, 0);
+                r = s == 0 ? V(0, 0) : c;
+                b = s == 0 ? c : V(0, 0);
+            } else {
+                /* Pass C, second half (lib/ref/ref_debayer.c:350-380) */
+                float g00 = G(0, 0), w = AT(swv, 2, mx, my);
+                float dh = 0.5f * ((V(-1, 0) - G(-1, 0)) + (V(1, 0) - G(1, 0)));
+                float ev = 0.5f * ((C(0, -1) - G(0, -1)) + (C(0, 1) - G(0, 1)));
+                float ch = w * (g00 + ev) + (1.0f - w) * (g00 + dh);
+                float dv = 0.5f * ((V(0, -1) - G(0, -1)) + (V(0, 1) - G(0, 1)));
+                float eh = 0.5f * ((C(-1, 0) - G(-1, 0)) + (C(1, 0) - G(1, 0)));
+                float cv = w * (g00 + dv) + (1.0f - w) * (g00 + eh);
+
+                r = s == 1 ? ch : cv;
+                g = V(0, 0);
+                b = s == 1 ? cv : ch;
+            }
+            rgbs[0][py][px] = r;
+            rgbs[1][py][px] = g;
+            rgbs[2][py][px] = b;
+        }
+        __syncthreads();
+
+        if (INTER)
+            kstore_inter<RC_TW, RC_TH, RC_NT>(rgbs, x0, y0, W, H, inter);
+        else
+            kfinish<RC_TW, RC_TH, RC_NT>(rgbs, x0, y0, W, H, col, out);
+    }
+#undef AT
+#undef V
+#undef L
+#undef G
+#undef C
+}
+
+/* ------------------------------------------------------------ chroma */
+
+#define CB_TX 16
+#define CB_TY 16
+
+/* lib/ref/ref_chroma.c:86-117: 9x9 joint bilateral on the quarter grid,
+ * guided by G, one reflection at the borders, summed row by row. */
+__global__ __launch_bounds__(CB_TX * CB_TY) void k_chroma_blur(KInter it)
+{
+    const int qx = blockIdx.x * CB_TX + threadIdx.x % CB_TX;
+    const int qy = blockIdx.y * CB_TY + threadIdx.x / CB_TX;
+    const size_t qplane = (size_t)it.wq * it.hq;
+    const float *d0 = it.down, *d1 = it.down + qplane, *d2 = it.down + 2 * qplane;
+    float c0, c1, c2, level, inv_range, inv_chroma;
+    float acc0 = 0.0f, acc1 = 0.0f, wsum = 0.0f;
+
+    if (qx >= it.wq || qy >= it.hq)
+        return;
+    c0 = d0[(size_t)qy * it.wq + qx];
+    c1 = d1[(size_t)qy * it.wq + qx];
+    c2 = d2[(size_t)qy * it.wq + qx];
+    level = (c2 > 0.0f ? c2 : 0.0f) + 0.02f;
+    inv_range = 1.0f / (0.35f * level);
+    inv_chroma = 1.0f / (0.35f * level);
+
+    for (int dy = -4; dy <= 4; dy++) {
+        int sy = qy + dy;
+        sy = sy < 0 ? -sy : (sy >= it.hq ? 2 * it.hq - 2 - sy : sy);
+        for (int dx = -4; dx <= 4; dx++) {
+            int sx = qx + dx;
+            size_t si;
+            float s0, s1, s2, dg, dd0, dd1, wgt;
+
+            sx = sx < 0 ? -sx : (sx >= it.wq ? 2 * it.wq - 2 - sx : sx);
+            si = (size_t)sy * it.wq + sx;
+            s0 = d0[si];
+            s1 = d1[si];
+            s2 = d2[si];
+            dg = (s2 - c2) * inv_range;
+            dd0 = (s0 - c0) * inv_chroma;
+            dd1 = (s1 - c1) * inv_chroma;
+            wgt = craw_dm_expf(-(float)(dx * dx + dy * dy) * 0.125f - dg * dg - dd0 * dd0
+                               - dd1 * dd1);
+            acc0 += wgt * s0;
+            acc1 += wgt * s1;
+            wsum += wgt;
+        }
+    }
+    it.blur[(size_t)qy * it.wq + qx] = acc0 / wsum;
+    it.blur[qplane + (size_t)qy * it.wq + qx] = acc1 / wsum;
+}
+
+#define CU_TW 64
+#define CU_TH 16
+#define CU_NT 256
+
+/* lib/ref/ref_chroma.c:119-155: the difference of the filtered and the
+ * unfiltered band, bilinearly up to every pixel and blended by the
+ * strength, into R and B; then the histogram and the colour stage. */
+__global__ __launch_bounds__(CU_NT) void k_chroma_up_finish(KInter it, float strength, int W,
+                                                            int H, KColor col, KOut out)
+{
+    __shared__ float rgbs[3][CU_TH][CU_TW];
+    const int tid = threadIdx.x;
+    const int x0 = blockIdx.x * CU_TW, y0 = blockIdx.y * CU_TH;
+    const size_t plane = (size_t)W * H, qplane = (size_t)it.wq * it.hq;
+
+#pragma unroll
+    for (int k = 0; k < CU_TW * CU_TH / CU_NT; k++) {
+        int p = tid + k * CU_NT, px = p % CU_TW, py = p / CU_TW;
+        int x = x0 + px, y = y0 + py;
+        float fy, fly, ty, fx, flx, tx, w00, w10, w01, w11, r, g, b, d[2];
+        int qy0, qy1, qx0, qx1;
+        size_t i00, i10, i01, i11;
+
+        if (x >= W || y >= H)
+            continue;
+        r = it.rgb[(size_t)y * W + x];
+        g = it.rgb[plane + (size_t)y * W + x];
+        b = it.rgb[2 * plane + (size_t)y * W + x];
+
+        fy = ((float)y + 0.5f) * 0.25f - 0.5f;
+        fly = floorf(fy);
+        ty = fy - fly;
+        qy0 = (int)fly;
+        qy0 = qy0 > 0 ? (qy0 < it.hq - 1 ? qy0 : it.hq - 1) : 0;
+        qy1 = (int)fly + 1;
+        qy1 = qy1 > 0 ? (qy1 < it.hq - 1 ? qy1 : it.hq - 1) : 0;
+        fx = ((float)x + 0.5f) * 0.25f - 0.5f;
+        flx = floorf(fx);
+        tx = fx - flx;
+        qx0 = (int)flx;
+        qx0 = qx0 > 0 ? (qx0 < it.wq - 1 ? qx0 : it.wq - 1) : 0;
+        qx1 = (int)flx + 1;
+        qx1 = qx1 > 0 ? (qx1 < it.wq - 1 ? qx1 : it.wq - 1) : 0;
+        w00 = (1.0f - tx) * (1.0f - ty);
+        w10 = tx * (1.0f - ty);
+        w01 = (1.0f - tx) * ty;
+        w11 = tx * ty;
+        i00 = (size_t)qy0 * it.wq + qx0;
+        i10 = (size_t)qy0 * it.wq + qx1;
+        i01 = (size_t)qy1 * it.wq + qx0;
+        i11 = (size_t)qy1 * it.wq + qx1;
+
+        d[0] = r - g;
+        d[1] = b - g;
+#pragma unroll
+        for (int c = 0; c < 2; c++) {
+            const float *dn = it.down + c * qplane, *bl = it.blur + c * qplane;
+            float lo = w00 * dn[i00] + w10 * dn[i10] + w01 * dn[i01] + w11 * dn[i11];
+            float lf = w00 * bl[i00] + w10 * bl[i10] + w01 * bl[i01] + w11 * bl[i11];
+            d[c] += strength * (lf - lo);
+        }
+        rgbs[0][py][px] = g + d[0];
+        rgbs[1][py][px] = g;
+        rgbs[2][py][px] = g + d[1];
+    }
+    __syncthreads();
+
+    kfinish<CU_TW, CU_TH, CU_NT>(rgbs, x0, y0, W, H, col, out);
+}
+
+/* ------------------------------------------------------------- misc */
+
+__global__ void k_zero_u32(uint32_t *p, size_t n)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        p[i] = 0u;
+}
+
+/* ---------------------------------------------------------- launchers */
+
+hipError_t craw_launch_zero_u32(uint32_t *p, size_t n, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_zero_u32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n);
+    return hipGetLastError();
+}
+
+hipError_t craw_launch_debayer(int method, bool to_inter, const uint16_t *src, int src_pitch,
+                               int W, int H, const KPrep &prep, const KColor &col,
+                               const KOut &out, const KInter &inter, hipStream_t s)
+{
+    if (method == 2) {
+        dim3 grid((W + RC_TW - 1) / RC_TW, (H + RC_TH - 1) / RC_TH);
+        if (to_inter)
+            hipLaunchKernelGGL(k_debayer_rcd<true>, grid, dim3(RC_NT), 0, s, src, src_pitch, W,
+                               H, prep, col, out, inter);
+        else
+            hipLaunchKernelGGL(k_debayer_rcd<false>, grid, dim3(RC_NT), 0, s, src, src_pitch, W,
+                               H, prep, col, out, inter);
+    } else {
+        dim3 grid((W + BL_TW - 1) / BL_TW, (H + BL_TH - 1) / BL_TH);
+        if (method == 0 && to_inter)
+            hipLaunchKernelGGL((k_debayer_simple<0, true>), grid, dim3(BL_NT), 0, s, src,
+                               src_pitch, W, H, prep, col, out, inter);
+        else if (method == 0)
+            hipLaunchKernelGGL((k_debayer_simple<0, false>), grid, dim3(BL_NT), 0, s, src,
+                               src_pitch, W, H, prep, col, out, inter);
+        else if (to_inter)
+            hipLaunchKernelGGL((k_debayer_simple<1, true>), grid, dim3(BL_NT), 0, s, src,
+                               src_pitch, W, H, prep, col, out, inter);
+        else
+            hipLaunchKernelGGL((k_debayer_simple<1, false>), grid, dim3(BL_NT), 0, s, src,
+                               src_pitch, W, H, prep, col, out, inter);
+    }
+    return hipGetLastError();
+}
+
+hipError_t craw_launch_chroma_blur(const KInter &it, hipStream_t s)
+{
+    dim3 grid((it.wq + CB_TX - 1) / CB_TX, (it.hq + CB_TY - 1) / CB_TY);
+    hipLaunchKernelGGL(k_chroma_blur, grid, dim3(CB_TX * CB_TY), 0, s, it);
+    return hipGetLastError();
+}
+
+hipError_t craw_launch_chroma_up_finish(const KInter &it, float strength, int W, int H,
+                                        const KColor &col, const KOut &out, hipStream_t s)
+{
+    dim3 grid((W + CU_TW - 1) / CU_TW, (H + CU_TH - 1) / CU_TH);
+    hipLaunchKernelGGL(k_chroma_up_finish, grid, dim3(CU_NT), 0, s, it, strength, W, H, col,
+                       out);
+    return hipGetLastError();
+}

@@ -1,0 +1,1100 @@
+/*
+ * The oracle's path arithmetic: lib/ref of cyanreg/libcraw restated, one
+ * source for three modes (see oracle.h). TEST INFRASTRUCTURE.
+ *
+ * Build with ORACLE_MODE 1 (float, glibc), 2 (float, detmath.h) or 3
+ * (double), always with -ffp-contract=off: every expression below is
+ * evaluated as written, left to right, the way lib/ref's C is, so that the
+ * float modes round exactly where the reference rounds. Constants are
+ * given as a float literal and a double literal (K(f, d)) so that the
+ * float modes use the reference's own float constants.
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "oracle.h"
+#include "detmath.h"
+
+#if ORACLE_MODE == 1
+typedef float real;
+#define PFX(n) oracle_f32l_##n
+#define EXPF(x) expf(x)
+#define LOGF(x) logf(x)
+#define POWF(x, y) powf(x, y)
+#define SQRTF(x) sqrtf(x)
+#define RINTF(x) rintf(x)
+#define FLOORF(x) floorf(x)
+#define ROUNDF(x) roundf(x)
+#define FABSF(x) fabsf(x)
+#define K(f, d) (f)
+#elif ORACLE_MODE == 2
+typedef float real;
+#define PFX(n) oracle_f32d_##n
+#define EXPF(x) craw_dm_expf(x)
+#define LOGF(x) craw_dm_logf(x)
+#define POWF(x, y) craw_dm_powf(x, y)
+#define SQRTF(x) sqrtf(x)
+#define RINTF(x) rintf(x)
+#define FLOORF(x) floorf(x)
+#define ROUNDF(x) roundf(x)
+#define FABSF(x) fabsf(x)
+#define K(f, d) (f)
+#elif ORACLE_MODE == 3
+typedef double real;
+#define PFX(n) oracle_f64_##n
+#define EXPF(x) exp(x)
+#define LOGF(x) log(x)
+#define POWF(x, y) pow(x, y)
+#define SQRTF(x) sqrt(x)
+#define RINTF(x) rint(x)
+#define FLOORF(x) floor(x)
+#define ROUNDF(x) round(x)
+#define FABSF(x) fabs(x)
+#define K(f, d) (d)
+#else
+#error "ORACLE_MODE must be 1, 2 or 3"
+#endif
+
+/* The reference's ternary min/max/clamp (utils.h:41-43), NaN behaviour
+ * included. */
+#define OMIN(a, b) ((a) < (b) ? (a) : (b))
+#define OMAX(a, b) ((a) > (b) ? (a) : (b))
+#define OCLAMP(v, lo, hi) OMIN(OMAX((v), (lo)), (hi))
+
+static int o_threads = 1;
+
+void PFX(set_threads)(int n)
+{
+    o_threads = n > 0 ? n : 1;
+}
+
+#ifdef _OPENMP
+#define OMP_FOR _Pragma("omp parallel for schedule(static) num_threads(o_threads)")
+#else
+#define OMP_FOR
+#endif
+
+/* RGGB site to channel (ref_debayer.c:14). */
+static const int o_site_chan[4] = { 0, 1, 1, 2 };
+
+/* ref_debayer.c:20-33: mirror without repeating the edge sample. */
+static inline int o_mirror(int v, int n)
+{
+    if (n < 2)
+        return 0;
+    while (v < 0 || v >= n) {
+        if (v < 0)
+            v = -v;
+        if (v >= n)
+            v = 2 * n - 2 - v;
+    }
+    return v;
+}
+
+static inline real o_fetch(const real *p, uint32_t w, uint32_t h, int x, int y)
+{
+    return p[(size_t)o_mirror(y, (int)h) * w + o_mirror(x, (int)w)];
+}
+
+/* ref_debayer.c:42-63, DESIGN.md 4.1: black, range, gain, clip at 1.0;
+ * below black is kept. */
+void PFX(prep)(const uint16_t *bayer, uint32_t width, uint32_t height, size_t stride,
+               const CRAWProfile *p, const float wb[3], real *v)
+{
+    OMP_FOR
+    for (uint32_t y = 0; y < height; y++) {
+        for (uint32_t x = 0; x < width; x++) {
+            int s = (int)((y & 1) * 2 + (x & 1));
+            real raw = (real)bayer[(size_t)y * stride + x] * K(1.0f / 65535.0f, 1.0 / 65535.0);
+            real t = (raw - (real)p->black_level[s]) /
+                     ((real)p->white_level[s] - (real)p->black_level[s]);
+
+            t *= (real)wb[o_site_chan[s]];
+            v[(size_t)y * width + x] = OMIN(t, K(1.0f, 1.0));
+        }
+    }
+}
+
+#define V(dx, dy) o_fetch(v, width, height, x + (dx), y + (dy))
+
+/* ref_debayer.c:65-96, DESIGN.md 5.1. */
+static void o_bilinear(const real *v, uint32_t width, uint32_t height, real *rgb)
+{
+    OMP_FOR
+    for (int y = 0; y < (int)height; y++) {
+        for (int x = 0; x < (int)width; x++) {
+            real *o = &rgb[((size_t)y * width + x) * 4];
+            int s = (y & 1) * 2 + (x & 1);
+            real edge = K(0.25f, 0.25) * (V(0, -1) + V(0, 1) + V(-1, 0) + V(1, 0));
+            real corner = K(0.25f, 0.25) * (V(-1, -1) + V(1, -1) + V(-1, 1) + V(1, 1));
+            real horiz = K(0.5f, 0.5) * (V(-1, 0) + V(1, 0));
+            real vert = K(0.5f, 0.5) * (V(0, -1) + V(0, 1));
+
+            switch (s) {
+            case 0:  o[0] = V(0, 0); o[1] = edge;    o[2] = corner;  break;
+            case 1:  o[0] = horiz;   o[1] = V(0, 0); o[2] = vert;    break;
+            case 2:  o[0] = vert;    o[1] = V(0, 0); o[2] = horiz;   break;
+            default: o[0] = corner;  o[1] = edge;    o[2] = V(0, 0); break;
+            }
+            o[3] = K(1.0f, 1.0);
+        }
+    }
+}
+
+/* ref_debayer.c:101-145, DESIGN.md 5.2: the four MHC kernels over 8. */
+static inline real o_mhc_green(const real *v, uint32_t width, uint32_t height, int x, int y)
+{
+    return (K(4.0f, 4.0) * V(0, 0)
+            + K(2.0f, 2.0) * (V(0, -1) + V(0, 1) + V(-1, 0) + V(1, 0))
+            - (V(0, -2) + V(0, 2) + V(-2, 0) + V(2, 0))) * K(0.125f, 0.125);
+}
+
+static inline real o_mhc_cross_h(const real *v, uint32_t width, uint32_t height, int x, int y)
+{
+    return (K(5.0f, 5.0) * V(0, 0)
+            + K(4.0f, 4.0) * (V(-1, 0) + V(1, 0))
+            - (V(-1, -1) + V(1, -1) + V(-1, 1) + V(1, 1) + V(-2, 0) + V(2, 0))
+            + K(0.5f, 0.5) * (V(0, -2) + V(0, 2))) * K(0.125f, 0.125);
+}
+
+static inline real o_mhc_cross_v(const real *v, uint32_t width, uint32_t height, int x, int y)
+{
+    return (K(5.0f, 5.0) * V(0, 0)
+            + K(4.0f, 4.0) * (V(0, -1) + V(0, 1))
+            - (V(-1, -1) + V(1, -1) + V(-1, 1) + V(1, 1) + V(0, -2) + V(0, 2))
+            + K(0.5f, 0.5) * (V(-2, 0) + V(2, 0))) * K(0.125f, 0.125);
+}
+
+static inline real o_mhc_diag(const real *v, uint32_t width, uint32_t height, int x, int y)
+{
+    return (K(6.0f, 6.0) * V(0, 0)
+            + K(2.0f, 2.0) * (V(-1, -1) + V(1, -1) + V(-1, 1) + V(1, 1))
+            - K(1.5f, 1.5) * (V(0, -2) + V(0, 2) + V(-2, 0) + V(2, 0))) * K(0.125f, 0.125);
+}
+
+/* ref_debayer.c:147-176 */
+static void o_mhc(const real *v, uint32_t width, uint32_t height, real *rgb)
+{
+    OMP_FOR
+    for (int y = 0; y < (int)height; y++) {
+        for (int x = 0; x < (int)width; x++) {
+            real *o = &rgb[((size_t)y * width + x) * 4];
+            int s = (y & 1) * 2 + (x & 1);
+            real c = V(0, 0);
+
+            switch (s) {
+            case 0:
+                o[0] = c;
+                o[1] = o_mhc_green(v, width, height, x, y);
+                o[2] = o_mhc_diag(v, width, height, x, y);
+                break;
+            case 1:
+                o[0] = o_mhc_cross_h(v, width, height, x, y);
+                o[1] = c;
+                o[2] = o_mhc_cross_v(v, width, height, x, y);
+                break;
+            case 2:
+                o[0] = o_mhc_cross_v(v, width, height, x, y);
+                o[1] = c;
+                o[2] = o_mhc_cross_h(v, width, height, x, y);
+                break;
+            default:
+                o[0] = o_mhc_diag(v, width, height, x, y);
+                o[1] = o_mhc_green(v, width, height, x, y);
+                o[2] = c;
+                break;
+            }
+            o[3] = K(1.0f, 1.0);
+        }
+    }
+}
+
+#undef V
+
+/* ref_debayer.c:179-190: 3x3 mean, mirrored, summed row by row. */
+static inline real o_box3(const real *s, uint32_t width, uint32_t height, int x, int y)
+{
+    real acc = K(0.0f, 0.0);
+
+    for (int dy = -1; dy <= 1; dy++)
+        for (int dx = -1; dx <= 1; dx++)
+            acc += o_fetch(s, width, height, x + dx, y + dy);
+    return acc * K(1.0f / 9.0f, 1.0 / 9.0);
+}
+
+#define O_RCD_EPS K(1e-5f, 1e-5)
+
+/* ref_debayer.c:195-202: signed low-pass ratio, below black taken as 0. */
+static inline real o_rcd_ratio(real a, real b)
+{
+    a = OMAX(a, K(0.0f, 0.0));
+    b = OMAX(b, K(0.0f, 0.0));
+    return (a - b) / (a + b + O_RCD_EPS);
+}
+
+/* ref_debayer.c:204-395, DESIGN.md 5.3: RCD in full-frame sweeps. */
+static int o_rcd(const real *v, uint32_t width, uint32_t height, real *rgb)
+{
+    size_t n = (size_t)width * height;
+    real *lp = malloc(n * sizeof(real)), *wv = malloc(n * sizeof(real));
+    real *wp = malloc(n * sizeof(real)), *gp = malloc(n * sizeof(real));
+    real *cp = malloc(n * sizeof(real)), *ga = malloc(n * sizeof(real));
+    real *gb = malloc(n * sizeof(real));
+    int ret = 0;
+
+    if (!lp || !wv || !wp || !gp || !cp || !ga || !gb) {
+        ret = -ENOMEM;
+        goto fail;
+    }
+
+#define V(dx, dy) o_fetch(v, width, height, x + (dx), y + (dy))
+#define L(dx, dy) o_fetch(lp, width, height, x + (dx), y + (dy))
+#define G(dx, dy) o_fetch(gp, width, height, x + (dx), y + (dy))
+#define C(dx, dy) o_fetch(cp, width, height, x + (dx), y + (dy))
+
+    /* Pass A: binomial low pass. */
+    OMP_FOR
+    for (int y = 0; y < (int)height; y++)
+        for (int x = 0; x < (int)width; x++)
+            lp[(size_t)y * width + x] =
+                (K(4.0f, 4.0) * V(0, 0)
+                 + K(2.0f, 2.0) * (V(0, -1) + V(0, 1) + V(-1, 0) + V(1, 0))
+                 + V(-1, -1) + V(1, -1) + V(-1, 1) + V(1, 1)) * K(1.0f / 16.0f, 1.0 / 16.0);
+
+    /* Pass A: vertical / horizontal gradients, then their 3x3 means. */
+    OMP_FOR
+    for (int y = 0; y < (int)height; y++) {
+        for (int x = 0; x < (int)width; x++) {
+            size_t i = (size_t)y * width + x;
+            ga[i] = FABSF(V(0, -1) - V(0, 1))
+                  + FABSF(K(2.0f, 2.0) * V(0, 0) - V(0, -2) - V(0, 2));
+            gb[i] = FABSF(V(-1, 0) - V(1, 0))
+                  + FABSF(K(2.0f, 2.0) * V(0, 0) - V(-2, 0) - V(2, 0));
+        }
+    }
+    OMP_FOR
+    for (int y = 0; y < (int)height; y++) {
+        for (int x = 0; x < (int)width; x++) {
+            real vs = o_box3(ga, width, height, x, y);
+            real hs = o_box3(gb, width, height, x, y);
+            wv[(size_t)y * width + x] = hs / (hs + vs + O_RCD_EPS);
+        }
+    }
+
+    /* Pass A: diagonal gradients (P: NE-SW, Q: NW-SE). */
+    OMP_FOR
+    for (int y = 0; y < (int)height; y++) {
+        for (int x = 0; x < (int)width; x++) {
+            size_t i = (size_t)y * width + x;
+            ga[i] = FABSF(V(-1, 1) - V(1, -1))
+                  + FABSF(K(2.0f, 2.0) * V(0, 0) - V(-2, 2) - V(2, -2));
+            gb[i] = FABSF(V(-1, -1) - V(1, 1))
+                  + FABSF(K(2.0f, 2.0) * V(0, 0) - V(-2, -2) - V(2, 2));
+        }
+    }
+    OMP_FOR
+    for (int y = 0; y < (int)height; y++) {
+        for (int x = 0; x < (int)width; x++) {
+            real ps = o_box3(ga, width, height, x, y);
+            real qs = o_box3(gb, width, height, x, y);
+            wp[(size_t)y * width + x] = qs / (ps + qs + O_RCD_EPS);
+        }
+    }
+
+    /* Pass B: green at the red and blue sites. */
+    OMP_FOR
+    for (int y = 0; y < (int)height; y++) {
+        for (int x = 0; x < (int)width; x++) {
+            size_t i = (size_t)y * width + x;
+            int s = (y & 1) * 2 + (x & 1);
+            real l00, lu, ld, ll, lr, gv, gh, w;
+
+            if (s == 1 || s == 2) {
+                gp[i] = v[i];
+                continue;
+            }
+            l00 = L(0, 0);
+            lu = L(0, -2); ld = L(0, 2);
+            ll = L(-2, 0); lr = L(2, 0);
+            gv = K(0.5f, 0.5) * (V(0, -1) * (K(1.0f, 1.0) + o_rcd_ratio(l00, lu))
+                               + V(0, 1) * (K(1.0f, 1.0) + o_rcd_ratio(l00, ld)));
+            gh = K(0.5f, 0.5) * (V(-1, 0) * (K(1.0f, 1.0) + o_rcd_ratio(l00, ll))
+                               + V(1, 0) * (K(1.0f, 1.0) + o_rcd_ratio(l00, lr)));
+            w = wv[i];
+            gp[i] = w * gv + (K(1.0f, 1.0) - w) * gh;
+        }
+    }
+
+    /* Pass C, first half: red/blue at the blue/red sites from the diagonal
+     * colour differences. */
+    OMP_FOR
+    for (int y = 0; y < (int)height; y++) {
+        for (int x = 0; x < (int)width; x++) {
+            size_t i = (size_t)y * width + x;
+            int s = (y & 1) * 2 + (x & 1);
+            real g00, dsw, dne, dnw, dse, c, w;
+            real *o = &rgb[i * 4];
+
+            if (s == 1 || s == 2)
+                continue;
+            g00 = gp[i];
+            dsw = V(-1, 1) - G(-1, 1);
+            dne = V(1, -1) - G(1, -1);
+            dnw = V(-1, -1) - G(-1, -1);
+            dse = V(1, 1) - G(1, 1);
+            w = wp[i];
+            c = w * (g00 + K(0.5f, 0.5) * (dsw + dne))
+              + (K(1.0f, 1.0) - w) * (g00 + K(0.5f, 0.5) * (dnw + dse));
+            cp[i] = c;
+            o[0] = s == 0 ? v[i] : c;
+            o[1] = g00;
+            o[2] = s == 0 ? c : v[i];
+            o[3] = K(1.0f, 1.0);
+        }
+    }
+
+    /* Pass C, second half: both colours at the green sites. */
+    OMP_FOR
+    for (int y = 0; y < (int)height; y++) {
+        for (int x = 0; x < (int)width; x++) {
+            size_t i = (size_t)y * width + x;
+            int s = (y & 1) * 2 + (x & 1);
+            real g00, dh, dv, eh, ev, ch, cv, w;
+            real *o = &rgb[i * 4];
+
+            if (s == 0 || s == 3)
+                continue;
+            g00 = gp[i];
+            w = wv[i];
+            dh = K(0.5f, 0.5) * ((V(-1, 0) - G(-1, 0)) + (V(1, 0) - G(1, 0)));
+            ev = K(0.5f, 0.5) * ((C(0, -1) - G(0, -1)) + (C(0, 1) - G(0, 1)));
+            ch = w * (g00 + ev) + (K(1.0f, 1.0) - w) * (g00 + dh);
+            dv = K(0.5f, 0.5) * ((V(0, -1) - G(0, -1)) + (V(0, 1) - G(0, 1)));
+            eh = K(0.5f, 0.5) * ((C(-1, 0) - G(-1, 0)) + (C(1, 0) - G(1, 0)));
+            cv = w * (g00 + dv) + (K(1.0f, 1.0) - w) * (g00 + eh);
+            o[0] = s == 1 ? ch : cv;
+            o[1] = v[i];
+            o[2] = s == 1 ? cv : ch;
+            o[3] = K(1.0f, 1.0);
+        }
+    }
+
+#undef V
+#undef L
+#undef G
+#undef C
+
+fail:
+    free(lp); free(wv); free(wp); free(gp); free(cp); free(ga); free(gb);
+    return ret;
+}
+
+/* ref_debayer.c:456-472 */
+int PFX(debayer_mosaic)(const real *v, uint32_t width, uint32_t height, int method, real *rgb)
+{
+    if (!v || !rgb)
+        return -EINVAL;
+    if (width < 2 || height < 2 || (width & 1) || (height & 1))
+        return -EINVAL;
+    switch (method) {
+    case CRAW_DEBAYER_BILINEAR: o_bilinear(v, width, height, rgb); return 0;
+    case CRAW_DEBAYER_MHC:      o_mhc(v, width, height, rgb); return 0;
+    case CRAW_DEBAYER_RCD:      return o_rcd(v, width, height, rgb);
+    default:                    return -EINVAL;
+    }
+}
+
+/* ref_debayer.c:397-454 */
+int PFX(debayer_wb)(const uint16_t *bayer, uint32_t width, uint32_t height, size_t stride,
+                    const CRAWProfile *p, const float wb[3], int method, real *rgb)
+{
+    real *v;
+    int ret;
+
+    if (!bayer || !p || !wb || !rgb)
+        return -EINVAL;
+    if ((unsigned)method >= CRAW_DEBAYER_NB)
+        return -EINVAL;
+    if (p->cfa != CRAW_CFA_RGGB)
+        return -ENOTSUP;
+    if (width < 2 || height < 2 || (width & 1) || (height & 1))
+        return -EINVAL;
+    if (stride < width)
+        return -EINVAL;
+    for (int s = 0; s < 4; s++)
+        if (!(p->white_level[s] > p->black_level[s]))
+            return -EINVAL;
+
+    v = malloc((size_t)width * height * sizeof(*v));
+    if (!v)
+        return -ENOMEM;
+    PFX(prep)(bayer, width, height, stride, p, wb, v);
+    ret = PFX(debayer_mosaic)(v, width, height, method, rgb);
+    free(v);
+    return ret;
+}
+
+/* ------------------------------------------------------------ denoise */
+
+#define O_DN_SEARCH_R 3
+#define O_DN_PATCH_R  1
+#define O_DN_GUARD_LO K(0.90f, 0.90)
+#define O_DN_GUARD_HI K(1.00f, 1.00)
+#define O_DN_VAR_MIN  K(1.0e-4f, 1.0e-4)
+
+/* ref_denoise.c:26-58, DESIGN.md 14.3: the generalised Anscombe transform
+ * with its tangent below black, and the algebraic inverse. */
+struct o_gat {
+    real a, c, sc, fwd, inv, off, t0;
+};
+
+static void o_gat_init(struct o_gat *g, float a, float b)
+{
+    real ra = (real)a, rb = (real)b;
+
+    g->a = ra;
+    g->c = K(3.0f, 3.0) * ra * ra / K(8.0f, 8.0) + OMAX(rb, K(0.0f, 0.0));
+    g->sc = SQRTF(g->c);
+    g->fwd = K(2.0f, 2.0) / ra;
+    g->inv = ra / K(4.0f, 4.0);
+    g->off = g->c / ra;
+    g->t0 = g->fwd * g->sc;
+}
+
+static inline real o_gat(const struct o_gat *g, real u)
+{
+    if (u >= K(0.0f, 0.0))
+        return g->fwd * SQRTF(g->a * u + g->c);
+    return g->t0 + u / g->sc;
+}
+
+static inline real o_igat(const struct o_gat *g, real t)
+{
+    if (t >= g->t0)
+        return g->inv * t * t - g->off;
+    return (t - g->t0) * g->sc;
+}
+
+/* ref_denoise.c:63-76: mirror in plane coordinates. */
+static inline int o_dn_mirror(int v, int n)
+{
+    return o_mirror(v, n);
+}
+
+/* ref_denoise.c:82-97: the temporal weight, 3x3 of (prev - cur)^2 summed
+ * row by row. */
+static real o_alpha(const real *cur, const real *prev, int w2, int h2, int i, int k,
+                    real alpha_max, real inv_h2)
+{
+    real d2 = K(0.0f, 0.0);
+
+    for (int ky = -1; ky <= 1; ky++) {
+        for (int kx = -1; kx <= 1; kx++) {
+            size_t n = (size_t)o_dn_mirror(k + ky, h2) * w2 + o_dn_mirror(i + kx, w2);
+            real e = prev[n] - cur[n];
+            d2 += e * e;
+        }
+    }
+    return alpha_max * EXPF(-OMAX(d2 * K(1.0f / 9.0f, 1.0 / 9.0) * inv_h2 - K(2.0f, 2.0),
+                                  K(0.0f, 0.0)));
+}
+
+/* ref_denoise.c:100-105: the variance as the state keeps it, its square
+ * root in eight bits. */
+static inline real o_var_store(real v)
+{
+    real q = RINTF(SQRTF(OCLAMP(v, K(0.0f, 0.0), K(1.0f, 1.0))) * K(255.0f, 255.0)) *
+             K(1.0f / 255.0f, 1.0 / 255.0);
+    return q * q;
+}
+
+/* ref_denoise.c:114-154: the non-local means at one sample, 7x7 search,
+ * 3x3 patch, distances and weighted sums accumulated in raster order. */
+static real o_nlm(const real *plane, const real *var, int w2, const int *mx, const int *my,
+                  real inv_h2)
+{
+    const int sr = O_DN_SEARCH_R, pr = O_DN_PATCH_R;
+    const real inv_np = K(1.0f, 1.0) / (real)((2 * pr + 1) * (2 * pr + 1));
+    real cp[(2 * O_DN_PATCH_R + 1) * (2 * O_DN_PATCH_R + 1)];
+    real acc = K(0.0f, 0.0), wsum = K(0.0f, 0.0);
+    int n = 0;
+
+    for (int ky = -pr; ky <= pr; ky++)
+        for (int kx = -pr; kx <= pr; kx++)
+            cp[n++] = plane[(size_t)my[ky] * w2 + mx[kx]];
+
+    for (int dy = -sr; dy <= sr; dy++) {
+        for (int dx = -sr; dx <= sr; dx++) {
+            real d2 = K(0.0f, 0.0), wgt, sv;
+
+            n = 0;
+            for (int ky = -pr; ky <= pr; ky++) {
+                for (int kx = -pr; kx <= pr; kx++) {
+                    real e = cp[n++] - plane[(size_t)my[dy + ky] * w2 + mx[dx + kx]];
+                    d2 += e * e;
+                }
+            }
+            if (var) {
+                real vc = var[(size_t)my[0] * w2 + mx[0]];
+                real v2 = vc + var[(size_t)my[dy] * w2 + mx[dx]];
+                wgt = EXPF(-OMAX(d2 * inv_np * inv_h2 - v2, K(0.0f, 0.0)) / vc);
+            } else {
+                wgt = EXPF(-OMAX(d2 * inv_np * inv_h2 - K(2.0f, 2.0), K(0.0f, 0.0)));
+            }
+            sv = plane[(size_t)my[dy] * w2 + mx[dx]];
+            wsum += wgt;
+            acc += wgt * sv;
+        }
+    }
+    return acc / wsum;
+}
+
+/* ref_denoise.c:168-336: temporal average (when alpha_max > 0 and a
+ * previous state exists), NLM (when strength > 0), inverse transform,
+ * highlight guard and the 16-bit re-encode. `prev`/`next` are states of
+ * 2 * w * h values indexed like the mosaic: the level, then the variance.
+ * `out_level`, when not NULL, receives the value the re-encode rounds
+ * (black + u * range, clamped, times 65535) for error statements. */
+int PFX(denoise_temporal)(const uint16_t *bayer, uint32_t width, uint32_t height,
+                          size_t stride, const CRAWProfile *profile,
+                          const CRAWNoiseModel *model, float strength, float alpha_max,
+                          const real *prev, real *next, uint16_t *out, size_t out_stride,
+                          real *out_level)
+{
+    const int sr = O_DN_SEARCH_R, pr = O_DN_PATCH_R;
+    const real guard_inv = K(1.0f, 1.0) / (O_DN_GUARD_HI - O_DN_GUARD_LO);
+    struct o_gat g;
+    real h, inv_h2, ralpha = (real)alpha_max;
+    real *plane = NULL, *cur = NULL, *tprev = NULL, *var = NULL;
+    int *mx = NULL, *my = NULL;
+    int w2, h2, rad = sr + pr, ret = 0;
+
+    if (!bayer || !profile || !model || !out)
+        return -EINVAL;
+    if (profile->cfa != CRAW_CFA_RGGB)
+        return -ENOTSUP;
+    if ((width & 1) || (height & 1) ||
+        width < (uint32_t)(2 * (sr + pr + 2)) || height < (uint32_t)(2 * (sr + pr + 2)))
+        return -EINVAL;
+    if (stride < width || out_stride < width)
+        return -EINVAL;
+    if (!(model->a > 0.0f))
+        return -EINVAL;
+    if (!(strength >= 0.0f) || !(alpha_max >= 0.0f) || !(alpha_max <= 1.0f) ||
+        !(strength > 0.0f || alpha_max > 0.0f))
+        return -EINVAL;
+    if (alpha_max > 0.0f && !next)
+        return -EINVAL;
+    if (!(alpha_max > 0.0f))
+        prev = NULL;
+    for (int s = 0; s < 4; s++)
+        if (!(profile->white_level[s] > profile->black_level[s]))
+            return -EINVAL;
+
+    o_gat_init(&g, model->a, model->b);
+    h = strength > 0.0f ? (real)strength : K(1.0f, 1.0);
+    inv_h2 = K(1.0f, 1.0) / (h * h);
+
+    w2 = (int)(width / 2);
+    h2 = (int)(height / 2);
+    plane = malloc((size_t)w2 * h2 * sizeof(*plane));
+    mx = malloc((size_t)(w2 + 2 * rad) * sizeof(*mx));
+    my = malloc((size_t)(h2 + 2 * rad) * sizeof(*my));
+    if (prev) {
+        cur = malloc((size_t)w2 * h2 * sizeof(*cur));
+        tprev = malloc((size_t)w2 * h2 * sizeof(*tprev));
+        var = malloc((size_t)w2 * h2 * sizeof(*var));
+    }
+    if (!plane || !mx || !my || (prev && (!cur || !tprev || !var))) {
+        ret = -ENOMEM;
+        goto done;
+    }
+    for (int i = 0; i < w2 + 2 * rad; i++)
+        mx[i] = o_dn_mirror(i - rad, w2);
+    for (int k = 0; k < h2 + 2 * rad; k++)
+        my[k] = o_dn_mirror(k - rad, h2);
+
+    for (int pl = 0; pl < 4; pl++) {
+        int px = pl & 1, py = pl >> 1;
+        real black = (real)profile->black_level[pl];
+        real range = (real)profile->white_level[pl] - (real)profile->black_level[pl];
+        real inv_range = K(1.0f, 1.0) / range;
+
+        OMP_FOR
+        for (int k = 0; k < h2; k++) {
+            for (int i = 0; i < w2; i++) {
+                size_t mp = (size_t)(2 * k + py) * width + 2 * i + px;
+                real raw = (real)bayer[(size_t)(2 * k + py) * stride + 2 * i + px] *
+                           K(1.0f / 65535.0f, 1.0 / 65535.0);
+                real t = o_gat(&g, (raw - black) * inv_range);
+
+                if (prev) {
+                    cur[(size_t)k * w2 + i] = t;
+                    tprev[(size_t)k * w2 + i] = o_gat(&g, prev[mp]);
+                }
+                plane[(size_t)k * w2 + i] = t;
+            }
+        }
+        if (prev) {
+            const real *prev_var = prev + (size_t)width * height;
+
+            OMP_FOR
+            for (int k = 0; k < h2; k++) {
+                for (int i = 0; i < w2; i++) {
+                    size_t c = (size_t)k * w2 + i;
+                    size_t mp = (size_t)(2 * k + py) * width + 2 * i + px;
+                    real a = o_alpha(cur, tprev, w2, h2, i, k, ralpha, inv_h2);
+
+                    plane[c] = cur[c] * (K(1.0f, 1.0) - a) + tprev[c] * a;
+                    var[c] = OMAX((K(1.0f, 1.0) - a) * (K(1.0f, 1.0) - a) + a * a * prev_var[mp],
+                                  O_DN_VAR_MIN);
+                }
+            }
+        }
+        if (alpha_max > 0.0f) {
+            real *next_var = next + (size_t)width * height;
+
+            OMP_FOR
+            for (int k = 0; k < h2; k++) {
+                for (int i = 0; i < w2; i++) {
+                    size_t mp = (size_t)(2 * k + py) * width + 2 * i + px;
+
+                    next[mp] = o_igat(&g, plane[(size_t)k * w2 + i]);
+                    next_var[mp] = prev ? o_var_store(var[(size_t)k * w2 + i]) : K(1.0f, 1.0);
+                }
+            }
+        }
+
+        OMP_FOR
+        for (int k = 0; k < h2; k++) {
+            for (int i = 0; i < w2; i++) {
+                real raw0, u0, u, gr, raw;
+
+                if (strength > 0.0f)
+                    u = o_igat(&g, o_nlm(plane, prev ? var : NULL, w2, mx + rad + i,
+                                         my + rad + k, inv_h2));
+                else
+                    u = o_igat(&g, plane[(size_t)k * w2 + i]);
+
+                raw0 = (real)bayer[(size_t)(2 * k + py) * stride + 2 * i + px] *
+                       K(1.0f / 65535.0f, 1.0 / 65535.0);
+                u0 = (raw0 - black) * inv_range;
+                gr = OCLAMP((u0 - O_DN_GUARD_LO) * guard_inv, K(0.0f, 0.0), K(1.0f, 1.0));
+                u = u + (u0 - u) * gr;
+
+                raw = OCLAMP(black + u * range, K(0.0f, 0.0), K(1.0f, 1.0));
+                out[(size_t)(2 * k + py) * out_stride + 2 * i + px] =
+                    (uint16_t)RINTF(raw * K(65535.0f, 65535.0));
+                if (out_level)
+                    out_level[(size_t)(2 * k + py) * width + 2 * i + px] =
+                        raw * K(65535.0f, 65535.0);
+            }
+        }
+    }
+
+done:
+    free(var); free(tprev); free(cur); free(my); free(mx); free(plane);
+    return ret;
+}
+
+/* ------------------------------------------------------------- chroma */
+
+#define O_CD_RADIUS    4
+#define O_CD_INV_2S2   K(1.0f / (2.0f * 2.0f * 2.0f), 1.0 / 8.0)
+#define O_CD_RANGE_K   K(0.35f, 0.35)
+#define O_CD_CHROMA_K  K(0.35f, 0.35)
+#define O_CD_RANGE_EPS K(0.02f, 0.02)
+
+/* ref_chroma.c:24-31: one reflection on the quarter grid. */
+static inline int o_cd_mirror(int v, int n)
+{
+    if (v < 0)
+        return -v;
+    if (v >= n)
+        return 2 * n - 2 - v;
+    return v;
+}
+
+/* ref_chroma.c:33-158, DESIGN.md 14.4. */
+int PFX(chroma_denoise)(real *rgb, uint32_t width, uint32_t height, float strength)
+{
+    real *down = NULL, *blur = NULL, rs = (real)strength;
+    int w, h, wq, hq, ret = 0;
+
+    if (!rgb || width < 4 * (O_CD_RADIUS + 1) || height < 4 * (O_CD_RADIUS + 1))
+        return -EINVAL;
+    if (!(strength > 0.0f) || !(strength <= 1.0f))
+        return -EINVAL;
+
+    w = (int)width;
+    h = (int)height;
+    wq = (w + 3) / 4;
+    hq = (h + 3) / 4;
+    down = malloc((size_t)wq * hq * 3 * sizeof(*down));
+    blur = malloc((size_t)wq * hq * 3 * sizeof(*blur));
+    if (!down || !blur) {
+        ret = -ENOMEM;
+        goto fail;
+    }
+
+    /* Pass 1: 4x4 block means of (R - G, B - G, G), row by row. */
+    OMP_FOR
+    for (int qy = 0; qy < hq; qy++) {
+        for (int qx = 0; qx < wq; qx++) {
+            real sum[3] = { K(0.0f, 0.0), K(0.0f, 0.0), K(0.0f, 0.0) }, n = K(0.0f, 0.0);
+            real *o = &down[3 * ((size_t)qy * wq + qx)];
+
+            for (int j = 0; j < 4; j++) {
+                for (int i = 0; i < 4; i++) {
+                    int x = 4 * qx + i, y = 4 * qy + j;
+                    const real *px;
+
+                    if (x >= w || y >= h)
+                        continue;
+                    px = &rgb[4 * ((size_t)y * w + x)];
+                    sum[0] += px[0] - px[1];
+                    sum[1] += px[2] - px[1];
+                    sum[2] += px[1];
+                    n += K(1.0f, 1.0);
+                }
+            }
+            for (int c = 0; c < 3; c++)
+                o[c] = sum[c] / n;
+        }
+    }
+
+    /* Pass 2: 9x9 joint bilateral guided by G, mirrored. */
+    OMP_FOR
+    for (int qy = 0; qy < hq; qy++) {
+        for (int qx = 0; qx < wq; qx++) {
+            const real *c = &down[3 * ((size_t)qy * wq + qx)];
+            real *o = &blur[3 * ((size_t)qy * wq + qx)];
+            real level = OMAX(c[2], K(0.0f, 0.0)) + O_CD_RANGE_EPS;
+            real inv_range = K(1.0f, 1.0) / (O_CD_RANGE_K * level);
+            real inv_chroma = K(1.0f, 1.0) / (O_CD_CHROMA_K * level);
+            real acc[2] = { K(0.0f, 0.0), K(0.0f, 0.0) }, wsum = K(0.0f, 0.0);
+
+            for (int dy = -O_CD_RADIUS; dy <= O_CD_RADIUS; dy++) {
+                int sy = o_cd_mirror(qy + dy, hq);
+
+                for (int dx = -O_CD_RADIUS; dx <= O_CD_RADIUS; dx++) {
+                    int sx = o_cd_mirror(qx + dx, wq);
+                    const real *s = &down[3 * ((size_t)sy * wq + sx)];
+                    real dg = (s[2] - c[2]) * inv_range;
+                    real dd0 = (s[0] - c[0]) * inv_chroma, dd1 = (s[1] - c[1]) * inv_chroma;
+                    real wgt = EXPF(-(real)(dx * dx + dy * dy) * O_CD_INV_2S2 - dg * dg
+                                    - dd0 * dd0 - dd1 * dd1);
+
+                    acc[0] += wgt * s[0];
+                    acc[1] += wgt * s[1];
+                    wsum += wgt;
+                }
+            }
+            o[0] = acc[0] / wsum;
+            o[1] = acc[1] / wsum;
+            o[2] = c[2];
+        }
+    }
+
+    /* Pass 3: bilinear up of (filtered - unfiltered), blended by strength. */
+    OMP_FOR
+    for (int y = 0; y < h; y++) {
+        real fy = ((real)y + K(0.5f, 0.5)) * K(0.25f, 0.25) - K(0.5f, 0.5);
+        real fly = FLOORF(fy), ty = fy - fly;
+        int y0 = OCLAMP((int)fly, 0, hq - 1), y1 = OCLAMP((int)fly + 1, 0, hq - 1);
+
+        for (int x = 0; x < w; x++) {
+            real fx = ((real)x + K(0.5f, 0.5)) * K(0.25f, 0.25) - K(0.5f, 0.5);
+            real flx = FLOORF(fx), tx = fx - flx;
+            int x0 = OCLAMP((int)flx, 0, wq - 1), x1 = OCLAMP((int)flx + 1, 0, wq - 1);
+            real w00 = (K(1.0f, 1.0) - tx) * (K(1.0f, 1.0) - ty), w10 = tx * (K(1.0f, 1.0) - ty);
+            real w01 = (K(1.0f, 1.0) - tx) * ty, w11 = tx * ty;
+            const real *d00 = &down[3 * ((size_t)y0 * wq + x0)], *d10 = &down[3 * ((size_t)y0 * wq + x1)];
+            const real *d01 = &down[3 * ((size_t)y1 * wq + x0)], *d11 = &down[3 * ((size_t)y1 * wq + x1)];
+            const real *b00 = &blur[3 * ((size_t)y0 * wq + x0)], *b10 = &blur[3 * ((size_t)y0 * wq + x1)];
+            const real *b01 = &blur[3 * ((size_t)y1 * wq + x0)], *b11 = &blur[3 * ((size_t)y1 * wq + x1)];
+            real *px = &rgb[4 * ((size_t)y * w + x)];
+            real g = px[1];
+            real d[2] = { px[0] - g, px[2] - g };
+
+            for (int c = 0; c < 2; c++) {
+                real lo = w00 * d00[c] + w10 * d10[c] + w01 * d01[c] + w11 * d11[c];
+                real lf = w00 * b00[c] + w10 * b10[c] + w01 * b01[c] + w11 * b11[c];
+
+                d[c] += rs * (lf - lo);
+            }
+            px[0] = g + d[0];
+            px[2] = g + d[1];
+        }
+    }
+
+fail:
+    free(blur);
+    free(down);
+    return ret;
+}
+
+/* -------------------------------------------------------------- colour */
+
+/* gamut.c:66-92: ACES 1.3 reference gamut compression of one AP1 triple. */
+static inline void o_gamut_compress(const real thr[3], const real scl[3], real pwr, real inv_pwr,
+                                    real rgb[3])
+{
+    real ach = OMAX(rgb[0], OMAX(rgb[1], rgb[2]));
+    real mag;
+
+    if (ach == K(0.0f, 0.0))
+        return;
+    mag = FABSF(ach);
+    for (int i = 0; i < 3; i++) {
+        real d = (ach - rgb[i]) / mag;
+        real nd, p;
+
+        if (d < thr[i])
+            continue;
+        nd = (d - thr[i]) / scl[i];
+        p = POWF(nd, pwr);
+        d = thr[i] + scl[i] * nd / POWF(K(1.0f, 1.0) + p, inv_pwr);
+        rgb[i] = ach - d * mag;
+    }
+}
+
+/* ref_color.c:10-76: matrix, gain and exposure to XYZ, XYZ to the output
+ * space (two matrix-vector products, as the reference does them), gamut
+ * compression for ACEScg, optional clamp; alpha 1. */
+int PFX(color)(real *rgb, uint32_t width, uint32_t height, const CRAWFrameParams *fp,
+               int out_space)
+{
+    CRAWColorParams par;
+    real m_out[3][3], cam_to_xyz[3][3], scale, thr[3], scl[3], pwr, inv_pwr;
+    bool clamp_negative, compress;
+    int ret;
+
+    if (!rgb || !fp || !fp->profile)
+        return -EINVAL;
+    ret = oracle_profile_eval(fp->profile, fp->wb_mode, &fp->illuminant, &par);
+    if (ret < 0)
+        return ret;
+
+#if ORACLE_MODE == 3
+    {
+        double md[3][3], td[3], sd[3], pd, ipd;
+
+        ret = oracle_colorspace_from_xyz_d((enum CRAWColorSpace)out_space, md);
+        if (ret < 0)
+            return ret;
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++)
+                m_out[i][j] = md[i][j];
+        oracle_gamut_params_d(td, sd, &pd, &ipd);
+        for (int i = 0; i < 3; i++) {
+            thr[i] = td[i];
+            scl[i] = sd[i];
+        }
+        pwr = pd;
+        inv_pwr = ipd;
+        scale = (double)par.gain * exp2((double)fp->exposure_stops);
+    }
+#else
+    {
+        float mf[3][3];
+
+        ret = oracle_colorspace_from_xyz_d65((enum CRAWColorSpace)out_space, mf);
+        if (ret < 0)
+            return ret;
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++)
+                m_out[i][j] = mf[i][j];
+        oracle_gamut_params(thr, scl, &pwr, &inv_pwr);
+        scale = par.gain * exp2f(fp->exposure_stops);
+    }
+#endif
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+            cam_to_xyz[i][j] = par.cam_to_xyz[i][j];
+
+    clamp_negative = (fp->flags & CRAW_FRAME_FLAG_CLAMP_NEGATIVE) != 0;
+    compress = out_space == CRAW_CS_ACES_AP1_LINEAR &&
+               !(fp->flags & CRAW_FRAME_FLAG_NO_GAMUT_COMPRESSION);
+
+    OMP_FOR
+    for (size_t i = 0; i < (size_t)width * height; i++) {
+        real *px = &rgb[i * 4];
+        real cam[3], xyz[3], out[3];
+
+        for (int c = 0; c < 3; c++)
+            cam[c] = px[c];
+        for (int r = 0; r < 3; r++)
+            xyz[r] = (cam_to_xyz[r][0] * cam[0] + cam_to_xyz[r][1] * cam[1]
+                      + cam_to_xyz[r][2] * cam[2]) * scale;
+        for (int r = 0; r < 3; r++)
+            out[r] = m_out[r][0] * xyz[0] + m_out[r][1] * xyz[1] + m_out[r][2] * xyz[2];
+        if (compress)
+            o_gamut_compress(thr, scl, pwr, inv_pwr, out);
+        if (clamp_negative)
+            for (int r = 0; r < 3; r++)
+                out[r] = OMAX(out[r], K(0.0f, 0.0));
+        px[0] = out[0];
+        px[1] = out[1];
+        px[2] = out[2];
+        px[3] = K(1.0f, 1.0);
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------ AWB */
+
+/* awb.c:213-296, DESIGN.md 4.4: one sample per 8x8 block (partial blocks
+ * at the edges average what they have), block means summed row by row,
+ * clipped and near-black blocks skipped, ln(R/G) and ln(B/G) of the raw
+ * response binned over [-2.5, 1] into 128 bins, weights in 16.16. */
+int PFX(awb_histogram)(const real *rgb, uint32_t width, uint32_t height, size_t stride,
+                       int channels, const float wb[3], uint32_t *count, uint32_t *weight,
+                       uint32_t *bright)
+{
+    const uint32_t bins = CRAW_AWB_HIST_BINS;
+    const real lmin = K(CRAW_AWB_HIST_LOG_MIN, -2.5), lmax = K(CRAW_AWB_HIST_LOG_MAX, 1.0);
+
+    if (!rgb || !wb || !count || !weight || !bright)
+        return -EINVAL;
+    if (channels != 3 && channels != 4)
+        return -EINVAL;
+    if (stride < (size_t)width * (size_t)channels)
+        return -EINVAL;
+    if (!(wb[0] > 0.0f) || !(wb[2] > 0.0f))
+        return -EINVAL;
+
+    memset(count, 0, (size_t)bins * bins * sizeof(*count));
+    memset(weight, 0, (size_t)bins * bins * sizeof(*weight));
+    memset(bright, 0, (size_t)bins * bins * sizeof(*bright));
+
+    for (uint32_t by = 0; by < height; by += 8) {
+        uint32_t y1 = OMIN(by + 8, height);
+
+        for (uint32_t bx = 0; bx < width; bx += 8) {
+            uint32_t x1 = OMIN(bx + 8, width);
+            real acc[3] = { K(0.0f, 0.0), K(0.0f, 0.0), K(0.0f, 0.0) };
+            real n = (real)((y1 - by) * (x1 - bx));
+            real r, g, b, rr, bb, x, y, w;
+            uint32_t ix, iy;
+
+            for (uint32_t py = by; py < y1; py++) {
+                const real *row = rgb + (size_t)py * stride;
+                for (uint32_t px = bx; px < x1; px++) {
+                    acc[0] += row[(size_t)px * channels + 0];
+                    acc[1] += row[(size_t)px * channels + 1];
+                    acc[2] += row[(size_t)px * channels + 2];
+                }
+            }
+            r = acc[0] / n;
+            g = acc[1] / n;
+            b = acc[2] / n;
+            if (OMAX(OMAX(r, g), b) >= K(0.98f, 0.98) || g < K(0.002f, 0.002))
+                continue;
+            rr = (r / (real)wb[0]) / g;
+            bb = (b / (real)wb[2]) / g;
+            x = LOGF(rr);
+            y = LOGF(bb);
+            x = x > lmin ? (x < lmax ? x : lmax) : lmin;
+            y = y > lmin ? (y < lmax ? y : lmax) : lmin;
+            {
+                real span = lmax - lmin;
+                int bin_x = (int)FLOORF((x - lmin) / span * (real)bins);
+                int bin_y = (int)FLOORF((y - lmin) / span * (real)bins);
+
+                ix = (uint32_t)OCLAMP(bin_x, 0, (int)bins - 1);
+                iy = (uint32_t)OCLAMP(bin_y, 0, (int)bins - 1);
+            }
+            w = OMIN(g, K(0.5f, 0.5));
+            count[iy * bins + ix] += 1;
+            weight[iy * bins + ix] += (uint32_t)ROUNDF(w * K(65536.0f, 65536.0));
+            bright[iy * bins + ix] += (uint32_t)ROUNDF(g * g * g * g * K(65536.0f, 65536.0));
+        }
+    }
+    return 0;
+}
+
+/* -------------------------------------------------------------- process */
+
+/* ref_color.c:91-230 restricted to this path's stages: [temporal +]
+ * [spatial denoise,] debayer with the frame's gains, [chroma denoise,]
+ * [AWB histogram of that intermediate, with the frame's gains, as
+ * pipeline.c:2235-2253 builds it,] colour. `prev`/`next` give the temporal
+ * state as craw_ref_denoise_temporal() takes it (NULL prev: first frame);
+ * `clean` (w*h), `inter` (w*h*4) and the histogram arrays are optional
+ * outputs. */
+int PFX(process)(const uint16_t *bayer, uint32_t width, uint32_t height, size_t stride,
+                 const CRAWPipelineParams *pp, const CRAWFrameParams *fp, const real *prev,
+                 real *next, uint16_t *clean_out, real *inter, uint32_t *count,
+                 uint32_t *weight, uint32_t *bright, real *out)
+{
+    uint16_t *clean = NULL;
+    real *state = NULL;
+    float strength, alpha_max;
+    CRAWColorParams par;
+    int out_space, ret;
+
+    if (!pp || !fp || !fp->profile || !out)
+        return -EINVAL;
+    if (pp->width && (pp->width != width || pp->height != height))
+        return -EINVAL;
+    if (pp->cfa != fp->profile->cfa)
+        return -EINVAL;
+
+    strength = (pp->flags & CRAW_PIPELINE_FLAG_DENOISE) ? fp->denoise_strength : 0.0f;
+    alpha_max = (pp->flags & CRAW_PIPELINE_FLAG_TEMPORAL_DENOISE)
+              ? fp->denoise_temporal_strength : 0.0f;
+    strength = OMAX(strength, 0.0f);
+    alpha_max = OMAX(alpha_max, 0.0f);
+    if (strength > 0.0f || alpha_max > 0.0f) {
+        CRAWNoiseModel nm;
+
+        ret = oracle_noise_model(fp, &nm);
+        if (ret < 0)
+            return ret;
+        clean = malloc((size_t)width * height * sizeof(*clean));
+        if (alpha_max > 0.0f && !next)
+            state = malloc(2 * (size_t)width * height * sizeof(*state));
+        if (!clean || (alpha_max > 0.0f && !next && !state)) {
+            ret = -ENOMEM;
+            goto done;
+        }
+        ret = PFX(denoise_temporal)(bayer, width, height, stride, fp->profile, &nm,
+                                    strength, alpha_max, prev, next ? next : state,
+                                    clean, width, NULL);
+        if (ret < 0)
+            goto done;
+        if (clean_out)
+            memcpy(clean_out, clean, (size_t)width * height * sizeof(*clean));
+        bayer = clean;
+        stride = width;
+    }
+
+    ret = oracle_profile_eval(fp->profile, fp->wb_mode, &fp->illuminant, &par);
+    if (ret < 0)
+        goto done;
+    ret = PFX(debayer_wb)(bayer, width, height, stride, fp->profile, par.wb, pp->debayer, out);
+    if (ret < 0)
+        goto done;
+    if ((pp->flags & CRAW_PIPELINE_FLAG_CHROMA_DENOISE) && fp->denoise_chroma_strength > 0.0f) {
+        ret = PFX(chroma_denoise)(out, width, height, fp->denoise_chroma_strength);
+        if (ret < 0)
+            goto done;
+    }
+    if (inter)
+        memcpy(inter, out, (size_t)width * height * 4 * sizeof(*out));
+    if (count && weight && bright) {
+        ret = PFX(awb_histogram)(out, width, height, (size_t)width * 4, 4, par.wb, count,
+                                 weight, bright);
+        if (ret < 0)
+            goto done;
+    }
+    out_space = fp->out_space_set ? (int)fp->out_space : (int)pp->out_space;
+    ret = PFX(color)(out, width, height, fp, out_space);
+
+done:
+    free(state);
+    free(clean);
+    return ret;
+}
