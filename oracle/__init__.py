@@ -124,6 +124,13 @@ class Mode:
                                ctypes.byref(fp), ctypes.c_int(out_space))
         return ret, rgb
 
+    def gamut_compress(self, rgb3):
+        rgb3 = np.ascontiguousarray(rgb3, self.dtype).copy()
+        f = getattr(lib(), self.prefix + "gamut_compress")
+        f.restype = None
+        f(_ptr(rgb3), ctypes.c_size_t(rgb3.size // 3))
+        return rgb3
+
     def awb_histogram(self, rgb, wb, channels=4, stride=None):
         rgb = np.ascontiguousarray(rgb, self.dtype)
         h, w = rgb.shape[0], rgb.shape[1]

@@ -90,6 +90,7 @@ void oracle_gamut_params_d(double thr[3], double scl[3], double *pwr, double *in
     int P##chroma_denoise(real *rgb, uint32_t w, uint32_t h, float strength);        \
     int P##color(real *rgb, uint32_t w, uint32_t h, const CRAWFrameParams *fp,       \
                  int out_space);                                                     \
+    void P##gamut_compress(real *rgb, size_t n);                                     \
     int P##awb_histogram(const real *rgb, uint32_t w, uint32_t h, size_t stride,     \
                          int channels, const float wb[3], uint32_t *count,           \
                          uint32_t *weight, uint32_t *bright);                        \

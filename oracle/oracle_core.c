@@ -860,6 +860,21 @@ static inline void o_gamut_compress(const real thr[3], const real scl[3], real p
     }
 }
 
+/* The operator alone on `n` AP1 triples in place, with the float (or,
+ * in the double mode, double) constants: gamut.c:66-92. */
+void PFX(gamut_compress)(real *rgb, size_t n)
+{
+    real thr[3], scl[3], pwr, inv_pwr;
+
+#if ORACLE_MODE == 3
+    oracle_gamut_params_d(thr, scl, &pwr, &inv_pwr);
+#else
+    oracle_gamut_params(thr, scl, &pwr, &inv_pwr);
+#endif
+    for (size_t i = 0; i < n; i++)
+        o_gamut_compress(thr, scl, pwr, inv_pwr, &rgb[3 * i]);
+}
+
 /* ref_color.c:10-76: matrix, gain and exposure to XYZ, XYZ to the output
  * space (two matrix-vector products, as the reference does them), gamut
  * compression for ACEScg, optional clamp; alpha 1. */

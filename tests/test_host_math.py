@@ -1,0 +1,316 @@
+"""Host math of the path: colour spaces, profile evaluation, noise model,
+gamut compression constants.
+
+Pins: the reference's own lib/colorspace.c, profile.c, noise.c and gamut.c
+built from their sources under oracle/_ref (bit for bit, both for the
+oracle's restatement and for libcraw.so's own host code), and the
+published matrices and points the reference's tests hold
+(tests/golden/reference_vectors.json)."""
+import ctypes
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from libcraw_amd import capi
+
+GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden",
+                                   "reference_vectors.json")))
+F9 = ctypes.c_float * 9
+REF = oracle.ref_lib()
+needs_ref = pytest.mark.skipif(REF is None, reason="oracle/_ref not built (no reference tree)")
+
+
+def _mat(fn, cs):
+    m = F9()
+    ret = fn(ctypes.c_int(cs), m)
+    return ret, np.array(m[:], dtype=np.float32).reshape(3, 3)
+
+
+def impls():
+    out = [("libcraw", capi.lib(), "craw_"), ("oracle", oracle.lib(), "oracle_")]
+    if REF is not None:
+        out.append(("reference", REF, "craw_"))
+    return out
+
+
+@pytest.mark.parametrize("cs", range(capi.CRAW_CS_NB))
+def test_colorspace_matrices_agree_bitwise(cs):
+    mats = {}
+    for name, l, pfx in impls():
+        for d in ("from", "to"):
+            f = getattr(l, pfx + "colorspace_%s_xyz_d65" % d)
+            f.restype = ctypes.c_int
+            ret, m = _mat(f, cs)
+            assert ret == 0
+            mats.setdefault(d, []).append((name, m))
+    for d, lst in mats.items():
+        for name, m in lst[1:]:
+            assert np.array_equal(m.view(np.uint32), lst[0][1].view(np.uint32)), (d, name)
+
+
+def test_colorspace_golden():
+    l = capi.lib()
+    _, ap0 = _mat(l.craw_colorspace_from_xyz_d65, capi.CRAW_CS_ACES_AP0_LINEAR)
+    _, ap1 = _mat(l.craw_colorspace_from_xyz_d65, capi.CRAW_CS_ACES_AP1_LINEAR)
+    _, ap1_to = _mat(l.craw_colorspace_to_xyz_d65, capi.CRAW_CS_ACES_AP1_LINEAR)
+    _, s709 = _mat(l.craw_colorspace_to_xyz_d65, capi.CRAW_CS_BT709_LINEAR)
+    g = GOLD
+    assert np.abs(ap0 - np.array(g["xyz_to_ap0"]["m"])).max() <= g["xyz_to_ap0"]["tol"]
+    assert np.abs(ap1 - np.array(g["xyz_to_ap1"]["m"])).max() <= g["xyz_to_ap1"]["tol"]
+    srgb_ap1 = ap1.astype(np.float64) @ s709.astype(np.float64)
+    assert np.abs(srgb_ap1 - np.array(g["srgb_to_ap1"]["m"])).max() <= g["srgb_to_ap1"]["tol"]
+    ap1_ap0 = ap0.astype(np.float64) @ ap1_to.astype(np.float64)
+    assert np.abs(ap1_ap0 - np.array(g["ap1_to_ap0"]["m"])).max() <= g["ap1_to_ap0"]["tol"]
+    for cs in range(capi.CRAW_CS_NB):
+        _, a = _mat(l.craw_colorspace_from_xyz_d65, cs)
+        _, b = _mat(l.craw_colorspace_to_xyz_d65, cs)
+        assert np.abs(a.astype(np.float64) @ b - np.eye(3)).max() < 1e-6
+    # D65 white maps to equal RGB in every D65 space
+    w = np.array([0.3127 / 0.3290, 1.0, (1 - 0.3127 - 0.3290) / 0.3290])
+    for cs in (capi.CRAW_CS_BT709_LINEAR, capi.CRAW_CS_BT2020_LINEAR, capi.CRAW_CS_P3D65_LINEAR):
+        _, a = _mat(l.craw_colorspace_from_xyz_d65, cs)
+        rgb = a.astype(np.float64) @ w
+        assert np.abs(rgb - 1.0).max() < 1e-5
+
+
+def test_planckian_and_bradford_golden():
+    l = capi.lib()
+    xy = (ctypes.c_float * 2)()
+    for cct, x, y in GOLD["planckian"]["pts"]:
+        assert l.craw_planckian_xy(ctypes.c_float(cct), xy) == 0
+        assert abs(xy[0] - x) <= 5e-4 and abs(xy[1] - y) <= 5e-4
+    b = GOLD["bradford_d65_d50"]
+    m = F9()
+    assert l.craw_bradford_adaptation((ctypes.c_float * 2)(*b["src"]),
+                                      (ctypes.c_float * 2)(*b["dst"]), m) == 0
+    assert np.abs(np.array(m[:]).reshape(3, 3) - np.array(b["m"])).max() <= b["tol"]
+    assert l.craw_planckian_xy(ctypes.c_float(float("nan")), xy) == -capi.EINVAL
+
+
+def _profile_with_tables():
+    p = capi.CRAWProfile()
+    capi.lib().craw_profile_init(ctypes.byref(p))
+    knots = [(2000.0, (0.8454, 1.0, 3.9482)), (3200.0, (1.3763, 1.0, 2.4707)),
+             (5500.0, (2.02539, 1.0, 1.66797)), (7500.0, (2.2, 1.0, 1.45))]
+    for i, (cct, wb) in enumerate(knots):
+        p.wb_knots[i].cct = cct
+        for c in range(3):
+            p.wb_knots[i].wb[c] = wb[c]
+    p.nb_wb_knots = len(knots)
+    mats = [(2300.0, 0.6535), (4000.0, 0.6759), (6500.0, 0.7088)]
+    for i, (cct, m00) in enumerate(mats):
+        p.matrix_knots[i].cct = cct
+        for r in range(3):
+            for c in range(3):
+                p.matrix_knots[i].m[r][c] = (m00 if r == c else 0.05 * (r - c)) + 0.01 * i
+    p.nb_matrix_knots = len(mats)
+    p.gain = 1.5
+    p.as_shot_ei = 800.0
+    p.exposure_index = 1600.0
+    for i, k in enumerate(GOLD["noise_knots"]["kn"]):
+        p.noise_knots[i].iso, p.noise_knots[i].a, p.noise_knots[i].b = k
+    p.nb_noise_knots = 3
+    return p
+
+
+def _eval(l, fn, p, mode, cct, tint):
+    cp = capi.CRAWColorParams()
+    il = capi.CRAWIlluminant(cct, tint)
+    f = getattr(l, fn)
+    f.restype = ctypes.c_int
+    ret = f(ctypes.byref(p), ctypes.c_int(mode), ctypes.byref(il), ctypes.byref(cp))
+    return ret, np.array(list(cp.wb) + [v for row in cp.cam_to_xyz for v in row] + [cp.gain],
+                         np.float32)
+
+
+@pytest.mark.parametrize("cct,tint", [(1500.0, 0.0), (2000.0, 0.1), (2750.0, -0.05),
+                                      (4100.0, 0.0), (5500.0, 0.02), (7000.0, -0.1),
+                                      (9000.0, 0.0)])
+def test_profile_eval_agrees_bitwise(cct, tint):
+    p = _profile_with_tables()
+    res = []
+    for name, l, pfx in impls():
+        for mode in (capi.CRAW_WB_AS_SHOT, capi.CRAW_WB_REBALANCE):
+            ret, v = _eval(l, pfx + "profile_eval", p, mode, cct, tint)
+            assert ret == 0
+            res.append((name, mode, v))
+    for name, mode, v in res:
+        ref = [r for r in res if r[1] == mode][0][2]
+        assert np.array_equal(v.view(np.uint32), ref.view(np.uint32)), (name, mode)
+    # the knot itself comes back exactly, times the exposure index ratio
+    _, v = _eval(capi.lib(), "craw_profile_eval", p, capi.CRAW_WB_REBALANCE, 3200.0, 0.0)
+    if cct == 1500.0:
+        assert abs(v[0] - 1.3763) < 1e-6 and abs(v[2] - 2.4707) < 1e-6
+        assert v[-1] == np.float32(1.5 * 2.0)
+
+
+def test_profile_eval_errors():
+    l = capi.lib()
+    p = capi.CRAWProfile()
+    l.craw_profile_init(ctypes.byref(p))
+    ret, _ = _eval(l, "craw_profile_eval", p, capi.CRAW_WB_REBALANCE, 5000.0, 0.0)
+    assert ret == -capi.ENOTSUP
+    ret, _ = _eval(l, "craw_profile_eval", p, 7, 5000.0, 0.0)
+    assert ret == -capi.EINVAL
+
+
+def _noise(l, fn, knots, iso):
+    arr = (capi.CRAWNoiseKnot * len(knots))(*[capi.CRAWNoiseKnot(*k) for k in knots])
+    m = capi.CRAWNoiseModel()
+    f = getattr(l, fn)
+    f.restype = ctypes.c_int
+    ret = f(arr, ctypes.c_uint32(len(knots)), ctypes.c_float(iso), ctypes.byref(m))
+    return ret, np.array([m.a, m.b], np.float32)
+
+
+@pytest.mark.parametrize("iso", [0.0, -1.0, 100.0, 400.0, 800.0, 1600.0 * 2 ** 0.5, 2263.0,
+                                 6400.0, 25600.0, float("nan"), float("inf")])
+def test_noise_model_eval(iso):
+    kn = GOLD["noise_knots"]["kn"]
+    res = [(name, _noise(l, pfx + "noise_model_eval", kn, iso)) for name, l, pfx in impls()]
+    for name, (ret, v) in res:
+        assert ret == 0
+        assert np.array_equal(v.view(np.uint32), res[0][1][1].view(np.uint32)), name
+    a, b = res[0][1][1]
+    # tests/test_profile.c:616-662 of the reference
+    if iso == 800.0:
+        assert abs(a - (1e-5 * 2e-5) ** 0.5) <= 1e-5 * a
+        assert abs(b - (4e-8 * 1e-9) ** 0.5) <= 1e-5 * b
+    if iso == 25600.0:
+        assert abs(a - 8e-5 * 4) <= 1e-5 * a and abs(b - 1.6e-8 * 16) <= 1e-5 * b
+    if iso == 100.0:
+        assert abs(a - 1e-5 * 0.25) <= 1e-5 * a and b == np.float32(4e-8)
+    if not (iso > 0) or iso == float("inf"):
+        assert a == np.float32(1e-5) and b == np.float32(4e-8)
+
+
+def test_noise_model_zero_b_and_empty():
+    zb = GOLD["noise_knots"]["zero_b"]
+    ret, v = _noise(capi.lib(), "craw_noise_model_eval", zb, 200.0)
+    assert ret == 0 and abs(v[1] - 2e-9) <= 1e-5 * 2e-9
+    ret, v = _noise(capi.lib(), "craw_noise_model_eval", zb, 50.0)
+    assert v[1] == 0.0
+    ret, _ = _noise(capi.lib(), "craw_noise_model_eval", [], 800.0)
+    assert ret == -capi.EINVAL
+
+
+def test_gamut_constants():
+    thr, scl = (ctypes.c_float * 3)(), (ctypes.c_float * 3)()
+    pw, ipw = ctypes.c_float(), ctypes.c_float()
+    f = oracle.lib().oracle_gamut_params
+    f.restype = None
+    f(thr, scl, ctypes.byref(pw), ctypes.byref(ipw))
+    assert np.abs(np.array(scl[:]) - np.array(GOLD["rgc_scales"]["scl"])).max() < 1e-6
+    assert np.allclose(thr[:], [0.815, 0.803, 0.880]) and abs(pw.value - 1.2) < 1e-7
+    if REF is not None:
+        # CRAWGamutCompress: thr[3], scl[3], pwr, inv_pwr
+        buf = (ctypes.c_float * 8)()
+        REF.craw_gamut_compress_params.restype = None
+        REF.craw_gamut_compress_params(buf)
+        assert list(buf[:3]) == list(thr[:]) and list(buf[3:6]) == list(scl[:])
+        assert buf[6] == pw.value and buf[7] == ipw.value
+
+
+@needs_ref
+def test_gamut_compress_operator_vs_reference():
+    """The oracle's ACES 1.3 RGC against the reference's own
+    craw_gamut_compress() (lib/gamut.c, built from its source) on AP1
+    triples spread over and around the thresholds and limits: the
+    float/glibc mode bit for bit; the deterministic-pow mode within an ulp,
+    and identical wherever glibc's powf rounds correctly."""
+    rng = np.random.default_rng(3)
+    trip = rng.uniform(-0.6, 1.5, size=(20000, 3)).astype(np.float32)
+    trip[:50] = 0.0
+    trip[50:100] = trip[50:100, :1]            # neutrals: never touched
+    buf = (ctypes.c_float * 8)()
+    REF.craw_gamut_compress_params.restype = None
+    REF.craw_gamut_compress_params(buf)
+    REF.craw_gamut_compress.restype = None
+    want = trip.copy()
+    for i in range(len(want)):
+        v = (ctypes.c_float * 3)(*want[i])
+        REF.craw_gamut_compress(buf, v)
+        want[i] = v[:]
+    got_l = oracle.f32l.gamut_compress(trip)
+    assert np.array_equal(got_l.view(np.uint32), want.view(np.uint32))
+    got_d = oracle.f32d.gamut_compress(trip)
+    diff = got_d != want
+    assert diff.mean() < 1e-3
+    assert np.all(np.abs(got_d[diff] - want[diff]) <= 2 * np.spacing(np.abs(want[diff])))
+    moved = np.any(want != trip, axis=1)
+    assert moved.sum() > 1000
+    ach = trip.max(1)
+    assert np.all(want.max(1) == ach)            # the largest channel never moves
+    assert not np.any(moved[:100])
+
+
+@needs_ref
+def test_product_and_oracle_histogram_vs_reference():
+    """craw_awb_histogram_from_rgb: libcraw.so's host builder and the
+    oracle's two float modes against the reference's lib/awb.c, bit for
+    bit, on random camera RGB with clipped, black, negative and partial
+    blocks."""
+    rng = np.random.default_rng(9)
+    w, h = 203, 117   # partial blocks on both edges
+    rgb = rng.uniform(0.0, 1.0, size=(h, w, 4)).astype(np.float32) ** 2
+    rgb[:16, :16, :3] = 0.99
+    rgb[20:36, 40:56, :3] = 0.0005
+    rgb[50:58, 80:96, 0] = -0.3
+    wb = np.array([2.02539, 1.0, 1.66797], np.float32)
+    outs = {}
+    n = 128 * 128
+    for name, l in (("reference", REF), ("libcraw", capi.lib())):
+        c, wt, br = (np.zeros(n, np.uint32) for _ in range(3))
+        f = l.craw_awb_histogram_from_rgb
+        f.restype = ctypes.c_int
+        P = lambda a: a.ctypes.data_as(ctypes.POINTER(
+            ctypes.c_float if a.dtype == np.float32 else ctypes.c_uint32))
+        ret = f(P(rgb), ctypes.c_uint32(w), ctypes.c_uint32(h), ctypes.c_size_t(w * 4),
+                ctypes.c_int(4), P(wb), P(c), P(wt), P(br))
+        assert ret == 0
+        outs[name] = (c, wt, br)
+    for m in (oracle.f32l, oracle.f32d):
+        ret, c, wt, br = m.awb_histogram(rgb, wb)
+        assert ret == 0
+        outs[m.prefix] = (c, wt, br)
+    ref = outs["reference"]
+    assert ref[0].sum() > 100
+    for name, hh in outs.items():
+        for a, b in zip(hh, ref):
+            assert np.array_equal(a, b), name
+
+
+def test_histogram_known_answers():
+    """tests/test_awb.c:156-281 of the reference: a flat 3200 K neutral lands
+    in one bin with count = blocks and weight = blocks * round(0.25 * 65536);
+    clipped and near-black frames give nothing; partial edge blocks count;
+    three-channel input; error codes."""
+    g = GOLD["awb_flat_neutral"]
+    w, h = g["w"], g["h"]
+    wb, k = np.array(g["as_shot_wb"], np.float32), g["knot_3200"]
+    rgb = np.ones((h, w, 4), np.float32)
+    rgb[..., 0] = np.float32(0.25) * np.float32(np.float32(wb[0]) / np.float32(k[0]))
+    rgb[..., 1] = 0.25
+    rgb[..., 2] = np.float32(0.25) * np.float32(np.float32(wb[2]) / np.float32(k[2]))
+    for m in (oracle.f32l, oracle.f32d):
+        ret, c, wt, br = m.awb_histogram(rgb, wb)
+        assert ret == 0
+        nz = np.flatnonzero(c)
+        assert len(nz) == 1 and c.sum() == (w // 8) * (h // 8)
+        ix, iy = nz[0] % 128, nz[0] // 128
+        sc = 128 / 3.5
+        assert ix == int(np.floor((np.log(1 / k[0]) + 2.5) * sc))
+        assert iy == int(np.floor((np.log(1 / k[2]) + 2.5) * sc))
+        assert wt[nz[0]] == (w // 8) * (h // 8) * round(0.25 * 65536)
+        for v in (0.99, 0.001):
+            rgb2 = np.full((h, w, 4), v, np.float32)
+            assert m.awb_histogram(rgb2, wb)[1].sum() == 0
+        small = np.full((4, 12, 4), 0.3, np.float32)
+        assert m.awb_histogram(small, wb)[1].sum() == 2
+        three = np.ascontiguousarray(rgb[:8, :8, :3])
+        assert m.awb_histogram(three, wb, channels=3)[1].sum() == 1
+        assert m.awb_histogram(rgb[:8, :8], wb, channels=2)[0] == -capi.EINVAL
+        assert m.awb_histogram(rgb[:8, :8], wb, stride=8)[0] == -capi.EINVAL
