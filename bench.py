@@ -97,8 +97,9 @@ def cpu_baseline(threads):
 def main():
     args = parse()
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU; the path shards by frames and streams "
-              "(DESIGN.md), and no multi-GPU figure is measured. Use --gpus 1.",
+        print("bench.py: this build runs on one GPU; the development path shards by frames "
+              "and clip streams across GPUs, and no multi-GPU figure is measured. "
+              "Use --gpus 1.",
               file=sys.stderr)
         sys.exit(2)
 

@@ -64,6 +64,12 @@ enum CRAWDebayer {
  * stays within 2^-11 of the clip level of an FP64 evaluation. */
 #define CRAW_PIPELINE_FLAG_NLM_FP16                 (1 << 16)
 
+/* Also this library's own: the NLM in FP32 with its 3x3 sums reordered
+ * (each squared difference computed once, summed across lanes and down
+ * registers) and the hardware exp2. Within a few ulps of the bit-exact
+ * form in every weight; no longer bit for bit the CPU reference. */
+#define CRAW_PIPELINE_FLAG_NLM_FAST                 (1 << 17)
+
 #define CRAW_FALSE_COLOR_MAX_PASSES 5
 #define CRAW_SHARPEN_MAX_RADIUS     2.0f
 #define CRAW_SHARPEN_MAX_ITERATIONS 100

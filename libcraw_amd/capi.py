@@ -38,6 +38,7 @@ CRAW_PIPELINE_FLAG_TEMPORAL_DENOISE = 1 << 7
 CRAW_PIPELINE_FLAG_SHARPEN = 1 << 8
 CRAW_PIPELINE_FLAG_PROFILE = 1 << 9
 CRAW_PIPELINE_FLAG_NLM_FP16 = 1 << 16
+CRAW_PIPELINE_FLAG_NLM_FAST = 1 << 17
 
 CRAW_FRAME_FLAG_CLAMP_NEGATIVE = 1 << 0
 CRAW_FRAME_FLAG_NO_GAMUT_COMPRESSION = 1 << 2
@@ -45,7 +46,7 @@ CRAW_FRAME_FLAG_TEMPORAL_RESET = 1 << 4
 
 CRAW_AWB_HIST_BINS = 128
 
-EINVAL, ENOTSUP, EAGAIN, ENOMEM, ENODEV, EIO = 22, 95, 11, 12, 19, 5
+EINVAL, ENOTSUP, EAGAIN, ENOMEM, ENODEV, EIO, ENOSPC = 22, 95, 11, 12, 19, 5, 28
 
 
 class CRAWIlluminant(ctypes.Structure):
@@ -151,6 +152,18 @@ class CRAWStageTime(ctypes.Structure):
     _fields_ = [("name", ctypes.c_char_p), ("count", c_u32), ("ms", ctypes.c_double)]
 
 
+class CRAWBatchParams(ctypes.Structure):
+    _fields_ = [("max_frames", c_u32), ("max_width", c_u32), ("max_height", c_u32),
+                ("flags", c_u32), ("max_pixels", ctypes.c_uint64), ("debayer", c_i32),
+                ("out_space", c_i32), ("out_format", c_i32), ("use_graphs", c_u32),
+                ("padding", ctypes.c_uint8 * (256 - 4 * 4 - 8 - 4 * 4))]
+
+
+class CRAWBatchFrame(ctypes.Structure):
+    _fields_ = [("src", CRAWImage), ("dst", CRAWImage),
+                ("params", ctypes.POINTER(CRAWFrameParams)), ("state", ctypes.c_void_p)]
+
+
 P = ctypes.POINTER
 vp = ctypes.c_void_p
 
@@ -180,6 +193,16 @@ SIGNATURES = [
     ("craw_bradford_adaptation", c_i32, [P(c_float), P(c_float), P(c_float)]),
     ("craw_awb_histogram_from_rgb", c_i32, [P(c_float), c_u32, c_u32, ctypes.c_size_t, c_i32,
                                              P(c_float), P(c_u32), P(c_u32), P(c_u32)]),
+    ("craw_batch_params_init", None, [P(CRAWBatchParams)]),
+    ("craw_batch_create", c_i32, [vp, P(CRAWBatchParams), P(vp)]),
+    ("craw_batch_free", None, [P(vp)]),
+    ("craw_batch_record", c_i32, [vp, vp, P(CRAWBatchFrame), c_u32]),
+    ("craw_batch_get_awb_histogram", c_i32, [vp, c_u32, P(CRAWAwbHistogram)]),
+    ("craw_batch_get_stage_times", c_i32, [vp, P(CRAWStageTime), c_u32]),
+    ("craw_batch_autotune", c_i32, [vp, vp, P(CRAWBatchFrame), c_u32, c_u32]),
+    ("craw_temporal_state_create", c_i32, [vp, c_u32, c_u32, P(vp)]),
+    ("craw_temporal_state_free", None, [P(vp)]),
+    ("craw_temporal_state_reset", None, [vp]),
 ]
 
 _lib = None

@@ -1,0 +1,581 @@
+/*
+ * libcraw for MI355X: the batched form (include/craw/batch.h).
+ *
+ * A recording plans every frame exactly as craw_pipeline_record() plans
+ * one (craw_frame_plan()), with the frame's intermediates carved out of
+ * per-kind arenas, then builds one work-list of tiles per stage variant
+ * (denoise mode; debayer to the chroma stage or to the output; chroma
+ * blur; chroma up) across all frames and launches each list once as a
+ * persistent kernel. Per-frame constants and the work-lists travel to the
+ * device through a small ring of pinned staging buffers, so recordings can
+ * be queued back to back. With graphs on, the launches of each distinct
+ * recording shape are captured once into a hipGraph and replayed.
+ */
+#include <stdlib.h>
+#include <string.h>
+#include <vector>
+
+#include "internal.h"
+
+#define RING 4
+#define NB_LISTS 10     /* 5 denoise modes, 2 debayer targets, blur, up, spare */
+#define L_DN0 0
+#define L_DB_OUT 5
+#define L_DB_INTER 6
+#define L_CB 7
+#define L_CU 8
+
+static const int dn_modes[5] = {
+    KDN_SPATIAL, KDN_TEMPORAL, KDN_TEMPORAL | KDN_HISTORY, KDN_SPATIAL | KDN_TEMPORAL,
+    KDN_SPATIAL | KDN_TEMPORAL | KDN_HISTORY,
+};
+
+struct Staging {
+    void *host;             /* pinned: frames table, then the tile lists */
+    hipEvent_t done;        /* the H2D copies out of it have executed */
+    bool used;
+};
+
+struct CRAWBatch {
+    CRAWContext *ctx;
+    CRAWBatchParams params;
+    CRAWStageConfig cfg;
+    int form;
+
+    /* arenas */
+    uint16_t *clean;
+    float *inter_rgb, *down, *blur;
+    uint32_t *hist;
+    uint32_t *hist_host;    /* pinned, max_frames histograms */
+    uint32_t nb_last;
+
+    /* device tables */
+    KFrame *frames;
+    uint32_t *tiles;
+    size_t tiles_cap;
+    Staging ring[RING];
+    uint32_t ring_pos;
+
+    /* persistent workgroups per CU for each list kind: denoise, debayer,
+     * blur, up */
+    int wg_per_cu[4];
+
+    /* graph of the last captured shape */
+    std::vector<uint32_t> graph_key;
+    hipGraph_t graph;
+    hipGraphExec_t exec;
+
+    CRAWProfiler prof;
+};
+
+/* Most tiles `stage` can have over the frames of one recording: a frame of
+ * w x h grid samples has at most (w/tw + 1)(h/th + 1) tiles per plane, and
+ * w <= W/div + 1, h <= H/div + 1; summed over at most max_frames frames of
+ * at most max_pixels pixels together. */
+static size_t stage_tiles_bound(int stage, const CRAWBatchParams *p, int method, bool fast)
+{
+    int tw, th, div, planes;
+    double F = p->max_frames, wm, hm, area, n;
+
+    craw_stage_geom(stage, method, fast, &tw, &th, &div, &planes);
+    wm = (double)p->max_width / div + 1.0;
+    hm = (double)p->max_height / div + 1.0;
+    area = (double)p->max_pixels / ((double)div * div) + F * (wm + hm);
+    n = planes * (area / ((double)tw * th) + F * (wm / tw + hm / th + 1.0));
+    return (size_t)n + 64;
+}
+
+extern "C" {
+
+CRAW_API void craw_batch_params_init(CRAWBatchParams *p)
+{
+    if (!p)
+        return;
+    memset(p, 0, sizeof(*p));
+    p->max_frames = 16;
+    p->debayer = CRAW_DEBAYER_RCD;
+    p->out_space = CRAW_CS_XYZ_D65;
+    p->out_format = CRAW_FORMAT_R32G32B32A32_SFLOAT;
+}
+
+CRAW_API void craw_batch_free(CRAWBatch **pb)
+{
+    CRAWBatch *b;
+
+    if (!pb || !*pb)
+        return;
+    b = *pb;
+    CRAW_IGNORE(hipSetDevice(b->ctx->device));
+    if (b->exec)
+        CRAW_IGNORE(hipGraphExecDestroy(b->exec));
+    if (b->graph)
+        CRAW_IGNORE(hipGraphDestroy(b->graph));
+    for (int i = 0; i < RING; i++) {
+        if (b->ring[i].done)
+            CRAW_IGNORE(hipEventDestroy(b->ring[i].done));
+        if (b->ring[i].host)
+            CRAW_IGNORE(hipHostFree(b->ring[i].host));
+    }
+    CRAW_IGNORE(hipFree(b->clean));
+    CRAW_IGNORE(hipFree(b->inter_rgb));
+    CRAW_IGNORE(hipFree(b->down));
+    CRAW_IGNORE(hipFree(b->blur));
+    CRAW_IGNORE(hipFree(b->hist));
+    CRAW_IGNORE(hipFree(b->frames));
+    CRAW_IGNORE(hipFree(b->tiles));
+    if (b->hist_host)
+        CRAW_IGNORE(hipHostFree(b->hist_host));
+    craw_profiler_free(&b->prof);
+    delete b;
+    *pb = NULL;
+}
+
+CRAW_API int craw_batch_create(CRAWContext *ctx, const CRAWBatchParams *params, CRAWBatch **out)
+{
+    CRAWBatch *b;
+    size_t px, qpx, staging;
+    int ret;
+
+    if (!ctx || !params || !out)
+        return CRAW_ERROR(EINVAL);
+    if (!params->max_frames || params->max_frames > 255 || !params->max_pixels ||
+        params->max_width < 8 || params->max_height < 8 ||
+        (uint64_t)params->max_width * params->max_height > params->max_pixels)
+        return CRAW_ERROR(EINVAL);
+
+    b = new CRAWBatch();
+    b->ctx = ctx;
+    b->params = *params;
+    ret = craw_stage_config(ctx, params->flags, params->debayer, params->out_space,
+                            params->out_format, &b->cfg);
+    if (ret < 0) {
+        delete b;
+        return ret;
+    }
+    b->form = b->cfg.fp16 ? KFORM_FP16 : (b->cfg.fast ? KFORM_FAST : KFORM_EXACT);
+    for (int i = 0; i < 4; i++)
+        b->wg_per_cu[i] = 4;
+    if (hipSetDevice(ctx->device) != hipSuccess) {
+        delete b;
+        return CRAW_ERROR(ENODEV);
+    }
+
+    px = params->max_pixels;
+    /* quarter grid of every frame, with a row and a column of rounding */
+    qpx = px / 16 + (size_t)params->max_frames * ((params->max_width + params->max_height) / 4 + 2);
+    ret = CRAW_ERROR(ENOMEM);
+    if ((b->cfg.has_denoise || b->cfg.has_temporal) &&
+        hipMalloc((void **)&b->clean, px * sizeof(uint16_t)) != hipSuccess)
+        goto fail;
+    if (b->cfg.has_chroma &&
+        (hipMalloc((void **)&b->inter_rgb, 3 * px * sizeof(float)) != hipSuccess ||
+         hipMalloc((void **)&b->down, 3 * qpx * sizeof(float)) != hipSuccess ||
+         hipMalloc((void **)&b->blur, 2 * qpx * sizeof(float)) != hipSuccess))
+        goto fail;
+    if (b->cfg.awb &&
+        (hipMalloc((void **)&b->hist, (size_t)params->max_frames * CRAW_HIST_WORDS * 4) !=
+             hipSuccess ||
+         hipHostMalloc((void **)&b->hist_host, (size_t)params->max_frames * CRAW_HIST_WORDS * 4,
+                       hipHostMallocDefault) != hipSuccess))
+        goto fail;
+
+    b->tiles_cap = 0;
+    for (int st = 0; st < KSTAGE_NB; st++)
+        b->tiles_cap += stage_tiles_bound(st, params, (int)params->debayer,
+                                          b->form != KFORM_EXACT);
+    b->tiles_cap += NB_LISTS * 64;
+    if (hipMalloc((void **)&b->frames, params->max_frames * sizeof(KFrame)) != hipSuccess ||
+        hipMalloc((void **)&b->tiles, b->tiles_cap * sizeof(uint32_t)) != hipSuccess)
+        goto fail;
+    staging = params->max_frames * sizeof(KFrame) + b->tiles_cap * sizeof(uint32_t);
+    for (int i = 0; i < RING; i++) {
+        if (hipHostMalloc(&b->ring[i].host, staging, hipHostMallocDefault) != hipSuccess) {
+            b->ring[i].host = NULL;
+            goto fail;
+        }
+        if (hipEventCreateWithFlags(&b->ring[i].done, hipEventDisableTiming) != hipSuccess) {
+            b->ring[i].done = NULL;
+            goto fail;
+        }
+    }
+    if ((params->flags & CRAW_PIPELINE_FLAG_PROFILE) && craw_profiler_init(&b->prof) < 0)
+        goto fail;
+
+    *out = b;
+    return 0;
+
+fail:
+    craw_batch_free(&b);
+    return ret;
+}
+
+} /* extern "C" */
+
+/* The recording of one batch, planned on the host. */
+struct BatchPlan {
+    std::vector<CRAWFramePlan> plans;
+    std::vector<uint32_t> lists[NB_LISTS];
+    std::vector<uint32_t> key;
+};
+
+static int batch_plan(CRAWBatch *b, const CRAWBatchFrame *frames, uint32_t nb, BatchPlan &bp)
+{
+    size_t off_px = 0, off_q = 0;
+    bool fast = b->form != KFORM_EXACT;
+
+    if (!nb || nb > b->params.max_frames)
+        return CRAW_ERROR(ENOSPC);
+    bp.plans.resize(nb);
+    for (uint32_t i = 0; i < nb; i++) {
+        const CRAWBatchFrame &fr = frames[i];
+        uint32_t w = fr.src.width, h = fr.src.height;
+        size_t q = (size_t)((w + 3) / 4) * ((h + 3) / 4);
+        CRAWFrameBuffers buf = {};
+        int ret;
+
+        if (!fr.params || !fr.params->profile)
+            return CRAW_ERROR(EINVAL);
+        if (w > b->params.max_width || h > b->params.max_height ||
+            off_px + (size_t)w * h > b->params.max_pixels)
+            return CRAW_ERROR(ENOSPC);
+        buf.clean = b->clean ? b->clean + off_px : NULL;
+        buf.inter_rgb = b->inter_rgb ? b->inter_rgb + 3 * off_px : NULL;
+        buf.down = b->down ? b->down + 3 * off_q : NULL;
+        buf.blur = b->blur ? b->blur + 2 * off_q : NULL;
+        buf.hist = b->hist ? b->hist + (size_t)i * CRAW_HIST_WORDS : NULL;
+        buf.ts = fr.state;
+        ret = craw_frame_plan(b->ctx, b->cfg, buf, w, h, &fr.src, &fr.dst, fr.params,
+                              &bp.plans[i]);
+        if (ret < 0)
+            return ret;
+        off_px += (size_t)w * h;
+        off_q += q;
+    }
+    /* one temporal state may appear once per recording */
+    for (uint32_t i = 0; i < nb; i++)
+        for (uint32_t j = i + 1; j < nb; j++)
+            if (frames[i].state && frames[i].state == frames[j].state)
+                return CRAW_ERROR(EINVAL);
+
+    for (uint32_t i = 0; i < nb; i++) {
+        const CRAWFramePlan &pl = bp.plans[i];
+        int W = pl.f.W, H = pl.f.H, ntx, n;
+
+        if (pl.dn_mode) {
+            int li = -1;
+            for (int m = 0; m < 5; m++)
+                if (dn_modes[m] == pl.dn_mode)
+                    li = L_DN0 + m;
+            n = craw_stage_tiles(KSTAGE_DENOISE, W, H, pl.f.method, fast, &ntx);
+            for (int t = 0; t < n; t++)
+                bp.lists[li].push_back(KTILE(i, t));
+        }
+        n = craw_stage_tiles(KSTAGE_DEBAYER, W, H, pl.f.method, fast, &ntx);
+        for (int t = 0; t < n; t++)
+            bp.lists[pl.chroma ? L_DB_INTER : L_DB_OUT].push_back(KTILE(i, t));
+        if (pl.chroma) {
+            n = craw_stage_tiles(KSTAGE_CHROMA_BLUR, W, H, pl.f.method, fast, &ntx);
+            for (int t = 0; t < n; t++)
+                bp.lists[L_CB].push_back(KTILE(i, t));
+            n = craw_stage_tiles(KSTAGE_CHROMA_UP, W, H, pl.f.method, fast, &ntx);
+            for (int t = 0; t < n; t++)
+                bp.lists[L_CU].push_back(KTILE(i, t));
+        }
+        bp.key.push_back((uint32_t)W);
+        bp.key.push_back((uint32_t)H);
+        bp.key.push_back((uint32_t)pl.dn_mode | (pl.chroma ? 0x100u : 0u));
+    }
+    {
+        size_t total = 0;
+        for (int l = 0; l < NB_LISTS; l++)
+            total += bp.lists[l].size();
+        if (total > b->tiles_cap)
+            return CRAW_ERROR(ENOSPC);
+    }
+    for (int k = 0; k < 4; k++)
+        bp.key.push_back((uint32_t)b->wg_per_cu[k]);
+    return 0;
+}
+
+static int grid_for(CRAWBatch *b, int kind, size_t ntiles)
+{
+    size_t g = (size_t)b->wg_per_cu[kind] * (size_t)b->ctx->cu_count;
+    return (int)(ntiles < g ? ntiles : g);
+}
+
+/* The stage launches of a planned recording, in order. */
+static hipError_t batch_launch(CRAWBatch *b, const BatchPlan &bp, const size_t *list_off,
+                               hipStream_t s, CRAWProfRun *pr)
+{
+    const uint32_t nb = (uint32_t)bp.plans.size();
+    hipError_t e = hipSuccess;
+
+#define PB(name) do { if (pr) craw_profiler_begin(*pr, name); } while (0)
+#define PE() do { if (pr) craw_profiler_end(*pr); } while (0)
+    if (b->cfg.awb) {
+        PB("awb_clear");
+        e = craw_launch_zero_u32(b->hist, (size_t)nb * CRAW_HIST_WORDS, s);
+        PE();
+    }
+    for (int m = 0; m < 5 && e == hipSuccess; m++) {
+        size_t n = bp.lists[L_DN0 + m].size();
+        if (!n)
+            continue;
+        PB(b->form == KFORM_FP16 ? "denoise_nlm_fp16" : (b->form == KFORM_FAST ?
+                                                            "denoise_nlm_fast" : "denoise_nlm"));
+        e = craw_launch_batch_denoise(dn_modes[m], b->form, b->frames, b->tiles + list_off[L_DN0 + m],
+                                      (uint32_t)n, grid_for(b, 0, n), s);
+        PE();
+    }
+    for (int l = L_DB_OUT; l <= L_DB_INTER && e == hipSuccess; l++) {
+        size_t n = bp.lists[l].size();
+        static const char *names[3] = { "debayer_bilinear", "debayer_mhc", "rcd" };
+        if (!n)
+            continue;
+        PB(names[b->cfg.debayer]);
+        e = craw_launch_batch_debayer((int)b->cfg.debayer, l == L_DB_INTER, b->frames,
+                                      b->tiles + list_off[l], (uint32_t)n, grid_for(b, 1, n), s);
+        PE();
+    }
+    if (e == hipSuccess && bp.lists[L_CB].size()) {
+        size_t n = bp.lists[L_CB].size();
+        PB("chroma_blur");
+        e = craw_launch_batch_chroma_blur(b->frames, b->tiles + list_off[L_CB], (uint32_t)n,
+                                          grid_for(b, 2, n), s);
+        PE();
+    }
+    if (e == hipSuccess && bp.lists[L_CU].size()) {
+        size_t n = bp.lists[L_CU].size();
+        PB("chroma_up_color");
+        e = craw_launch_batch_chroma_up(b->frames, b->tiles + list_off[L_CU], (uint32_t)n,
+                                        grid_for(b, 3, n), s);
+        PE();
+    }
+#undef PB
+#undef PE
+    return e;
+}
+
+static int batch_record(CRAWBatch *b, hipStream_t s, const CRAWBatchFrame *frames, uint32_t nb,
+                        bool graphs)
+{
+    BatchPlan bp;
+    Staging *stg;
+    size_t list_off[NB_LISTS], pos = 0, frames_bytes;
+    hipError_t e;
+    int ret;
+
+    if (!b || !frames)
+        return CRAW_ERROR(EINVAL);
+    ret = batch_plan(b, frames, nb, bp);
+    if (ret < 0)
+        return ret;
+    if (hipSetDevice(b->ctx->device) != hipSuccess)
+        return CRAW_ERROR(ENODEV);
+
+    /* staging: wait until the slot's previous copies have executed */
+    stg = &b->ring[b->ring_pos];
+    if (stg->used && hipEventSynchronize(stg->done) != hipSuccess)
+        return CRAW_ERROR(EIO);
+    frames_bytes = nb * sizeof(KFrame);
+    for (uint32_t i = 0; i < nb; i++)
+        memcpy((char *)stg->host + i * sizeof(KFrame), &bp.plans[i].f, sizeof(KFrame));
+    for (int l = 0; l < NB_LISTS; l++) {
+        list_off[l] = pos;
+        if (bp.lists[l].size())
+            memcpy((char *)stg->host + frames_bytes + pos * 4, bp.lists[l].data(),
+                   bp.lists[l].size() * 4);
+        pos += bp.lists[l].size();
+    }
+    e = hipMemcpyAsync(b->frames, stg->host, frames_bytes, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess && pos)
+        e = hipMemcpyAsync(b->tiles, (char *)stg->host + frames_bytes, pos * 4,
+                           hipMemcpyHostToDevice, s);
+    if (e == hipSuccess)
+        e = hipEventRecord(stg->done, s);
+    if (e != hipSuccess)
+        return CRAW_ERROR(EIO);
+    stg->used = true;
+    b->ring_pos = (b->ring_pos + 1) % RING;
+
+    if (graphs) {
+        if (!b->exec || bp.key != b->graph_key) {
+            hipGraph_t g = NULL;
+            hipGraphExec_t x = NULL;
+
+            e = hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed);
+            if (e == hipSuccess) {
+                hipError_t le = batch_launch(b, bp, list_off, s, NULL);
+                e = hipStreamEndCapture(s, &g);
+                if (le != hipSuccess)
+                    e = le;
+            }
+            if (e == hipSuccess)
+                e = hipGraphInstantiate(&x, g, NULL, NULL, 0);
+            if (e != hipSuccess) {
+                if (g)
+                    CRAW_IGNORE(hipGraphDestroy(g));
+                craw_log(b->ctx, CRAW_LOG_ERROR, "Graph capture failed: %s\n",
+                         hipGetErrorString(e));
+                return CRAW_ERROR(EIO);
+            }
+            if (b->exec)
+                CRAW_IGNORE(hipGraphExecDestroy(b->exec));
+            if (b->graph)
+                CRAW_IGNORE(hipGraphDestroy(b->graph));
+            b->graph = g;
+            b->exec = x;
+            b->graph_key = bp.key;
+        }
+        if (b->prof.on) {
+            CRAWProfRun pr = craw_profiler_start(&b->prof, s);
+            craw_profiler_begin(pr, "batch_graph");
+            e = hipGraphLaunch(b->exec, s);
+            craw_profiler_end(pr);
+            craw_profiler_finish(pr);
+        } else {
+            e = hipGraphLaunch(b->exec, s);
+        }
+    } else if (b->prof.on) {
+        CRAWProfRun pr = craw_profiler_start(&b->prof, s);
+        e = batch_launch(b, bp, list_off, s, &pr);
+        craw_profiler_finish(pr);
+    } else {
+        e = batch_launch(b, bp, list_off, s, NULL);
+    }
+    if (e == hipSuccess && b->cfg.awb)
+        e = hipMemcpyAsync(b->hist_host, b->hist, (size_t)nb * CRAW_HIST_WORDS * 4,
+                           hipMemcpyDeviceToHost, s);
+    if (e != hipSuccess) {
+        craw_log(b->ctx, CRAW_LOG_ERROR, "Batch launch failed: %s\n", hipGetErrorString(e));
+        return CRAW_ERROR(EIO);
+    }
+    b->nb_last = nb;
+    for (uint32_t i = 0; i < nb; i++)
+        craw_frame_commit(&bp.plans[i]);
+    return 0;
+}
+
+extern "C" {
+
+CRAW_API int craw_batch_record(CRAWBatch *b, CRAWStream stream, const CRAWBatchFrame *frames,
+                               uint32_t nb)
+{
+    if (!b)
+        return CRAW_ERROR(EINVAL);
+    /* the legacy default stream cannot be captured: launch directly there */
+    return batch_record(b, (hipStream_t)stream, frames, nb,
+                        b->params.use_graphs != 0 && stream != NULL);
+}
+
+CRAW_API int craw_batch_get_awb_histogram(CRAWBatch *b, uint32_t index, CRAWAwbHistogram *out)
+{
+    const uint32_t *base;
+
+    if (!b || !out)
+        return CRAW_ERROR(EINVAL);
+    if (!b->cfg.awb || index >= b->nb_last)
+        return CRAW_ERROR(EINVAL);
+    base = b->hist_host + (size_t)index * CRAW_HIST_WORDS;
+    out->bins = CRAW_AWB_HIST_BINS;
+    out->log_min = CRAW_AWB_HIST_LOG_MIN;
+    out->log_max = CRAW_AWB_HIST_LOG_MAX;
+    out->count = base;
+    out->weight = base + CRAW_AWB_HIST_BINS * CRAW_AWB_HIST_BINS;
+    out->weight_bright = base + 2 * CRAW_AWB_HIST_BINS * CRAW_AWB_HIST_BINS;
+    return 0;
+}
+
+CRAW_API int craw_batch_get_stage_times(CRAWBatch *b, CRAWStageTime *out, uint32_t max)
+{
+    if (!b)
+        return CRAW_ERROR(EINVAL);
+    return craw_profiler_times(b->ctx, &b->prof, 0, out, max);
+}
+
+/* Times every stage of `iterations` recordings for each candidate number
+ * of persistent workgroups per CU and keeps, per stage kind, the fastest.
+ * The temporal states of the frames are left as they were. */
+CRAW_API int craw_batch_autotune(CRAWBatch *b, CRAWStream stream, const CRAWBatchFrame *frames,
+                                 uint32_t nb, uint32_t iterations)
+{
+    static const int cand[] = { 1, 2, 4, 8, 16 };
+    const int ncand = (int)(sizeof(cand) / sizeof(cand[0]));
+    double best[4] = { 1e30, 1e30, 1e30, 1e30 }, total = 0.0;
+    int pick[4] = { 4, 4, 4, 4 };
+    hipStream_t s = (hipStream_t)stream;
+    CRAWProfiler saved;
+    std::vector<int> prev(nb), cur(nb);
+    int ret = 0;
+
+    if (!b || !frames || !nb)
+        return CRAW_ERROR(EINVAL);
+    if (!iterations)
+        iterations = 3;
+    for (uint32_t i = 0; i < nb; i++) {
+        prev[i] = frames[i].state ? frames[i].state->prev : -1;
+        cur[i] = frames[i].state ? frames[i].state->cur : -1;
+    }
+    saved = b->prof;
+    memset(&b->prof, 0, sizeof(b->prof));
+    if ((ret = craw_profiler_init(&b->prof)) < 0) {
+        b->prof = saved;
+        return ret;
+    }
+    for (int c = 0; c < ncand && ret >= 0; c++) {
+        double acc[4] = { 0, 0, 0, 0 };
+
+        for (int k = 0; k < 4; k++)
+            b->wg_per_cu[k] = cand[c];
+        for (uint32_t it = 0; it <= iterations && ret >= 0; it++) {
+            CRAWStageTime st[CRAW_MAX_STAGES];
+            int n;
+
+            for (uint32_t i = 0; i < nb; i++)
+                if (frames[i].state) {
+                    frames[i].state->prev = prev[i];
+                    frames[i].state->cur = cur[i];
+                }
+            ret = batch_record(b, s, frames, nb, false);
+            if (ret < 0 || hipStreamSynchronize(s) != hipSuccess) {
+                ret = ret < 0 ? ret : CRAW_ERROR(EIO);
+                break;
+            }
+            if (!it)
+                continue;   /* the first run of each candidate is a warm-up */
+            n = craw_profiler_times(b->ctx, &b->prof, 0, st, CRAW_MAX_STAGES);
+            for (int j = 0; j < n && j < CRAW_MAX_STAGES; j++) {
+                int kind = !strncmp(st[j].name, "denoise", 7) ? 0 :
+                           !strcmp(st[j].name, "chroma_blur") ? 2 :
+                           !strcmp(st[j].name, "chroma_up_color") ? 3 :
+                           !strcmp(st[j].name, "awb_clear") ? -1 : 1;
+                if (kind >= 0)
+                    acc[kind] += st[j].ms;
+            }
+        }
+        for (int k = 0; k < 4; k++)
+            if (acc[k] > 0.0 && acc[k] < best[k]) {
+                best[k] = acc[k];
+                pick[k] = cand[c];
+            }
+    }
+    craw_profiler_free(&b->prof);
+    b->prof = saved;
+    for (uint32_t i = 0; i < nb; i++)
+        if (frames[i].state) {
+            frames[i].state->prev = prev[i];
+            frames[i].state->cur = cur[i];
+        }
+    if (ret < 0)
+        return ret;
+    for (int k = 0; k < 4; k++) {
+        b->wg_per_cu[k] = pick[k];
+        if (best[k] < 1e30)
+            total += best[k] / iterations;
+    }
+    craw_log(b->ctx, CRAW_LOG_VERBOSE, "Autotune: workgroups per CU %d/%d/%d/%d\n", pick[0],
+             pick[1], pick[2], pick[3]);
+    return (int)(total * 1000.0);
+}
+
+} /* extern "C" */

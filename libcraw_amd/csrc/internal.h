@@ -1,5 +1,6 @@
 /*
- * libcraw for MI355X: internal state of contexts and pipelines.
+ * libcraw for MI355X: internal state of contexts, pipelines, batches and
+ * temporal states.
  */
 #ifndef CRAW_INTERNAL_H
 #define CRAW_INTERNAL_H
@@ -7,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <craw/pipeline.h>
+#include <craw/batch.h>
 #include "hostmath.h"
 #include "kernels.h"
 
@@ -18,71 +20,108 @@ struct CRAWContext {
 };
 
 #define CRAW_MAX_STAGES 8
+#define CRAW_HIST_WORDS (3u * CRAW_AWB_HIST_BINS * CRAW_AWB_HIST_BINS)
 
-/* Everything one recording enqueues, worked out on the host before any
- * of it is launched, so that a failing record changes nothing. */
+/* Two sets of (level, variance) for one clip stream and which set holds
+ * what (lib/pipeline.c:435-445, 1796-1818, 2303-2313 of the reference):
+ * `cur` the set the last recording wrote, `prev` the set it read, -1
+ * nothing valid. */
+struct CRAWTemporalState {
+    CRAWContext *ctx;
+    uint32_t width, height;
+    float *level[2];            /* 4 planes of (w/2) x (h/2) */
+    uint8_t *var[2];            /* square root of the variance, 8 bits */
+    int prev, cur;
+};
+
+/* What a pipeline or a batch runs; fixed at creation. */
+struct CRAWStageConfig {
+    enum CRAWDebayer debayer;
+    enum CRAWColorSpace out_space;
+    enum CRAWFormat out_format;
+    bool awb, has_denoise, has_temporal, has_chroma, fp16, fast;
+};
+
+/* The device buffers one frame's recording uses. */
+struct CRAWFrameBuffers {
+    uint16_t *clean;            /* W x H, when denoising */
+    float *inter_rgb;           /* 3 x W x H, with chroma denoise */
+    float *down, *blur;         /* 3 and 2 quarter planes */
+    uint32_t *hist;             /* CRAW_HIST_WORDS, with the histogram */
+    CRAWTemporalState *ts;      /* with the temporal pass, or NULL */
+};
+
+/* Everything one frame's recording enqueues, worked out on the host
+ * before any of it is launched, so that a failing record changes nothing. */
 struct CRAWFramePlan {
-    KPrep prep;
-    KColor col;
-    KOut out;
-    KInter inter;
-    KDenoise dn;
-    KDenoiseState st;
-    int dn_mode;                /* 0: no denoise */
+    KFrame f;                   /* what the kernels read */
+    int dn_mode;                /* KDN_* bits, 0: no denoise */
     bool chroma;
-    float chroma_strength;
     int t_read, t_write;        /* temporal state sets, -1: none */
     bool temporal, replay;
-    const uint16_t *src;
-    int src_pitch;              /* uint16 elements */
-    int method;
+    CRAWTemporalState *ts;
+};
+
+struct CRAWProfiler {
+    hipEvent_t ev[CRAW_PIPELINE_PROFILE_SLOTS][CRAW_MAX_STAGES][2];
+    const char *name[CRAW_PIPELINE_PROFILE_SLOTS][CRAW_MAX_STAGES];
+    uint32_t nb[CRAW_PIPELINE_PROFILE_SLOTS];
+    uint64_t recorded;
+    CRAWStageTime agg[CRAW_MAX_STAGES];
+    bool on;
+};
+
+/* One recording's launches being profiled. */
+struct CRAWProfRun {
+    CRAWProfiler *pr;
+    hipStream_t s;
+    uint32_t slot, n;
 };
 
 struct CRAWPipeline {
     CRAWContext *ctx;
     CRAWPipelineParams params;
-    bool awb, has_denoise, has_temporal, has_chroma, profile, fp16;
-
-    uint16_t *clean;            /* denoised mosaic, W x H */
-    float *tn_level[2];         /* temporal state sets, 4 planes of w2 x h2 */
-    uint8_t *tn_var[2];
-    int tn_prev, tn_cur;
-    float *inter_rgb;           /* 3 x W x H */
-    float *down, *blur;         /* quarter grid, 3 and 2 planes */
-    int wq, hq;
-    uint32_t *hist;             /* count, weight, bright, 128^2 each */
+    CRAWStageConfig cfg;
+    CRAWFrameBuffers buf;
+    CRAWTemporalState ts;
     uint32_t *hist_host[4];     /* pinned readback slots */
     uint32_t nb_slots;
     uint64_t nb_recorded;
-
-    /* profiling: a pair of events around every launch, four recordings */
-    hipEvent_t ev[CRAW_PIPELINE_PROFILE_SLOTS][CRAW_MAX_STAGES][2];
-    const char *ev_name[CRAW_PIPELINE_PROFILE_SLOTS][CRAW_MAX_STAGES];
-    uint32_t ev_nb[CRAW_PIPELINE_PROFILE_SLOTS];
-    uint32_t prof_slot;
-    uint64_t prof_recorded;
-    CRAWStageTime agg[CRAW_MAX_STAGES];
+    CRAWProfiler prof;
 };
-
-#define CRAW_HIST_WORDS (3u * CRAW_AWB_HIST_BINS * CRAW_AWB_HIST_BINS)
 
 extern "C" {
 void craw_vlog_ctx(CRAWContext *ctx, enum CRAWLogLevel level, const char *fmt, va_list ap);
-
-/* Plans a frame (validation, constants, state selection); 0 or a negative
- * errno, nothing changed on failure. Shared by the plain and the batched
- * forms. */
-int craw_pipeline_plan(CRAWPipeline *p, const CRAWImage *src, const CRAWImage *dst,
-                       const CRAWFrameParams *fp, CRAWFramePlan *plan);
-
-/* Enqueues a planned frame on `stream`; a launch failure is reported as
- * CRAW_ERROR(EIO). */
-int craw_pipeline_enqueue(CRAWPipeline *p, const CRAWFramePlan *plan, hipStream_t stream);
-
-/* Moves the temporal indices and the histogram slot after a successful
- * enqueue. */
-void craw_pipeline_commit(CRAWPipeline *p, const CRAWFramePlan *plan);
 }
+
+/* Checks and resolves a stage configuration's flags. */
+int craw_stage_config(CRAWContext *ctx, uint32_t flags, enum CRAWDebayer debayer,
+                      enum CRAWColorSpace out_space, enum CRAWFormat out_format,
+                      CRAWStageConfig *cfg);
+
+/* Plans one frame of W x H (validation, constants, state selection); 0 or
+ * a negative errno, nothing changed on failure. */
+int craw_frame_plan(CRAWContext *ctx, const CRAWStageConfig &cfg, const CRAWFrameBuffers &buf,
+                    uint32_t W, uint32_t H, const CRAWImage *src, const CRAWImage *dst,
+                    const CRAWFrameParams *fp, CRAWFramePlan *plan);
+
+/* Moves a planned frame's temporal indices once it has been enqueued. */
+void craw_frame_commit(const CRAWFramePlan *plan);
+
+/* A temporal state's device memory. */
+int craw_temporal_alloc(CRAWContext *ctx, uint32_t w, uint32_t h, CRAWTemporalState *ts);
+void craw_temporal_release(CRAWTemporalState *ts);
+
+/* Profiling: an event pair around every launch of a recording, the last
+ * CRAW_PIPELINE_PROFILE_SLOTS recordings kept. */
+int craw_profiler_init(CRAWProfiler *pr);
+void craw_profiler_free(CRAWProfiler *pr);
+CRAWProfRun craw_profiler_start(CRAWProfiler *pr, hipStream_t s);
+void craw_profiler_begin(CRAWProfRun &r, const char *name);
+void craw_profiler_end(CRAWProfRun &r);
+void craw_profiler_finish(CRAWProfRun &r);
+int craw_profiler_times(CRAWContext *ctx, CRAWProfiler *pr, uint32_t frames_back,
+                        CRAWStageTime *out, uint32_t max);
 
 /* hipFree and friends on teardown paths, whose errors change nothing. */
 #define CRAW_IGNORE(x) do { hipError_t craw_ignored_ = (x); (void)craw_ignored_; } while (0)

@@ -135,17 +135,20 @@ KDEV void kstore_inter(const float (*rgbs)[TH][TW], int x0, int y0, int W, int H
 #define BL_A  2
 #define BL_NT 256
 
+struct SmSimple {
+    float vs[BL_TH + 2 * BL_A][BL_TW + 2 * BL_A];
+    float rgbs[3][BL_TH][BL_TW];
+};
+
 template <int METHOD, bool INTER>
-__global__ __launch_bounds__(BL_NT) void k_debayer_simple(const uint16_t *__restrict__ src,
-                                                          int src_pitch, int W, int H,
-                                                          KPrep prep, KColor col, KOut out,
-                                                          KInter inter)
+KDEV void dev_debayer_simple(SmSimple &sm, const KFrame &f, int x0, int y0)
 {
-    constexpr int SW = BL_TW + 2 * BL_A, SH = BL_TH + 2 * BL_A;
-    __shared__ float vs[SH][SW];
-    __shared__ float rgbs[3][BL_TH][BL_TW];
+    float (&vs)[BL_TH + 2 * BL_A][BL_TW + 2 * BL_A] = sm.vs;
+    float (&rgbs)[3][BL_TH][BL_TW] = sm.rgbs;
+    const uint16_t *__restrict__ src = f.src;
+    const int src_pitch = f.src_pitch, W = f.W, H = f.H;
+    const KPrep &prep = f.prep;
     const int tid = threadIdx.x;
-    const int x0 = blockIdx.x * BL_TW, y0 = blockIdx.y * BL_TH;
     const int xe = min(x0 + BL_TW, W), ye = min(y0 + BL_TH, H);
     const int rw = xe - x0 + 2 * BL_A, rh = ye - y0 + 2 * BL_A;
 
@@ -206,9 +209,9 @@ __global__ __launch_bounds__(BL_NT) void k_debayer_simple(const uint16_t *__rest
     __syncthreads();
 
     if (INTER)
-        kstore_inter<BL_TW, BL_TH, BL_NT>(rgbs, x0, y0, W, H, inter);
+        kstore_inter<BL_TW, BL_TH, BL_NT>(rgbs, x0, y0, W, H, f.inter);
     else
-        kfinish<BL_TW, BL_TH, BL_NT>(rgbs, x0, y0, W, H, col, out);
+        kfinish<BL_TW, BL_TH, BL_NT>(rgbs, x0, y0, W, H, f.col, f.out);
 }
 
 /* --------------------------------------------------------------- RCD */
@@ -238,27 +241,32 @@ KDEV float rcd_ratio(float a, float b)
  * colour estimate 1, output 0. Every stage computes at the mirrored
  * position of each apron position, from the previous stage's values at
  * that position's own (virtual) neighbourhood. */
+struct SmRcd {
+    float sv[RcStage<5>::N];
+    float slp[RcStage<4>::N];
+    /* gradients: vertical, horizontal, NE-SW, NW-SE; reused as the RGB
+     * tile once the weights are formed */
+    float sg[4][RcStage<3>::N];
+    float swv[RcStage<2>::N];
+    float swp[RcStage<1>::N];
+    float sgp[RcStage<2>::N];
+    float scp[RcStage<1>::N];
+};
+
 template <bool INTER>
-__global__ __launch_bounds__(RC_NT) void k_debayer_rcd(const uint16_t *__restrict__ src,
-                                                       int src_pitch, int W, int H, KPrep prep,
-                                                       KColor col, KOut out, KInter inter)
+KDEV void dev_debayer_rcd(SmRcd &sm, const KFrame &f, int x0, int y0)
 {
     typedef RcStage<5> S5;
     typedef RcStage<4> S4;
     typedef RcStage<3> S3;
     typedef RcStage<2> S2;
     typedef RcStage<1> S1;
-    __shared__ float sv[S5::N];
-    __shared__ float slp[S4::N];
-    /* gradients: vertical, horizontal, NE-SW, NW-SE; reused as the RGB
-     * tile once the weights are formed */
-    __shared__ float sg[4][S3::N];
-    __shared__ float swv[S2::N];
-    __shared__ float swp[S1::N];
-    __shared__ float sgp[S2::N];
-    __shared__ float scp[S1::N];
+    float *sv = sm.sv, *slp = sm.slp, *swv = sm.swv, *swp = sm.swp, *sgp = sm.sgp, *scp = sm.scp;
+    float (&sg)[4][S3::N] = sm.sg;
+    const uint16_t *__restrict__ src = f.src;
+    const int src_pitch = f.src_pitch, W = f.W, H = f.H;
+    const KPrep &prep = f.prep;
     const int tid = threadIdx.x;
-    const int x0 = blockIdx.x * RC_TW, y0 = blockIdx.y * RC_TH;
     const int xe = min(x0 + RC_TW, W), ye = min(y0 + RC_TH, H);
     const int tw = xe - x0, th = ye - y0;
 
@@ -436,9 +444,9 @@ __global__ __launch_bounds__(RC_NT) void k_debayer_rcd(const uint16_t *__restric
         __syncthreads();
 
         if (INTER)
-            kstore_inter<RC_TW, RC_TH, RC_NT>(rgbs, x0, y0, W, H, inter);
+            kstore_inter<RC_TW, RC_TH, RC_NT>(rgbs, x0, y0, W, H, f.inter);
         else
-            kfinish<RC_TW, RC_TH, RC_NT>(rgbs, x0, y0, W, H, col, out);
+            kfinish<RC_TW, RC_TH, RC_NT>(rgbs, x0, y0, W, H, f.col, f.out);
     }
 #undef AT
 #undef V
@@ -454,10 +462,11 @@ __global__ __launch_bounds__(RC_NT) void k_debayer_rcd(const uint16_t *__restric
 
 /* lib/ref/ref_chroma.c:86-117: 9x9 joint bilateral on the quarter grid,
  * guided by G, one reflection at the borders, summed row by row. */
-__global__ __launch_bounds__(CB_TX * CB_TY) void k_chroma_blur(KInter it)
+KDEV void dev_chroma_blur(const KFrame &f, int qx0, int qy0)
 {
-    const int qx = blockIdx.x * CB_TX + threadIdx.x % CB_TX;
-    const int qy = blockIdx.y * CB_TY + threadIdx.x / CB_TX;
+    const KInter &it = f.inter;
+    const int qx = qx0 + threadIdx.x % CB_TX;
+    const int qy = qy0 + threadIdx.x / CB_TX;
     const size_t qplane = (size_t)it.wq * it.hq;
     const float *d0 = it.down, *d1 = it.down + qplane, *d2 = it.down + 2 * qplane;
     float c0, c1, c2, level, inv_range, inv_chroma;
@@ -506,12 +515,17 @@ __global__ __launch_bounds__(CB_TX * CB_TY) void k_chroma_blur(KInter it)
 /* lib/ref/ref_chroma.c:119-155: the difference of the filtered and the
  * unfiltered band, bilinearly up to every pixel and blended by the
  * strength, into R and B; then the histogram and the colour stage. */
-__global__ __launch_bounds__(CU_NT) void k_chroma_up_finish(KInter it, float strength, int W,
-                                                            int H, KColor col, KOut out)
+struct SmUp {
+    float rgbs[3][CU_TH][CU_TW];
+};
+
+KDEV void dev_chroma_up_finish(SmUp &sm, const KFrame &f, int x0, int y0)
 {
-    __shared__ float rgbs[3][CU_TH][CU_TW];
+    float (&rgbs)[3][CU_TH][CU_TW] = sm.rgbs;
+    const KInter &it = f.inter;
+    const float strength = f.chroma_strength;
+    const int W = f.W, H = f.H;
     const int tid = threadIdx.x;
-    const int x0 = blockIdx.x * CU_TW, y0 = blockIdx.y * CU_TH;
     const size_t plane = (size_t)W * H, qplane = (size_t)it.wq * it.hq;
 
 #pragma unroll
@@ -566,10 +580,102 @@ __global__ __launch_bounds__(CU_NT) void k_chroma_up_finish(KInter it, float str
     }
     __syncthreads();
 
-    kfinish<CU_TW, CU_TH, CU_NT>(rgbs, x0, y0, W, H, col, out);
+    kfinish<CU_TW, CU_TH, CU_NT>(rgbs, x0, y0, W, H, f.col, f.out);
 }
 
-/* ------------------------------------------------------------- misc */
+/* ------------------------------------------------- plain and batched */
+
+template <int METHOD, bool INTER>
+__global__ __launch_bounds__(BL_NT) void k_debayer_simple(KFrame f)
+{
+    __shared__ SmSimple sm;
+    dev_debayer_simple<METHOD, INTER>(sm, f, blockIdx.x * BL_TW, blockIdx.y * BL_TH);
+}
+
+template <bool INTER>
+__global__ __launch_bounds__(RC_NT) void k_debayer_rcd(KFrame f)
+{
+    __shared__ SmRcd sm;
+    dev_debayer_rcd<INTER>(sm, f, blockIdx.x * RC_TW, blockIdx.y * RC_TH);
+}
+
+__global__ __launch_bounds__(CB_TX * CB_TY) void k_chroma_blur(KFrame f)
+{
+    dev_chroma_blur(f, blockIdx.x * CB_TX, blockIdx.y * CB_TY);
+}
+
+__global__ __launch_bounds__(CU_NT) void k_chroma_up_finish(KFrame f)
+{
+    __shared__ SmUp sm;
+    dev_chroma_up_finish(sm, f, blockIdx.x * CU_TW, blockIdx.y * CU_TH);
+}
+
+/* Persistent forms: workgroup b takes tiles b, b + grid, b + 2 grid, ...
+ * of the work-list; every workgroup leaves the loop after the last tile.
+ * The barrier at the top of each iteration keeps a tile's LDS from being
+ * overwritten while another wave still reads the previous one. */
+template <int METHOD, bool INTER>
+__global__ __launch_bounds__(BL_NT) void kb_debayer_simple(const KFrame *__restrict__ frames,
+                                                           const uint32_t *__restrict__ tiles,
+                                                           uint32_t ntiles)
+{
+    __shared__ SmSimple sm;
+
+    for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const uint32_t d = tiles[t];
+        const KFrame &f = frames[KTILE_FRAME(d)];
+        const int ntx = (f.W + BL_TW - 1) / BL_TW, i = KTILE_IDX(d);
+
+        __syncthreads();
+        dev_debayer_simple<METHOD, INTER>(sm, f, (i % ntx) * BL_TW, (i / ntx) * BL_TH);
+    }
+}
+
+template <bool INTER>
+__global__ __launch_bounds__(RC_NT) void kb_debayer_rcd(const KFrame *__restrict__ frames,
+                                                        const uint32_t *__restrict__ tiles,
+                                                        uint32_t ntiles)
+{
+    __shared__ SmRcd sm;
+
+    for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const uint32_t d = tiles[t];
+        const KFrame &f = frames[KTILE_FRAME(d)];
+        const int ntx = (f.W + RC_TW - 1) / RC_TW, i = KTILE_IDX(d);
+
+        __syncthreads();
+        dev_debayer_rcd<INTER>(sm, f, (i % ntx) * RC_TW, (i / ntx) * RC_TH);
+    }
+}
+
+__global__ __launch_bounds__(CB_TX * CB_TY) void kb_chroma_blur(const KFrame *__restrict__ frames,
+                                                                const uint32_t *__restrict__ tiles,
+                                                                uint32_t ntiles)
+{
+    for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const uint32_t d = tiles[t];
+        const KFrame &f = frames[KTILE_FRAME(d)];
+        const int ntx = (f.inter.wq + CB_TX - 1) / CB_TX, i = KTILE_IDX(d);
+
+        dev_chroma_blur(f, (i % ntx) * CB_TX, (i / ntx) * CB_TY);
+    }
+}
+
+__global__ __launch_bounds__(CU_NT) void kb_chroma_up_finish(const KFrame *__restrict__ frames,
+                                                             const uint32_t *__restrict__ tiles,
+                                                             uint32_t ntiles)
+{
+    __shared__ SmUp sm;
+
+    for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const uint32_t d = tiles[t];
+        const KFrame &f = frames[KTILE_FRAME(d)];
+        const int ntx = (f.W + CU_TW - 1) / CU_TW, i = KTILE_IDX(d);
+
+        __syncthreads();
+        dev_chroma_up_finish(sm, f, (i % ntx) * CU_TW, (i / ntx) * CU_TH);
+    }
+}
 
 __global__ void k_zero_u32(uint32_t *p, size_t n)
 {
@@ -580,54 +686,100 @@ __global__ void k_zero_u32(uint32_t *p, size_t n)
 
 /* ---------------------------------------------------------- launchers */
 
+static_assert(BL_TW == KG_SIMPLE_TW && BL_TH == KG_SIMPLE_TH, "tile geometry");
+static_assert(RC_TW == KG_RCD_TW && RC_TH == KG_RCD_TH, "tile geometry");
+static_assert(CB_TX == KG_CB_T && CB_TY == KG_CB_T, "tile geometry");
+static_assert(CU_TW == KG_CU_TW && CU_TH == KG_CU_TH, "tile geometry");
+
 hipError_t craw_launch_zero_u32(uint32_t *p, size_t n, hipStream_t s)
 {
     hipLaunchKernelGGL(k_zero_u32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n);
     return hipGetLastError();
 }
 
-hipError_t craw_launch_debayer(int method, bool to_inter, const uint16_t *src, int src_pitch,
-                               int W, int H, const KPrep &prep, const KColor &col,
-                               const KOut &out, const KInter &inter, hipStream_t s)
+hipError_t craw_launch_debayer(bool to_inter, const KFrame &f, hipStream_t s)
 {
-    if (method == 2) {
-        dim3 grid((W + RC_TW - 1) / RC_TW, (H + RC_TH - 1) / RC_TH);
+    if (f.method == 2) {
+        dim3 grid((f.W + RC_TW - 1) / RC_TW, (f.H + RC_TH - 1) / RC_TH);
         if (to_inter)
-            hipLaunchKernelGGL(k_debayer_rcd<true>, grid, dim3(RC_NT), 0, s, src, src_pitch, W,
-                               H, prep, col, out, inter);
+            hipLaunchKernelGGL(k_debayer_rcd<true>, grid, dim3(RC_NT), 0, s, f);
         else
-            hipLaunchKernelGGL(k_debayer_rcd<false>, grid, dim3(RC_NT), 0, s, src, src_pitch, W,
-                               H, prep, col, out, inter);
+            hipLaunchKernelGGL(k_debayer_rcd<false>, grid, dim3(RC_NT), 0, s, f);
     } else {
-        dim3 grid((W + BL_TW - 1) / BL_TW, (H + BL_TH - 1) / BL_TH);
-        if (method == 0 && to_inter)
-            hipLaunchKernelGGL((k_debayer_simple<0, true>), grid, dim3(BL_NT), 0, s, src,
-                               src_pitch, W, H, prep, col, out, inter);
-        else if (method == 0)
-            hipLaunchKernelGGL((k_debayer_simple<0, false>), grid, dim3(BL_NT), 0, s, src,
-                               src_pitch, W, H, prep, col, out, inter);
+        dim3 grid((f.W + BL_TW - 1) / BL_TW, (f.H + BL_TH - 1) / BL_TH);
+        if (f.method == 0 && to_inter)
+            hipLaunchKernelGGL((k_debayer_simple<0, true>), grid, dim3(BL_NT), 0, s, f);
+        else if (f.method == 0)
+            hipLaunchKernelGGL((k_debayer_simple<0, false>), grid, dim3(BL_NT), 0, s, f);
         else if (to_inter)
-            hipLaunchKernelGGL((k_debayer_simple<1, true>), grid, dim3(BL_NT), 0, s, src,
-                               src_pitch, W, H, prep, col, out, inter);
+            hipLaunchKernelGGL((k_debayer_simple<1, true>), grid, dim3(BL_NT), 0, s, f);
         else
-            hipLaunchKernelGGL((k_debayer_simple<1, false>), grid, dim3(BL_NT), 0, s, src,
-                               src_pitch, W, H, prep, col, out, inter);
+            hipLaunchKernelGGL((k_debayer_simple<1, false>), grid, dim3(BL_NT), 0, s, f);
     }
     return hipGetLastError();
 }
 
-hipError_t craw_launch_chroma_blur(const KInter &it, hipStream_t s)
+hipError_t craw_launch_chroma_blur(const KFrame &f, hipStream_t s)
 {
-    dim3 grid((it.wq + CB_TX - 1) / CB_TX, (it.hq + CB_TY - 1) / CB_TY);
-    hipLaunchKernelGGL(k_chroma_blur, grid, dim3(CB_TX * CB_TY), 0, s, it);
+    dim3 grid((f.inter.wq + CB_TX - 1) / CB_TX, (f.inter.hq + CB_TY - 1) / CB_TY);
+    hipLaunchKernelGGL(k_chroma_blur, grid, dim3(CB_TX * CB_TY), 0, s, f);
     return hipGetLastError();
 }
 
-hipError_t craw_launch_chroma_up_finish(const KInter &it, float strength, int W, int H,
-                                        const KColor &col, const KOut &out, hipStream_t s)
+hipError_t craw_launch_chroma_up_finish(const KFrame &f, hipStream_t s)
 {
-    dim3 grid((W + CU_TW - 1) / CU_TW, (H + CU_TH - 1) / CU_TH);
-    hipLaunchKernelGGL(k_chroma_up_finish, grid, dim3(CU_NT), 0, s, it, strength, W, H, col,
-                       out);
+    dim3 grid((f.W + CU_TW - 1) / CU_TW, (f.H + CU_TH - 1) / CU_TH);
+    hipLaunchKernelGGL(k_chroma_up_finish, grid, dim3(CU_NT), 0, s, f);
+    return hipGetLastError();
+}
+
+hipError_t craw_launch_batch_debayer(int method, bool to_inter, const KFrame *frames,
+                                     const uint32_t *tiles, uint32_t ntiles, int grid,
+                                     hipStream_t s)
+{
+    if (!ntiles)
+        return hipSuccess;
+    if (method == 2) {
+        if (to_inter)
+            hipLaunchKernelGGL(kb_debayer_rcd<true>, dim3(grid), dim3(RC_NT), 0, s, frames, tiles,
+                               ntiles);
+        else
+            hipLaunchKernelGGL(kb_debayer_rcd<false>, dim3(grid), dim3(RC_NT), 0, s, frames,
+                               tiles, ntiles);
+    } else if (method == 0) {
+        if (to_inter)
+            hipLaunchKernelGGL((kb_debayer_simple<0, true>), dim3(grid), dim3(BL_NT), 0, s,
+                               frames, tiles, ntiles);
+        else
+            hipLaunchKernelGGL((kb_debayer_simple<0, false>), dim3(grid), dim3(BL_NT), 0, s,
+                               frames, tiles, ntiles);
+    } else {
+        if (to_inter)
+            hipLaunchKernelGGL((kb_debayer_simple<1, true>), dim3(grid), dim3(BL_NT), 0, s,
+                               frames, tiles, ntiles);
+        else
+            hipLaunchKernelGGL((kb_debayer_simple<1, false>), dim3(grid), dim3(BL_NT), 0, s,
+                               frames, tiles, ntiles);
+    }
+    return hipGetLastError();
+}
+
+hipError_t craw_launch_batch_chroma_blur(const KFrame *frames, const uint32_t *tiles,
+                                         uint32_t ntiles, int grid, hipStream_t s)
+{
+    if (!ntiles)
+        return hipSuccess;
+    hipLaunchKernelGGL(kb_chroma_blur, dim3(grid), dim3(CB_TX * CB_TY), 0, s, frames, tiles,
+                       ntiles);
+    return hipGetLastError();
+}
+
+hipError_t craw_launch_batch_chroma_up(const KFrame *frames, const uint32_t *tiles,
+                                       uint32_t ntiles, int grid, hipStream_t s)
+{
+    if (!ntiles)
+        return hipSuccess;
+    hipLaunchKernelGGL(kb_chroma_up_finish, dim3(grid), dim3(CU_NT), 0, s, frames, tiles,
+                       ntiles);
     return hipGetLastError();
 }

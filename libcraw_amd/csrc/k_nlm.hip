@@ -57,23 +57,32 @@ KDEV float kvar_decode(uint8_t c)
 }
 
 template <int MODE>
-__global__ __launch_bounds__(DN_NT) void k_denoise(KDenoise p, const uint16_t *__restrict__ src,
-                                                   uint16_t *__restrict__ dst, KDenoiseState st)
+struct SmDn {
+    static constexpr bool HISTORY = MODE & KDN_HISTORY;
+    static constexpr int PW = DN_TW + 2 * DN_R, PH = DN_TH + 2 * DN_R;
+    static constexpr int CW = PW + 2, CH = PH + 2;
+    float splane[PH * PW];
+    float svar[HISTORY ? PH * PW : 1];
+    float scur[HISTORY ? CH * CW : 1];
+    float stp[HISTORY ? CH * CW : 1];
+};
+
+template <int MODE>
+KDEV void dev_denoise(SmDn<MODE> &sm, const KFrame &f, int x0, int y0, int pl)
 {
     constexpr bool SPATIAL = MODE & KDN_SPATIAL;
     constexpr bool TEMPORAL = MODE & KDN_TEMPORAL;
     constexpr bool HISTORY = MODE & KDN_HISTORY;
-    constexpr int PW = DN_TW + 2 * DN_R, PH = DN_TH + 2 * DN_R;
-    constexpr int CW = PW + 2, CH = PH + 2;
-    __shared__ float splane[PH * PW];
-    __shared__ float svar[HISTORY ? PH * PW : 1];
-    __shared__ float scur[HISTORY ? CH * CW : 1];
-    __shared__ float stp[HISTORY ? CH * CW : 1];
-
+    constexpr int PW = SmDn<MODE>::PW;
+    constexpr int CW = SmDn<MODE>::CW;
+    float *splane = sm.splane, *svar = sm.svar, *scur = sm.scur, *stp = sm.stp;
+    const KDenoise &p = f.dn;
+    const KDenoiseState &st = f.st;
+    const uint16_t *__restrict__ src = f.dn_src;
+    uint16_t *__restrict__ dst = f.dn_dst;
     const int tid = threadIdx.x;
-    const int pl = blockIdx.z, px = pl & 1, py = pl >> 1;
+    const int px = pl & 1, py = pl >> 1;
     const int w2 = p.w2, h2 = p.h2;
-    const int x0 = blockIdx.x * DN_TW, y0 = blockIdx.y * DN_TH;
     const int xe = min(x0 + DN_TW, w2), ye = min(y0 + DN_TH, h2);
     const int tw = xe - x0, th = ye - y0;
     const float black = ksel4(pl, p.black[0], p.black[1], p.black[2], p.black[3]);
@@ -230,31 +239,68 @@ __global__ __launch_bounds__(DN_NT) void k_denoise(KDenoise p, const uint16_t *_
 #undef RAW
 }
 
-hipError_t craw_launch_denoise(int mode, bool fp16, const KDenoise &p, const uint16_t *src,
-                               uint16_t *dst, const KDenoiseState &st, hipStream_t s)
+template <int MODE>
+__global__ __launch_bounds__(DN_NT) void k_denoise(KFrame f)
 {
-    dim3 grid((p.w2 + DN_TW - 1) / DN_TW, (p.h2 + DN_TH - 1) / DN_TH, 4);
+    __shared__ SmDn<MODE> sm;
+    dev_denoise<MODE>(sm, f, blockIdx.x * DN_TW, blockIdx.y * DN_TH, blockIdx.z);
+}
 
-    (void)fp16;
+/* Persistent form over a work-list of (frame, plane, tile) entries. */
+template <int MODE>
+__global__ __launch_bounds__(DN_NT) void kb_denoise(const KFrame *__restrict__ frames,
+                                                    const uint32_t *__restrict__ tiles,
+                                                    uint32_t ntiles)
+{
+    __shared__ SmDn<MODE> sm;
+
+    for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const uint32_t d = tiles[t];
+        const KFrame &f = frames[KTILE_FRAME(d)];
+        const int ntx = (f.dn.w2 + DN_TW - 1) / DN_TW, nty = (f.dn.h2 + DN_TH - 1) / DN_TH;
+        const int i = KTILE_IDX(d), pl = i / (ntx * nty), r = i % (ntx * nty);
+
+        __syncthreads();
+        dev_denoise<MODE>(sm, f, (r % ntx) * DN_TW, (r / ntx) * DN_TH, pl);
+    }
+}
+
+static_assert(DN_TW == KG_DN_TW && DN_TH == KG_DN_TH, "tile geometry");
+
+#define DN_MODES(X)                                                                        \
+    X(KDN_SPATIAL) X(KDN_TEMPORAL) X(KDN_TEMPORAL | KDN_HISTORY)                           \
+    X(KDN_SPATIAL | KDN_TEMPORAL) X(KDN_SPATIAL | KDN_TEMPORAL | KDN_HISTORY)
+
+hipError_t craw_launch_denoise(int mode, const KFrame &f, hipStream_t s)
+{
+    dim3 grid((f.dn.w2 + DN_TW - 1) / DN_TW, (f.dn.h2 + DN_TH - 1) / DN_TH, 4);
+
     switch (mode) {
-    case KDN_SPATIAL:
-        hipLaunchKernelGGL(k_denoise<KDN_SPATIAL>, grid, dim3(DN_NT), 0, s, p, src, dst, st);
+#define X(M)                                                                               \
+    case M:                                                                                \
+        hipLaunchKernelGGL(k_denoise<M>, grid, dim3(DN_NT), 0, s, f);                      \
         break;
-    case KDN_TEMPORAL:
-        hipLaunchKernelGGL(k_denoise<KDN_TEMPORAL>, grid, dim3(DN_NT), 0, s, p, src, dst, st);
+    DN_MODES(X)
+#undef X
+    default:
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t craw_launch_batch_denoise_exact(int mode, const KFrame *frames, const uint32_t *tiles,
+                                           uint32_t ntiles, int grid, hipStream_t s)
+{
+    if (!ntiles)
+        return hipSuccess;
+    switch (mode) {
+#define X(M)                                                                               \
+    case M:                                                                                \
+        hipLaunchKernelGGL(kb_denoise<M>, dim3(grid), dim3(DN_NT), 0, s, frames, tiles,     \
+                           ntiles);                                                        \
         break;
-    case KDN_TEMPORAL | KDN_HISTORY:
-        hipLaunchKernelGGL(k_denoise<KDN_TEMPORAL | KDN_HISTORY>, grid, dim3(DN_NT), 0, s, p,
-                           src, dst, st);
-        break;
-    case KDN_SPATIAL | KDN_TEMPORAL:
-        hipLaunchKernelGGL(k_denoise<KDN_SPATIAL | KDN_TEMPORAL>, grid, dim3(DN_NT), 0, s, p,
-                           src, dst, st);
-        break;
-    case KDN_SPATIAL | KDN_TEMPORAL | KDN_HISTORY:
-        hipLaunchKernelGGL(k_denoise<KDN_SPATIAL | KDN_TEMPORAL | KDN_HISTORY>, grid,
-                           dim3(DN_NT), 0, s, p, src, dst, st);
-        break;
+    DN_MODES(X)
+#undef X
     default:
         return hipErrorInvalidValue;
     }

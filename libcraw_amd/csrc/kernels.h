@@ -1,5 +1,12 @@
 /*
- * libcraw for MI355X: the kernel launchers the host pipeline calls.
+ * libcraw for MI355X: the per-frame constants the kernels read, and the
+ * launchers the host calls.
+ *
+ * Every stage is a per-tile device function. The plain form launches it
+ * over one frame's tiles (tile = blockIdx); the batched form launches it
+ * persistently over a work-list of tiles drawn from many frames of
+ * different sizes, each workgroup taking tiles until the list is done, and
+ * each tile reading its frame's constants from a device-resident table.
  */
 #ifndef CRAW_KERNELS_H
 #define CRAW_KERNELS_H
@@ -46,21 +53,128 @@ struct KDenoiseState {
     uint8_t *next_var;
 };
 
+/* Everything one frame's kernels read. */
+struct KFrame {
+    int W, H;
+    const uint16_t *src;         /* the mosaic the debayer reads */
+    int src_pitch;               /* uint16 elements */
+    int method;                  /* 0 bilinear, 1 MHC, 2 RCD */
+    KPrep prep;
+    KColor col;
+    KOut out;
+    KInter inter;
+    float chroma_strength;
+    /* denoise: input mosaic, constants, state, output mosaic */
+    const uint16_t *dn_src;
+    uint16_t *dn_dst;
+    KDenoise dn;
+    KDenoiseState st;
+};
+
+/* A tile of the batched form's work-list: frame index in the top 8 bits,
+ * tile index within the frame's grid for the stage in the low 24. */
+#define KTILE(frame, idx) (((uint32_t)(frame) << 24) | (uint32_t)(idx))
+#define KTILE_FRAME(t) ((int)((t) >> 24))
+#define KTILE_IDX(t) ((int)((t) & 0xffffff))
+
+/* Tile geometry of each stage (frame pixels, or plane samples for the
+ * denoise), for building work-lists on the host. */
+#define KG_SIMPLE_TW 64
+#define KG_SIMPLE_TH 16
+#define KG_RCD_TW 32
+#define KG_RCD_TH 16
+#define KG_CB_T 16
+#define KG_CU_TW 64
+#define KG_CU_TH 16
+#define KG_DN_TW 32
+#define KG_DN_TH 16
+#define KG_FT_TW 28
+#define KG_FT_TH 64
+
+enum KStage {
+    KSTAGE_DENOISE = 0,
+    KSTAGE_DEBAYER,
+    KSTAGE_CHROMA_BLUR,
+    KSTAGE_CHROMA_UP,
+    KSTAGE_NB,
+};
+
+/* Tile geometry of `stage`: tw x th samples of a grid `div` times coarser
+ * than the frame (2: the four planes of the mosaic; 4: the quarter grid of
+ * the chroma stage), `planes` grids per frame. `method` picks the
+ * debayer's tile, `fast` the fast denoise's. */
+static inline void craw_stage_geom(int stage, int method, bool fast, int *tw, int *th, int *div,
+                                   int *planes)
+{
+    *tw = *th = *div = *planes = 1;
+    switch (stage) {
+    case KSTAGE_DENOISE:
+        *tw = fast ? KG_FT_TW : KG_DN_TW;
+        *th = fast ? KG_FT_TH : KG_DN_TH;
+        *div = 2;
+        *planes = 4;
+        break;
+    case KSTAGE_DEBAYER:
+        *tw = method == 2 ? KG_RCD_TW : KG_SIMPLE_TW;
+        *th = method == 2 ? KG_RCD_TH : KG_SIMPLE_TH;
+        break;
+    case KSTAGE_CHROMA_BLUR:
+        *tw = *th = KG_CB_T;
+        *div = 4;
+        break;
+    case KSTAGE_CHROMA_UP:
+        *tw = KG_CU_TW;
+        *th = KG_CU_TH;
+        break;
+    }
+}
+
+/* Number of tiles of `stage` for a frame of W x H (even W and H; the
+ * quarter grid rounds up). */
+static inline int craw_stage_tiles(int stage, int W, int H, int method, bool fast, int *ntx)
+{
+    int tw, th, div, planes, w, h;
+
+    craw_stage_geom(stage, method, fast, &tw, &th, &div, &planes);
+    w = (W + div - 1) / div;
+    h = (H + div - 1) / div;
+    *ntx = (w + tw - 1) / tw;
+    return planes * *ntx * ((h + th - 1) / th);
+}
+
 hipError_t craw_launch_zero_u32(uint32_t *p, size_t n, hipStream_t s);
 
-/* method: 0 bilinear, 1 MHC, 2 RCD. to_inter: write the chroma stage's
- * inputs instead of finishing the frame. */
-hipError_t craw_launch_debayer(int method, bool to_inter, const uint16_t *src, int src_pitch,
-                               int W, int H, const KPrep &prep, const KColor &col,
-                               const KOut &out, const KInter &inter, hipStream_t s);
+/* Plain form: one frame, `f` by value. */
+hipError_t craw_launch_debayer(bool to_inter, const KFrame &f, hipStream_t s);
+hipError_t craw_launch_chroma_blur(const KFrame &f, hipStream_t s);
+hipError_t craw_launch_chroma_up_finish(const KFrame &f, hipStream_t s);
 
-hipError_t craw_launch_chroma_blur(const KInter &it, hipStream_t s);
+/* mode: KDN_* bits. The FP32 form that reproduces the CPU reference bit
+ * for bit. */
+hipError_t craw_launch_denoise(int mode, const KFrame &f, hipStream_t s);
 
-hipError_t craw_launch_chroma_up_finish(const KInter &it, float strength, int W, int H,
-                                        const KColor &col, const KOut &out, hipStream_t s);
+/* The throughput forms: FP32 with reordered sums and the hardware exp2, or
+ * with fp16 the packed-FP16 distances and weight arguments. */
+hipError_t craw_launch_denoise_fast(int mode, bool fp16, const KFrame &f, hipStream_t s);
 
-/* mode: KDN_* bits. fp16: the packed-FP16 distance form. */
-hipError_t craw_launch_denoise(int mode, bool fp16, const KDenoise &p, const uint16_t *src,
-                               uint16_t *dst, const KDenoiseState &st, hipStream_t s);
+/* Batched form: `frames` is a device table, `tiles` a device work-list of
+ * `ntiles` KTILE entries for the stage, `grid` the number of persistent
+ * workgroups. All frames share the method, the denoise mode and whether
+ * the chroma stage runs. */
+hipError_t craw_launch_batch_denoise(int mode, int form, const KFrame *frames,
+                                     const uint32_t *tiles, uint32_t ntiles, int grid,
+                                     hipStream_t s);
+hipError_t craw_launch_batch_debayer(int method, bool to_inter, const KFrame *frames,
+                                     const uint32_t *tiles, uint32_t ntiles, int grid,
+                                     hipStream_t s);
+hipError_t craw_launch_batch_chroma_blur(const KFrame *frames, const uint32_t *tiles,
+                                         uint32_t ntiles, int grid, hipStream_t s);
+hipError_t craw_launch_batch_chroma_up(const KFrame *frames, const uint32_t *tiles,
+                                       uint32_t ntiles, int grid, hipStream_t s);
+
+/* denoise forms for the batched launcher */
+#define KFORM_EXACT 0
+#define KFORM_FAST  1
+#define KFORM_FP16  2
 
 #endif /* CRAW_KERNELS_H */

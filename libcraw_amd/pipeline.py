@@ -2,7 +2,8 @@
 
 Same operations, argument meaning and error codes as the C API, which it
 calls through ctypes: craw_context_create, craw_pipeline_create, record,
-get_awb_histogram(_n), get_stage_times(_n). Errors come back as the C API's
+get_awb_histogram(_n), get_stage_times(_n), and the batched form of
+include/craw/batch.h. Errors come back as the C API's
 negative errno; `CrawError` carries it.
 """
 import ctypes
@@ -116,6 +117,106 @@ class Pipeline:
     def close(self):
         if self.handle.value:
             self._lib.craw_pipeline_free(ctypes.byref(self.handle))
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def batch_params(max_frames, max_width, max_height, max_pixels, debayer=capi.CRAW_DEBAYER_RCD,
+                 out_space=capi.CRAW_CS_XYZ_D65,
+                 out_format=capi.CRAW_FORMAT_R32G32B32A32_SFLOAT, flags=0, use_graphs=False):
+    bp = capi.CRAWBatchParams()
+    capi.lib().craw_batch_params_init(ctypes.byref(bp))
+    bp.max_frames, bp.max_width, bp.max_height = max_frames, max_width, max_height
+    bp.max_pixels = max_pixels
+    bp.debayer, bp.out_space, bp.out_format = debayer, out_space, out_format
+    bp.flags, bp.use_graphs = flags, 1 if use_graphs else 0
+    return bp
+
+
+class TemporalState:
+    """One clip stream's temporal denoise state (craw_temporal_state_*)."""
+
+    def __init__(self, ctx, width, height):
+        self._lib = capi.lib()
+        self.ctx = ctx
+        self.handle = ctypes.c_void_p()
+        _ok(self._lib.craw_temporal_state_create(ctx.handle, width, height,
+                                                 ctypes.byref(self.handle)),
+            "craw_temporal_state_create")
+
+    def reset(self):
+        self._lib.craw_temporal_state_reset(self.handle)
+
+    def close(self):
+        if self.handle.value:
+            self._lib.craw_temporal_state_free(ctypes.byref(self.handle))
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def batch_frame(src, dst, fp, state=None):
+    f = capi.CRAWBatchFrame()
+    f.src, f.dst = src, dst
+    f.params = ctypes.pointer(fp)
+    f.state = state.handle.value if state is not None else None
+    f._refs = (fp, state)
+    return f
+
+
+class Batch:
+    """Many frames of any sizes per recording (include/craw/batch.h)."""
+
+    def __init__(self, ctx, params):
+        self._lib = capi.lib()
+        self.ctx = ctx
+        self.params = params
+        self.handle = ctypes.c_void_p()
+        _ok(self._lib.craw_batch_create(ctx.handle, ctypes.byref(params),
+                                        ctypes.byref(self.handle)), "craw_batch_create")
+
+    @staticmethod
+    def _array(frames):
+        arr = (capi.CRAWBatchFrame * max(len(frames), 1))()
+        for i, f in enumerate(frames):
+            arr[i] = f
+        return arr
+
+    def record(self, stream, frames):
+        """Returns the C API's return code (0 or a negative errno)."""
+        return self._lib.craw_batch_record(self.handle, stream.handle if stream else None,
+                                           self._array(frames), len(frames))
+
+    def autotune(self, stream, frames, iterations=3):
+        return _ok(self._lib.craw_batch_autotune(self.handle, stream.handle if stream else None,
+                                                 self._array(frames), len(frames), iterations),
+                   "craw_batch_autotune")
+
+    def histogram(self, index):
+        h = capi.CRAWAwbHistogram()
+        _ok(self._lib.craw_batch_get_awb_histogram(self.handle, index, ctypes.byref(h)),
+            "craw_batch_get_awb_histogram")
+        n = h.bins * h.bins
+        return tuple(np.ctypeslib.as_array(p, shape=(n,)).copy()
+                     for p in (h.count, h.weight, h.weight_bright))
+
+    def stage_times(self):
+        out = (capi.CRAWStageTime * 8)()
+        n = self._lib.craw_batch_get_stage_times(self.handle, out, 8)
+        if n < 0:
+            raise CrawError("craw_batch_get_stage_times", n)
+        return [(out[i].name.decode(), out[i].count, out[i].ms) for i in range(min(n, 8))]
+
+    def close(self):
+        if self.handle.value:
+            self._lib.craw_batch_free(ctypes.byref(self.handle))
 
     def __del__(self):
         try:

@@ -1,0 +1,246 @@
+"""The batched form (include/craw/batch.h) against the plain pipeline.
+
+Bar: every frame of a batch recording equal, bit for bit, to what
+craw_pipeline_record() develops from the same frame with the same
+parameters (output and AWB histogram), whichever frames share the
+recording, in every NLM form, with graphs on or off, and after the
+autotuner has changed the launch shapes; temporal denoise per clip stream
+equal to a plain pipeline running that clip alone. The plain pipeline is
+itself bit-exact to the oracle (test_gpu_parity.py); one case here also
+checks a batch frame against the oracle directly.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from libcraw_amd import capi, hip, scene
+from libcraw_amd.pipeline import (Batch, TemporalState, batch_frame, batch_params,
+                                  frame_params, image)
+
+from gpu_util import Runner, assert_bit_exact, context, half_to_float
+
+pytestmark = pytest.mark.gpu
+
+F32 = capi.CRAW_FORMAT_R32G32B32A32_SFLOAT
+F16 = capi.CRAW_FORMAT_R16G16B16A16_SFLOAT
+FULL = (capi.CRAW_PIPELINE_FLAG_DENOISE | capi.CRAW_PIPELINE_FLAG_CHROMA_DENOISE |
+        capi.CRAW_PIPELINE_FLAG_AWB_HISTOGRAM)
+# uneven sizes: none a multiple of every tile, the smallest near the minimum
+SIZES = [(256, 160), (96, 64), (202, 122), (40, 24), (130, 66)]
+
+
+class Frames:
+    """Device source and destination buffers for a list of frame sizes."""
+
+    def __init__(self, sizes, fmt=F32):
+        self.sizes, self.fmt = sizes, fmt
+        bpp = 16 if fmt == F32 else 8
+        self.src = [hip.DeviceBuffer(w * h * 2) for w, h in sizes]
+        self.dst = [hip.DeviceBuffer(w * h * bpp) for w, h in sizes]
+
+    def upload(self, mosaics):
+        for buf, m in zip(self.src, mosaics):
+            buf.upload(m)
+
+    def images(self, i):
+        w, h = self.sizes[i]
+        return (image(self.src[i], w, h, capi.CRAW_FORMAT_R16_UNORM),
+                image(self.dst[i], w, h, self.fmt))
+
+    def output(self, i):
+        w, h = self.sizes[i]
+        if self.fmt == F32:
+            return self.dst[i].download(np.float32, (h, w, 4))
+        return half_to_float(self.dst[i].download(np.uint16, (h, w, 4)))
+
+
+def plain(size, mosaic, fp, flags, debayer=capi.CRAW_DEBAYER_RCD, half=False):
+    w, h = size
+    run = Runner(w, h, debayer=debayer, flags=flags, half=half)
+    assert run.run(mosaic, fp) == 0
+    hist = run.pipe.histogram() if flags & capi.CRAW_PIPELINE_FLAG_AWB_HISTOGRAM else None
+    return run.output(), hist
+
+
+def mosaics(sizes, seed):
+    return [scene.make_frame(w, h, seed=seed + i) for i, (w, h) in enumerate(sizes)]
+
+
+def params_for(prof, i):
+    # different strengths per frame: the frame table carries each frame's own
+    return frame_params(prof, denoise_strength=0.8 + 0.15 * (i % 3),
+                        denoise_chroma_strength=0.5 + 0.1 * (i % 4),
+                        exposure_stops=0.25 * (i % 2))
+
+
+def new_batch(sizes, flags, debayer=capi.CRAW_DEBAYER_RCD, graphs=False, fmt=F32,
+              max_frames=None):
+    mw = max(w for w, _ in sizes)
+    mh = max(h for _, h in sizes)
+    bp = batch_params(max_frames or len(sizes), mw, mh, sum(w * h for w, h in sizes),
+                      debayer=debayer, out_format=fmt, flags=flags, use_graphs=graphs)
+    return Batch(context(), bp)
+
+
+def check_against_plain(batch, frames, fps, mos, flags, debayer, what, half=False):
+    for i, size in enumerate(frames.sizes):
+        ref, hist = plain(size, mos[i], fps[i], flags, debayer=debayer, half=half)
+        assert_bit_exact(frames.output(i), ref, "%s frame %d %s" % (what, i, size))
+        if hist is not None:
+            got = batch.histogram(i)
+            for name, a, b in zip(("count", "weight", "bright"), got, hist):
+                assert np.array_equal(a, b), "%s frame %d: histogram %s differs" % (
+                    what, i, name)
+
+
+@pytest.mark.parametrize("debayer", [capi.CRAW_DEBAYER_BILINEAR, capi.CRAW_DEBAYER_MHC,
+                                     capi.CRAW_DEBAYER_RCD])
+def test_uneven_frames_equal_plain(gpu_available, debayer):
+    prof = scene.make_profile()
+    fr = Frames(SIZES)
+    mos = mosaics(SIZES, 40 + debayer)
+    fr.upload(mos)
+    fps = [params_for(prof, i) for i in range(len(SIZES))]
+    b = new_batch(SIZES, FULL, debayer=debayer)
+    stream = hip.Stream()
+    assert b.record(stream, [batch_frame(*fr.images(i), fps[i]) for i in range(len(SIZES))]) == 0
+    stream.synchronize()
+    check_against_plain(b, fr, fps, mos, FULL, debayer, "debayer %d" % debayer)
+
+
+def test_batch_frame_equals_oracle(gpu_available):
+    prof = scene.make_profile()
+    sizes = [(128, 96), (72, 40)]
+    fr = Frames(sizes)
+    mos = mosaics(sizes, 7)
+    fr.upload(mos)
+    fps = [params_for(prof, i) for i in range(2)]
+    b = new_batch(sizes, FULL)
+    stream = hip.Stream()
+    assert b.record(stream, [batch_frame(*fr.images(i), fps[i]) for i in range(2)]) == 0
+    stream.synchronize()
+    for i, (w, h) in enumerate(sizes):
+        run = Runner(w, h, flags=FULL)
+        ref = oracle.f32d.process(mos[i], run.pp, fps[i], want_hist=True)
+        assert ref["ret"] == 0
+        assert_bit_exact(fr.output(i), ref["out"], "frame %d vs oracle" % i)
+        for a, r in zip(b.histogram(i), ref["hist"]):
+            assert np.array_equal(a, r)
+
+
+@pytest.mark.parametrize("form", ["fast", "fp16"])
+def test_nlm_forms_equal_plain(gpu_available, form):
+    extra = (capi.CRAW_PIPELINE_FLAG_NLM_FAST if form == "fast"
+             else capi.CRAW_PIPELINE_FLAG_NLM_FP16)
+    flags = FULL | extra
+    prof = scene.make_profile()
+    fr = Frames(SIZES, fmt=F16)
+    mos = mosaics(SIZES, 90)
+    fr.upload(mos)
+    fps = [params_for(prof, i) for i in range(len(SIZES))]
+    b = new_batch(SIZES, flags, fmt=F16)
+    stream = hip.Stream()
+    assert b.record(stream, [batch_frame(*fr.images(i), fps[i]) for i in range(len(SIZES))]) == 0
+    stream.synchronize()
+    check_against_plain(b, fr, fps, mos, flags, capi.CRAW_DEBAYER_RCD, form, half=True)
+
+
+def test_graph_replays_follow_new_inputs(gpu_available):
+    prof = scene.make_profile()
+    sizes = SIZES[:3]
+    fr = Frames(sizes)
+    fps = [params_for(prof, i) for i in range(3)]
+    b = new_batch(sizes, FULL, graphs=True)
+    stream = hip.Stream()
+    frames = [batch_frame(*fr.images(i), fps[i]) for i in range(3)]
+    for rep in range(3):
+        mos = mosaics(sizes, 200 + 10 * rep)
+        fr.upload(mos)
+        assert b.record(stream, frames) == 0
+        stream.synchronize()
+        check_against_plain(b, fr, fps, mos, FULL, capi.CRAW_DEBAYER_RCD, "replay %d" % rep)
+    # a different shape captures a new graph; the old shape still replays right
+    assert b.record(stream, frames[:2]) == 0
+    stream.synchronize()
+    mos = mosaics(sizes, 300)
+    fr.upload(mos)
+    assert b.record(stream, frames) == 0
+    stream.synchronize()
+    check_against_plain(b, fr, fps, mos, FULL, capi.CRAW_DEBAYER_RCD, "after reshape")
+
+
+def test_temporal_streams(gpu_available):
+    """Two clips of different sizes, interleaved in one batch per time step,
+    each equal to a plain pipeline developing that clip alone; a reset and a
+    replay on one clip leave the other untouched."""
+    flags = capi.CRAW_PIPELINE_FLAG_DENOISE | capi.CRAW_PIPELINE_FLAG_TEMPORAL_DENOISE
+    sizes = [(128, 96), (96, 64)]
+    prof = scene.make_profile()
+    fr = Frames(sizes)
+    b = new_batch(sizes, flags)
+    states = [TemporalState(context(), w, h) for w, h in sizes]
+    runs = [Runner(w, h, flags=flags) for w, h in sizes]
+    stream = hip.Stream()
+    # (clip 0 frame flags, clip 1 frame flags) per step
+    steps = [(0, 0), (0, 0), ("reset", 0), (0, "replay"), (0, 0)]
+    for t, kinds in enumerate(steps):
+        mos = [scene.clip_frame(w, h, t, seed=60 + c) for c, (w, h) in enumerate(sizes)]
+        fr.upload(mos)
+        fps = []
+        for c, kind in enumerate(kinds):
+            fp = frame_params(prof, denoise_strength=1.0, denoise_temporal_strength=0.6)
+            if kind == "reset":
+                fp.flags |= capi.CRAW_FRAME_FLAG_TEMPORAL_RESET
+            if kind == "replay":
+                fp.denoise_temporal_replay = 1
+            fps.append(fp)
+        assert b.record(stream, [batch_frame(*fr.images(c), fps[c], states[c])
+                                 for c in range(2)]) == 0
+        stream.synchronize()
+        for c in range(2):
+            assert runs[c].run(mos[c], fps[c]) == 0
+            assert_bit_exact(fr.output(c), runs[c].output(), "step %d clip %d" % (t, c))
+
+
+def test_autotune_keeps_outputs(gpu_available):
+    prof = scene.make_profile()
+    fr = Frames(SIZES)
+    mos = mosaics(SIZES, 500)
+    fr.upload(mos)
+    fps = [params_for(prof, i) for i in range(len(SIZES))]
+    b = new_batch(SIZES, FULL | capi.CRAW_PIPELINE_FLAG_PROFILE)
+    stream = hip.Stream()
+    frames = [batch_frame(*fr.images(i), fps[i]) for i in range(len(SIZES))]
+    us = b.autotune(stream, frames, iterations=2)
+    assert us > 0
+    assert b.record(stream, frames) == 0
+    stream.synchronize()
+    names = [n for n, _, _ in b.stage_times()]
+    assert "rcd" in names and "denoise_nlm" in names
+    check_against_plain(b, fr, fps, mos, FULL, capi.CRAW_DEBAYER_RCD, "autotuned")
+
+
+def test_limits_and_errors(gpu_available):
+    prof = scene.make_profile()
+    sizes = [(64, 48), (64, 48)]
+    fr = Frames(sizes + [(128, 96)])
+    b = new_batch(sizes, capi.CRAW_PIPELINE_FLAG_TEMPORAL_DENOISE, max_frames=2)
+    stream = hip.Stream()
+    fp = frame_params(prof, denoise_temporal_strength=0.5)
+    ok = [batch_frame(*fr.images(i), fp) for i in range(2)]
+    assert b.record(stream, ok) == 0
+    # more frames, more pixels, a larger frame than created for
+    assert b.record(stream, ok + ok[:1]) == -capi.ENOSPC
+    assert b.record(stream, [batch_frame(*fr.images(2), fp)]) == -capi.ENOSPC
+    assert b.record(stream, []) == -capi.ENOSPC
+    # one state twice in a recording; a state of the wrong size
+    st = TemporalState(context(), 64, 48)
+    assert b.record(stream, [batch_frame(*fr.images(i), fp, st) for i in range(2)]) == \
+        -capi.EINVAL
+    wrong = TemporalState(context(), 96, 64)
+    assert b.record(stream, [batch_frame(*fr.images(0), fp, wrong)]) == -capi.EINVAL
+    # stages this library does not have
+    bp = batch_params(2, 64, 48, 64 * 48 * 2, flags=capi.CRAW_PIPELINE_FLAG_SHARPEN)
+    with pytest.raises(Exception):
+        Batch(context(), bp)
+    stream.synchronize()

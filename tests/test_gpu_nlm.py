@@ -1,0 +1,122 @@
+"""The denoised mosaic itself, read back through the C API.
+
+With the 12-bit levels (black 512, white 4095), unit gains and an identity
+matrix, XYZ output and the bilinear debayer, the output's own channel at
+every site is the prep of the denoised sample, (code / 65535 - black) /
+range, which inverts to the code exactly (clipped at white, as the prep
+clips), so the 16-bit mosaic the denoise kernel wrote is recovered from the
+frame libcraw develops.
+
+  - FP32 form: equal to the oracle's deterministic float mode, every code;
+  - fast FP32 and packed-FP16 forms: every code within one of the FP64
+    rerun of the reference (one code is 1/3583 of the 12-bit clip range,
+    2^-11.8: within the 2^-11 bound), at low, mid and high noise and with
+    a temporal history.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+import oracle
+from libcraw_amd import capi, scene
+from libcraw_amd.pipeline import frame_params
+
+from gpu_util import Runner
+
+pytestmark = pytest.mark.gpu
+
+FORMS = {"exact": 0, "fast": capi.CRAW_PIPELINE_FLAG_NLM_FAST,
+         "fp16": capi.CRAW_PIPELINE_FLAG_NLM_FP16}
+
+
+BLACK, WHITE = np.float32(512 / 65535.0), np.float32(4095 / 65535.0)
+
+
+def identity_profile():
+    p = capi.CRAWProfile()
+    capi.lib().craw_profile_init(ctypes.byref(p))
+    for s in range(4):
+        p.black_level[s], p.white_level[s] = BLACK, WHITE
+    return p
+
+
+def recover_mosaic(out):
+    h, w, _ = out.shape
+    s = (np.arange(h)[:, None] & 1) * 2 + (np.arange(w)[None, :] & 1)
+    ch = np.choose(s, [0, 1, 1, 2])
+    v = np.take_along_axis(out, ch[..., None], axis=2)[..., 0].astype(np.float64)
+    raw = (v * np.float64(WHITE - BLACK) + np.float64(BLACK)) * 65535.0
+    code = np.rint(raw)
+    assert np.all(np.abs(raw - code) < 1e-2)
+    return code.astype(np.int64)
+
+
+def clip_white(codes):
+    return np.minimum(codes, 4095)
+
+
+def gpu_sequence(form, frames, strength, alpha, a, b):
+    h, w = frames[0].shape
+    flags = capi.CRAW_PIPELINE_FLAG_DENOISE | FORMS[form]
+    if alpha > 0:
+        flags |= capi.CRAW_PIPELINE_FLAG_TEMPORAL_DENOISE
+    run = Runner(w, h, debayer=capi.CRAW_DEBAYER_BILINEAR, flags=flags)
+    prof = identity_profile()
+    outs = []
+    for f in frames:
+        fp = frame_params(prof, denoise_strength=strength, denoise_temporal_strength=alpha,
+                          noise_override=(a, b))
+        assert run.run(f, fp) == 0
+        outs.append(recover_mosaic(run.output()))
+    return outs
+
+
+def oracle_sequence(mode, frames, strength, alpha, a, b):
+    prof = identity_profile()
+    prev, outs = None, []
+    for f in frames:
+        ret, out, nxt, _ = mode.denoise_temporal(f, prof, a, b, strength, alpha_max=alpha,
+                                                 prev=prev, want_state=alpha > 0)
+        assert ret == 0
+        outs.append(clip_white(out.astype(np.int64)))
+        prev = nxt
+    return outs
+
+
+NOISE = [(1e-4, 2e-6), (2e-4, 4e-6), (8e-4, 3e-5)]
+
+
+@pytest.mark.parametrize("a,b", NOISE)
+def test_exact_form_equals_oracle(gpu_available, a, b):
+    frames = [scene.make_frame(160, 128, seed=12, a=a, b=b)]
+    g = gpu_sequence("exact", frames, 1.0, 0.0, a, b)
+    o = oracle_sequence(oracle.f32d, frames, 1.0, 0.0, a, b)
+    assert np.array_equal(g[0], o[0])
+
+
+@pytest.mark.parametrize("form", ["fast", "fp16"])
+@pytest.mark.parametrize("a,b", NOISE)
+def test_fast_forms_within_one_code_of_fp64(gpu_available, form, a, b):
+    frames = [scene.make_frame(224, 160, seed=13, a=a, b=b)]
+    g = gpu_sequence(form, frames, 1.0, 0.0, a, b)[0]
+    o = oracle_sequence(oracle.f64, frames, 1.0, 0.0, a, b)[0]
+    d = np.abs(g - o)
+    assert d.max() <= 1, "%s: %d samples off by more than one code" % (form, int((d > 1).sum()))
+    assert (d > 0).mean() < 0.02
+
+
+@pytest.mark.parametrize("form", ["exact", "fast", "fp16"])
+def test_temporal_forms(gpu_available, form):
+    a, b = 4e-4, 1.6e-5
+    frames = [scene.clip_frame(96, 80, i, square=32, speed=4, a=a, b=b) for i in range(4)]
+    g = gpu_sequence(form, frames, 1.0, 0.8, a, b)
+    if form == "exact":
+        o = oracle_sequence(oracle.f32d, frames, 1.0, 0.8, a, b)
+        for i in range(4):
+            assert np.array_equal(g[i], o[i]), "frame %d" % i
+    else:
+        o = oracle_sequence(oracle.f64, frames, 1.0, 0.8, a, b)
+        for i in range(4):
+            d = np.abs(g[i] - o[i])
+            assert d.max() <= 1, "frame %d: max %d codes" % (i, int(d.max()))
