@@ -1,20 +1,35 @@
 """Benchmark: developed Mpix/s of the 5.8K full pipeline on one MI355X.
 
 One step develops one 5760x3240 RGGB 12-bit frame (black 512, white 4095)
-through the whole path (config 2 of the workload list): variance-
-stabilised NLM on the four CFA planes at strength 1.0, RCD debayer, chroma
-denoise 0.5, the AWB histogram, camera matrix to ACES AP1 linear with the
-ACES 1.3 gamut compression, RGBA32F out; FP32 throughout, the plain form:
-one craw_pipeline_record() per frame on one stream. The input frame is
-resident in HBM before timing starts.
+through the whole path: variance-stabilised NLM on the four CFA planes at
+strength 1.0, RCD debayer, chroma denoise 0.5, the AWB histogram, camera
+matrix to ACES AP1 linear with the ACES 1.3 gamut compression, RGBA32F
+out; FP32 throughout, the plain form: one craw_pipeline_record() per frame
+on one stream. The input frame is resident in HBM before timing starts.
+That is `value`.
 
 Prints one JSON line. `roofline` is the dominant kernel's: algorithmic
 bytes per launch over its average launch time, taken from the library's
 own hipEvent pair around each launch (CRAW_PIPELINE_FLAG_PROFILE, on the
-stream the kernels run on) over the last recordings of the timed region.
-`cpu_baseline` times the oracle (the CPU restatement of lib/ref) on the
-host: single-threaded on a bounded sample and on all cores on the full
-frame, both on the same seeded scene.
+stream the kernels run on) over the last recordings of the timed region;
+`kernels` gives the same for every kernel. `cpu_baseline` times the oracle
+(the CPU restatement of lib/ref) on the host: single-threaded on a bounded
+sample and on all cores on the full frame, both on the same seeded scene.
+
+`forms` measures the other forms on the same GPU (omitted with --plain-only):
+  - plain_fp16: the plain form with the packed-FP16 NLM;
+  - batched_5p8k: the 5.8K full pipeline, 4 frames per batch recording, FP32
+    and packed-FP16 NLM, hipGraph replay, autotuned launch shapes;
+  - batched_mix: 64 frames of uneven sizes per recording (16 each at
+    1920x1080, 4096x2160, 5760x3240, 6000x4000), per-frame noise and white
+    balance, RCD, RGBA16F, FP32 and packed-FP16 NLM, graphs, autotuned;
+  - served: a request loop fed on a wall-clock schedule by 16 clip streams
+    (8 at 4096x2160, 8 at 5760x3240, 24-60 fps with exponential jitter,
+    each with its own temporal state, noise model and white balance) and
+    6000x4000 stills at Poisson 5/s, within a 64 GB budget: full pipeline
+    with temporal 0.8 and packed-FP16 NLM, RGBA16F; Mpix/s and p50/p99
+    frame latency, beside the same schedule replayed through the plain
+    form (one frame recorded and waited for at a time, FP32 NLM).
 """
 import argparse
 import json
@@ -56,43 +71,342 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--plain-only", action="store_true")
+    ap.add_argument("--serve-seconds", type=float, default=3.0)
     return ap.parse_args()
 
 
 def cpu_baseline(threads):
-    """The oracle's float/glibc mode (lib/ref as written) on the host."""
+    """The oracle's float/glibc mode (lib/ref as written) on the host: one
+    full frame on one core, and three full frames on `threads` cores (one
+    OpenMP team over each frame's rows and planes); about 10-15 s."""
     import oracle
     from libcraw_amd import capi, scene
     from libcraw_amd.pipeline import frame_params, pipeline_params
 
     prof = scene.make_profile()
+    bayer = scene.make_frame(W, H, seed=1, a=NOISE[0], b=NOISE[1])
+    pp = pipeline_params(W, H, debayer=capi.CRAW_DEBAYER_RCD,
+                         out_space=capi.CRAW_CS_ACES_AP1_LINEAR,
+                         flags=capi.CRAW_PIPELINE_FLAG_DENOISE |
+                         capi.CRAW_PIPELINE_FLAG_CHROMA_DENOISE |
+                         capi.CRAW_PIPELINE_FLAG_AWB_HISTOGRAM)
+    fp = frame_params(prof, denoise_strength=1.0, denoise_chroma_strength=0.5,
+                      noise_override=NOISE)
     res = {}
-    for label, (w, h, n) in (("single", (2880, 1620, 1)), ("all", (W, H, threads))):
-        bayer = scene.make_frame(w, h, seed=1, a=NOISE[0], b=NOISE[1])
-        pp = pipeline_params(w, h, debayer=capi.CRAW_DEBAYER_RCD,
-                             out_space=capi.CRAW_CS_ACES_AP1_LINEAR,
-                             flags=capi.CRAW_PIPELINE_FLAG_DENOISE |
-                             capi.CRAW_PIPELINE_FLAG_CHROMA_DENOISE |
-                             capi.CRAW_PIPELINE_FLAG_AWB_HISTOGRAM)
-        fp = frame_params(prof, denoise_strength=1.0, denoise_chroma_strength=0.5,
-                          noise_override=NOISE)
+    for label, n, reps in (("single", 1, 1), ("all", threads, 3)):
         oracle.f32l.set_threads(n)
         t0 = time.perf_counter()
-        r = oracle.f32l.process(bayer, pp, fp, want_hist=True)
+        for _ in range(reps):
+            r = oracle.f32l.process(bayer, pp, fp, want_hist=True)
+            assert r["ret"] == 0
         dt = time.perf_counter() - t0
-        assert r["ret"] == 0
-        res[label] = (w * h / 1e6 / dt, n, w, h, dt)
-    mp_all, n_all, _, _, dt_all = res["all"]
-    mp_1, _, w1, h1, dt1 = res["single"]
+        res[label] = (W * H * reps / 1e6 / dt, dt)
+    mp_all, dt_all = res["all"]
+    mp_1, dt1 = res["single"]
     return {
-        "value": round(mp_all, 3), "unit": "Mpix/s", "cores": n_all, "kind": "port",
-        "sample": "oracle float/glibc mode (lib/ref restated), full path of the same "
-                  "workload on one 5760x3240 frame, %d OpenMP threads, %.1f s; "
-                  "single-threaded on a 2880x1620 frame of the same scene generator: "
-                  "%.3f Mpix/s (%.1f s)" % (n_all, dt_all, mp_1, dt1),
+        "value": round(mp_all, 3), "unit": "Mpix/s", "cores": threads, "kind": "port",
+        "sample": "oracle float/glibc mode (lib/ref restated), the same workload: 3 frames of "
+                  "5760x3240 on %d OpenMP threads in %.1f s; one frame on one thread: "
+                  "%.3f Mpix/s (%.1f s)" % (threads, dt_all, mp_1, dt1),
         "single_thread_value": round(mp_1, 3),
     }
 
+# ------------------------------------------------------------------ forms
+
+MIX_SIZES = [(1920, 1080), (4096, 2160), (5760, 3240), (6000, 4000)]
+SERVE_SIZES = [(4096, 2160), (5760, 3240)]
+STILL_SIZE = (6000, 4000)
+
+
+class Mosaics:
+    """One seeded mosaic per size, resident on the device (frames of one
+    size share it; each frame has its own destination)."""
+
+    def __init__(self, sizes, hip, scene):
+        self.buf = {}
+        for i, (w, h) in enumerate(sizes):
+            if (w, h) in self.buf:
+                continue
+            m = scene.make_frame(w, h, seed=20 + i, a=NOISE[0], b=NOISE[1])
+            b = hip.DeviceBuffer(w * h * 2)
+            b.upload(m)
+            self.buf[(w, h)] = b
+            print("bench: scene %dx%d ready" % (w, h), file=sys.stderr, flush=True)
+
+
+def frame_draw(rng, prof_of, frame_params, strength=1.0, chroma=0.5, temporal=0.0):
+    """Per-frame noise coefficients and white balance, as the batched and
+    served workloads draw them."""
+    a = float(rng.uniform(1e-4, 8e-4))
+    b = float(rng.uniform(2e-6, 3e-5))
+    wb = (float(rng.uniform(1.2, 2.6)), 1.0, float(rng.uniform(1.2, 2.6)))
+    prof = prof_of(wb)
+    return frame_params(prof, denoise_strength=strength, denoise_chroma_strength=chroma,
+                        denoise_temporal_strength=temporal, noise_override=(a, b))
+
+
+def time_batched(ctx, capi, hip, P, sizes, mos, fps, fmt, extra_flags, steps, warmup,
+                 out_space):
+    """Mpix/s of batch recordings of `sizes` (one frame each), graphs on,
+    after the autotuner has picked the launch shapes."""
+    bpp = 16 if fmt == capi.CRAW_FORMAT_R32G32B32A32_SFLOAT else 8
+    flags = (capi.CRAW_PIPELINE_FLAG_DENOISE | capi.CRAW_PIPELINE_FLAG_CHROMA_DENOISE |
+             capi.CRAW_PIPELINE_FLAG_AWB_HISTOGRAM | extra_flags)
+    npx = sum(w * h for w, h in sizes)
+    bp = P.batch_params(len(sizes), max(w for w, _ in sizes), max(h for _, h in sizes), npx,
+                        debayer=capi.CRAW_DEBAYER_RCD, out_space=out_space, out_format=fmt,
+                        flags=flags, use_graphs=True)
+    batch = P.Batch(ctx, bp)
+    dsts = [hip.DeviceBuffer(w * h * bpp) for w, h in sizes]
+    frames = [P.batch_frame(P.image(mos.buf[(w, h)], w, h, capi.CRAW_FORMAT_R16_UNORM),
+                            P.image(d, w, h, fmt), fp)
+              for (w, h), d, fp in zip(sizes, dsts, fps)]
+    stream = hip.Stream()
+    tune_us = batch.autotune(stream, frames, iterations=2)
+    for _ in range(warmup):
+        assert batch.record(stream, frames) == 0
+    stream.synchronize()
+    e0, e1 = hip.Event(), hip.Event()
+    e0.record(stream)
+    for _ in range(steps):
+        assert batch.record(stream, frames) == 0
+    e1.record(stream)
+    stream.synchronize()
+    ms = e0.elapsed_ms(e1) / steps
+    batch.close()
+    for d in dsts:
+        d.close()
+    return {"mpix_s": round(npx / 1e6 / (ms / 1e3), 1), "ms_per_batch": round(ms, 3),
+            "frames": len(sizes), "mpix_per_batch": round(npx / 1e6, 2),
+            "autotune_us": tune_us}
+
+
+def serve_schedule(rng, seconds):
+    """(time s, kind, stream, index), sorted: frame k of each of 16 clip
+    streams due at k / fps (fps drawn in 24-60) plus an exponential jitter
+    of mean a tenth of the period, and stills at Poisson 5/s."""
+    ev = []
+    for sidx in range(16):
+        fps = float(rng.uniform(24.0, 60.0))
+        phase = float(rng.uniform(0.0, 1.0 / fps))
+        k = 0
+        while phase + k / fps < seconds:
+            jitter = min(float(rng.exponential(0.1 / fps)), 0.9 / fps)
+            ev.append((phase + k / fps + jitter, "clip", sidx, k))
+            k += 1
+    t = float(rng.exponential(1.0 / 5.0))
+    n = 0
+    while t < seconds:
+        ev.append((t, "still", -1, n))
+        n += 1
+        t += float(rng.exponential(1.0 / 5.0))
+    ev.sort()
+    return ev
+
+
+def time_served(ctx, capi, hip, P, scene, mos, rng, seconds):
+    """The served form on a wall-clock schedule; returns its figures and the
+    schedule with each stream's sizes and frame parameters."""
+    fmt = capi.CRAW_FORMAT_R16G16B16A16_SFLOAT
+    flags = (capi.CRAW_PIPELINE_FLAG_DENOISE | capi.CRAW_PIPELINE_FLAG_CHROMA_DENOISE |
+             capi.CRAW_PIPELINE_FLAG_AWB_HISTOGRAM | capi.CRAW_PIPELINE_FLAG_TEMPORAL_DENOISE |
+             capi.CRAW_PIPELINE_FLAG_NLM_FP16)
+    sp = P.server_params(STILL_SIZE[0], STILL_SIZE[1], 8 * 5760 * 3240, max_batch_frames=16,
+                         nb_slots=2, hbm_budget=64 << 30, debayer=capi.CRAW_DEBAYER_RCD,
+                         out_space=capi.CRAW_CS_ACES_AP1_LINEAR, out_format=fmt, flags=flags,
+                         max_queue=4096)
+    srv = P.Server(ctx, sp)
+    ring = 4
+    sizes = [SERVE_SIZES[i % 2] for i in range(16)]
+    profiles = []
+
+    def prof_of(wb):
+        p = scene.make_profile(wb=wb)
+        profiles.append(p)
+        return p
+
+    fps_stream = [frame_draw(rng, prof_of, P.frame_params, temporal=0.8) for _ in range(16)]
+    fp_still = frame_draw(rng, prof_of, P.frame_params)
+    sids = [srv.open_stream(w, h) for w, h in sizes]
+    dst = {}
+    for i, (w, h) in enumerate(sizes):
+        dst[i] = [hip.DeviceBuffer(w * h * 8) for _ in range(ring)]
+    dst[-1] = [hip.DeviceBuffer(STILL_SIZE[0] * STILL_SIZE[1] * 8) for _ in range(ring)]
+    busy = {}                     # (stream, ring slot) -> tag
+    info = {}                     # tag -> (due s, submit perf s, pixels)
+    done = {}
+
+    def collect(cs):
+        for c in cs:
+            done[c["tag"]] = c
+            busy.pop(info[c["tag"]][3], None)
+
+    def submit(tag, due, sidx, k):
+        w, h = sizes[sidx] if sidx >= 0 else STILL_SIZE
+        key = (sidx, k % ring)
+        while key in busy:
+            collect(srv.poll(timeout_ms=-1))
+        fp = fps_stream[sidx] if sidx >= 0 else fp_still
+        t_sub = time.perf_counter()
+        ret = srv.submit(P.image(mos.buf[(w, h)], w, h, capi.CRAW_FORMAT_R16_UNORM),
+                         P.image(dst[sidx][k % ring], w, h, fmt), fp,
+                         stream=sids[sidx] if sidx >= 0 else -1, tag=tag)
+        assert ret == 0, ret
+        info[tag] = (due, t_sub, w * h, key)
+        busy[key] = tag
+
+    # warm-up: one frame of every stream and a still, not timed
+    for i in range(16):
+        submit(1_000_000 + i, 0.0, i, 0)
+    submit(1_000_100, 0.0, -1, 0)
+    srv.drain()
+    collect(srv.poll(max_n=4096, timeout_ms=0))
+    done.clear()
+
+    sched = serve_schedule(rng, seconds)
+    t0 = time.perf_counter()
+    for tag, (t, kind, sidx, k) in enumerate(sched):
+        while True:
+            now = time.perf_counter() - t0
+            if now >= t:
+                break
+            collect(srv.poll(timeout_ms=0))
+            if t - now > 0.001:
+                time.sleep(0.0005)
+        submit(tag, t0 + t, sidx, k + 1)
+    srv.drain()
+    while len(done) < len(sched):
+        collect(srv.poll(max_n=4096, timeout_ms=-1))
+    assert all(done[i]["status"] == 0 for i in range(len(sched)))
+
+    # server clock -> perf clock, from the submissions
+    off = np.median([info[i][1] * 1e3 - done[i]["submit_ms"] for i in range(len(sched))])
+    lat = np.array([done[i]["done_ms"] + off - info[i][0] * 1e3 for i in range(len(sched))])
+    svc = np.array([done[i]["done_ms"] - done[i]["launch_ms"] for i in range(len(sched))])
+    first = min(info[i][1] for i in range(len(sched))) * 1e3
+    last = max(done[i]["done_ms"] + off for i in range(len(sched)))
+    npx = sum(info[i][2] for i in range(len(sched)))
+    frames_per_batch = np.array([done[i]["batch_frames"] for i in range(len(sched))])
+    usage = srv.memory_usage()
+    for i in range(16):
+        srv.close_stream(sids[i])
+    srv.close()
+    for v in dst.values():
+        for d in v:
+            d.close()
+    offered = npx / 1e6 / seconds
+    return sched, sizes, fps_stream, fp_still, {
+        "mpix_s": round(npx / 1e6 / ((last - first) / 1e3), 1),
+        "offered_mpix_s": round(offered, 1), "frames": len(sched),
+        "stills": sum(1 for e in sched if e[1] == "still"),
+        "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
+                       "p99": round(float(np.percentile(lat, 99)), 3),
+                       "from": "due time to completion seen by the scheduler"},
+        "batch_gpu_ms_p50": round(float(np.percentile(svc, 50)), 3),
+        "frames_per_batch_mean": round(float(frames_per_batch.mean()), 2),
+        "hbm_held_bytes": usage, "hbm_budget_bytes": 64 << 30, "slots": 2,
+        "schedule_s": seconds,
+    }
+
+
+def time_plain_trace(ctx, capi, hip, P, mos, sched, sizes, fps_stream, fp_still):
+    """The served schedule replayed through the plain form: one
+    craw_pipeline_record() per frame, each waited for before the next, FP32
+    NLM, one pipeline per clip stream (its temporal state) and one for
+    stills."""
+    fmt = capi.CRAW_FORMAT_R16G16B16A16_SFLOAT
+    base = (capi.CRAW_PIPELINE_FLAG_DENOISE | capi.CRAW_PIPELINE_FLAG_CHROMA_DENOISE |
+            capi.CRAW_PIPELINE_FLAG_AWB_HISTOGRAM)
+    pipes, dsts = {}, {}
+    for i, (w, h) in list(enumerate(sizes)) + [(-1, STILL_SIZE)]:
+        flags = base | (capi.CRAW_PIPELINE_FLAG_TEMPORAL_DENOISE if i >= 0 else 0)
+        pipes[i] = P.Pipeline(ctx, P.pipeline_params(w, h, debayer=capi.CRAW_DEBAYER_RCD,
+                                                     out_space=capi.CRAW_CS_ACES_AP1_LINEAR,
+                                                     out_format=fmt, flags=flags))
+        dsts[i] = hip.DeviceBuffer(w * h * 8)
+    stream = hip.Stream()
+
+    def run(sidx):
+        w, h = sizes[sidx] if sidx >= 0 else STILL_SIZE
+        fp = fps_stream[sidx] if sidx >= 0 else fp_still
+        assert pipes[sidx].record(stream, P.image(mos.buf[(w, h)], w, h,
+                                                  capi.CRAW_FORMAT_R16_UNORM),
+                                  P.image(dsts[sidx], w, h, fmt), fp) == 0
+        stream.synchronize()
+        return w * h
+
+    for i in list(range(16)) + [-1]:
+        run(i)
+    lat, npx = [], 0
+    t0 = time.perf_counter()
+    for t, kind, sidx, k in sched:
+        while time.perf_counter() - t0 < t:
+            pass
+        npx += run(sidx)
+        lat.append((time.perf_counter() - t0 - t) * 1e3)
+    span = time.perf_counter() - t0 - sched[0][0]
+    for p in pipes.values():
+        p.close()
+    for d in dsts.values():
+        d.close()
+    lat = np.array(lat)
+    return {"mpix_s": round(npx / 1e6 / span, 1),
+            "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
+                           "p99": round(float(np.percentile(lat, 99)), 3),
+                           "from": "due time to completion"},
+            "frames": len(sched)}
+
+
+def measure_forms(args, ctx, capi, hip, P, scene, plain_value):
+    forms = {}
+    rng = np.random.Generator(np.random.Philox(2024))
+    mos = Mosaics(MIX_SIZES, hip, scene)
+    ap1 = capi.CRAW_CS_ACES_AP1_LINEAR
+    f32, f16 = capi.CRAW_FORMAT_R32G32B32A32_SFLOAT, capi.CRAW_FORMAT_R16G16B16A16_SFLOAT
+    prof = scene.make_profile()
+    fp_5p8k = P.frame_params(prof, denoise_strength=1.0, denoise_chroma_strength=0.5,
+                             noise_override=NOISE)
+
+    b58 = {}
+    for name, extra in (("fp32", 0), ("fp16", capi.CRAW_PIPELINE_FLAG_NLM_FP16)):
+        r = time_batched(ctx, capi, hip, P, [(W, H)] * 4, mos, [fp_5p8k] * 4, f32, extra,
+                         max(3, args.steps // 4), 1, ap1)
+        r["vs_plain"] = round(r["mpix_s"] / plain_value, 2)
+        b58[name] = r
+        print("bench: batched 5.8K %s %.0f Mpix/s" % (name, r["mpix_s"]), file=sys.stderr,
+              flush=True)
+    forms["batched_5p8k"] = b58
+
+    sizes = [sz for sz in MIX_SIZES for _ in range(16)]
+    profiles = []
+
+    def prof_of(wb):
+        p = scene.make_profile(wb=wb)
+        profiles.append(p)
+        return p
+
+    fps = [frame_draw(rng, prof_of, P.frame_params) for _ in sizes]
+    mix = {}
+    for name, extra in (("fp32", 0), ("fp16", capi.CRAW_PIPELINE_FLAG_NLM_FP16)):
+        mix[name] = time_batched(ctx, capi, hip, P, sizes, mos, fps, f16, extra, 3, 1, ap1)
+        print("bench: batched mix %s %.0f Mpix/s" % (name, mix[name]["mpix_s"]),
+              file=sys.stderr, flush=True)
+    forms["batched_mix"] = mix
+
+    sched, ssizes, sfps, sstill, served = time_served(ctx, capi, hip, P, scene, mos, rng,
+                                                      args.serve_seconds)
+    print("bench: served %.0f Mpix/s p99 %.1f ms" % (served["mpix_s"],
+                                                     served["latency_ms"]["p99"]),
+          file=sys.stderr, flush=True)
+    plain = time_plain_trace(ctx, capi, hip, P, mos, sched, ssizes, sfps, sstill)
+    print("bench: plain replay %.0f Mpix/s p99 %.1f ms" % (plain["mpix_s"],
+                                                          plain["latency_ms"]["p99"]),
+          file=sys.stderr, flush=True)
+    served["plain_replay"] = plain
+    served["vs_plain_replay"] = round(served["mpix_s"] / plain["mpix_s"], 2)
+    forms["served"] = served
+    return forms
 
 def main():
     args = parse()
@@ -166,9 +480,16 @@ def main():
                             "unit": "T fp32 lane-ops/s", "frac": round(tops / VALU_PEAK_TOPS, 3),
                             "ops_per_sample": NLM_OPS_PER_SAMPLE}
 
+    kernels = {}
+    for k, ms in stage_ms.items():
+        b = ALGO_BYTES_PER_PX.get(k, 0.0) * W * H
+        kernels[k] = {"avg_launch_ms": round(ms, 4), "algo_bytes": int(b),
+                      "achieved_gbs": round(b / (ms / 1e3) / 1e9, 1),
+                      "frac_hbm": round(b / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4)}
+
     # plain-form latency: each frame recorded and waited for alone
     lat = []
-    for _ in range(max(10, min(args.steps, 50))):
+    for _ in range(200):
         a, b = hip.Event(), hip.Event()
         a.record(stream)
         assert pipe.record(stream, si, di, fp) == 0
@@ -192,8 +513,30 @@ def main():
                        "p99": round(float(np.percentile(lat, 99)), 4), "n": int(lat.size),
                        "form": "plain"},
         "roofline": roofline,
+        "kernels": kernels,
     }
+    pipe.close()
     print("bench: gpu timing done", file=sys.stderr, flush=True)
+    if not args.plain_only:
+        import libcraw_amd.pipeline as P
+        # the plain form with the packed-FP16 NLM, same frame
+        pp16 = pipeline_params(W, H, debayer=capi.CRAW_DEBAYER_RCD,
+                               out_space=capi.CRAW_CS_ACES_AP1_LINEAR, out_format=fmt,
+                               flags=flags | capi.CRAW_PIPELINE_FLAG_NLM_FP16)
+        p16 = Pipeline(ctx, pp16)
+        for _ in range(2):
+            assert p16.record(stream, si, di, fp) == 0
+        e0.record(stream)
+        for _ in range(args.steps):
+            assert p16.record(stream, si, di, fp) == 0
+        e1.record(stream)
+        stream.synchronize()
+        ms16 = e0.elapsed_ms(e1) / args.steps
+        st16 = {n: round(ms / c, 4) for n, c, ms in p16.stage_times(0)}
+        p16.close()
+        out["forms"] = {"plain_fp16": {"mpix_s": round(mpix / (ms16 / 1e3), 1),
+                                       "ms_per_frame": round(ms16, 4), "stage_ms": st16}}
+        out["forms"].update(measure_forms(args, ctx, capi, hip, P, scene, value))
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
         out["cpu_baseline"] = cpu_baseline(threads)

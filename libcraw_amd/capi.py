@@ -164,6 +164,26 @@ class CRAWBatchFrame(ctypes.Structure):
                 ("params", ctypes.POINTER(CRAWFrameParams)), ("state", ctypes.c_void_p)]
 
 
+class CRAWServerParams(ctypes.Structure):
+    _fields_ = [("hbm_budget", ctypes.c_uint64), ("max_batch_pixels", ctypes.c_uint64),
+                ("nb_slots", c_u32), ("max_batch_frames", c_u32), ("max_width", c_u32),
+                ("max_height", c_u32), ("debayer", c_i32), ("out_space", c_i32),
+                ("out_format", c_i32), ("flags", c_u32), ("use_graphs", c_u32),
+                ("max_queue", c_u32), ("padding", ctypes.c_uint8 * (256 - 8 * 2 - 4 * 10))]
+
+
+class CRAWServeRequest(ctypes.Structure):
+    _fields_ = [("src", CRAWImage), ("dst", CRAWImage), ("params", CRAWFrameParams),
+                ("stream", c_i32), ("reserved", c_u32), ("tag", ctypes.c_uint64),
+                ("awb_histogram", ctypes.POINTER(c_u32))]
+
+
+class CRAWServeCompletion(ctypes.Structure):
+    _fields_ = [("tag", ctypes.c_uint64), ("status", c_i32), ("stream", c_i32),
+                ("submit_ms", ctypes.c_double), ("launch_ms", ctypes.c_double),
+                ("done_ms", ctypes.c_double), ("batch_frames", c_u32), ("slot", c_u32)]
+
+
 P = ctypes.POINTER
 vp = ctypes.c_void_p
 
@@ -203,6 +223,16 @@ SIGNATURES = [
     ("craw_temporal_state_create", c_i32, [vp, c_u32, c_u32, P(vp)]),
     ("craw_temporal_state_free", None, [P(vp)]),
     ("craw_temporal_state_reset", None, [vp]),
+    ("craw_server_params_init", None, [P(CRAWServerParams)]),
+    ("craw_server_create", c_i32, [vp, P(CRAWServerParams), P(vp)]),
+    ("craw_server_free", None, [P(vp)]),
+    ("craw_server_open_stream", c_i32, [vp, c_u32, c_u32, P(c_i32)]),
+    ("craw_server_close_stream", c_i32, [vp, c_i32]),
+    ("craw_server_submit", c_i32, [vp, P(CRAWServeRequest)]),
+    ("craw_server_submit_n", c_i32, [vp, P(CRAWServeRequest), c_u32]),
+    ("craw_server_poll", c_i32, [vp, P(CRAWServeCompletion), c_u32, c_i32]),
+    ("craw_server_drain", c_i32, [vp]),
+    ("craw_server_get_memory_usage", ctypes.c_uint64, [vp]),
 ]
 
 _lib = None

@@ -48,6 +48,7 @@ struct CRAWBatch {
     uint32_t *hist;
     uint32_t *hist_host;    /* pinned, max_frames histograms */
     uint32_t nb_last;
+    uint64_t device_bytes;
 
     /* device tables */
     KFrame *frames;
@@ -130,24 +131,81 @@ CRAW_API void craw_batch_free(CRAWBatch **pb)
     *pb = NULL;
 }
 
-CRAW_API int craw_batch_create(CRAWContext *ctx, const CRAWBatchParams *params, CRAWBatch **out)
+} /* extern "C" */
+
+/* Bytes of each device buffer a batch of `params` allocates. */
+struct BatchSizes {
+    size_t clean, inter_rgb, down, blur, hist, frames, tiles_cap;
+
+    size_t device() const
+    {
+        return clean + inter_rgb + down + blur + hist + frames + tiles_cap * sizeof(uint32_t);
+    }
+};
+
+static int batch_sizes(CRAWContext *ctx, const CRAWBatchParams *params, CRAWStageConfig *cfg,
+                       BatchSizes *sz)
 {
-    CRAWBatch *b;
-    size_t px, qpx, staging;
+    size_t px, qpx;
+    bool fast;
     int ret;
 
-    if (!ctx || !params || !out)
-        return CRAW_ERROR(EINVAL);
     if (!params->max_frames || params->max_frames > 255 || !params->max_pixels ||
         params->max_width < 8 || params->max_height < 8 ||
         (uint64_t)params->max_width * params->max_height > params->max_pixels)
         return CRAW_ERROR(EINVAL);
+    ret = craw_stage_config(ctx, params->flags, params->debayer, params->out_space,
+                            params->out_format, cfg);
+    if (ret < 0)
+        return ret;
+    fast = cfg->fp16 || cfg->fast;
+    px = params->max_pixels;
+    /* quarter grid of every frame, with a row and a column of rounding */
+    qpx = px / 16 + (size_t)params->max_frames * ((params->max_width + params->max_height) / 4 + 2);
+    memset(sz, 0, sizeof(*sz));
+    if (cfg->has_denoise || cfg->has_temporal)
+        sz->clean = px * sizeof(uint16_t);
+    if (cfg->has_chroma) {
+        sz->inter_rgb = 3 * px * sizeof(float);
+        sz->down = 3 * qpx * sizeof(float);
+        sz->blur = 2 * qpx * sizeof(float);
+    }
+    if (cfg->awb)
+        sz->hist = (size_t)params->max_frames * CRAW_HIST_WORDS * 4;
+    sz->frames = params->max_frames * sizeof(KFrame);
+    for (int st = 0; st < KSTAGE_NB; st++)
+        sz->tiles_cap += stage_tiles_bound(st, params, (int)params->debayer, fast);
+    sz->tiles_cap += NB_LISTS * 64;
+    return 0;
+}
 
+int craw_batch_footprint(CRAWContext *ctx, const CRAWBatchParams *params, uint64_t *bytes)
+{
+    CRAWStageConfig cfg;
+    BatchSizes sz;
+    int ret = batch_sizes(ctx, params, &cfg, &sz);
+
+    if (ret < 0)
+        return ret;
+    *bytes = sz.device();
+    return 0;
+}
+
+extern "C" {
+
+CRAW_API int craw_batch_create(CRAWContext *ctx, const CRAWBatchParams *params, CRAWBatch **out)
+{
+    CRAWBatch *b;
+    BatchSizes sz;
+    size_t staging;
+    int ret;
+
+    if (!ctx || !params || !out)
+        return CRAW_ERROR(EINVAL);
     b = new CRAWBatch();
     b->ctx = ctx;
     b->params = *params;
-    ret = craw_stage_config(ctx, params->flags, params->debayer, params->out_space,
-                            params->out_format, &b->cfg);
+    ret = batch_sizes(ctx, params, &b->cfg, &sz);
     if (ret < 0) {
         delete b;
         return ret;
@@ -160,34 +218,21 @@ CRAW_API int craw_batch_create(CRAWContext *ctx, const CRAWBatchParams *params, 
         return CRAW_ERROR(ENODEV);
     }
 
-    px = params->max_pixels;
-    /* quarter grid of every frame, with a row and a column of rounding */
-    qpx = px / 16 + (size_t)params->max_frames * ((params->max_width + params->max_height) / 4 + 2);
     ret = CRAW_ERROR(ENOMEM);
-    if ((b->cfg.has_denoise || b->cfg.has_temporal) &&
-        hipMalloc((void **)&b->clean, px * sizeof(uint16_t)) != hipSuccess)
+#define DALLOC(ptr, bytes) ((bytes) && hipMalloc((void **)&(ptr), (bytes)) != hipSuccess)
+    if (DALLOC(b->clean, sz.clean) || DALLOC(b->inter_rgb, sz.inter_rgb) ||
+        DALLOC(b->down, sz.down) || DALLOC(b->blur, sz.blur) || DALLOC(b->hist, sz.hist) ||
+        DALLOC(b->frames, sz.frames) || DALLOC(b->tiles, sz.tiles_cap * sizeof(uint32_t)))
         goto fail;
-    if (b->cfg.has_chroma &&
-        (hipMalloc((void **)&b->inter_rgb, 3 * px * sizeof(float)) != hipSuccess ||
-         hipMalloc((void **)&b->down, 3 * qpx * sizeof(float)) != hipSuccess ||
-         hipMalloc((void **)&b->blur, 2 * qpx * sizeof(float)) != hipSuccess))
+#undef DALLOC
+    if (sz.hist && hipHostMalloc((void **)&b->hist_host, sz.hist, hipHostMallocDefault) !=
+                       hipSuccess) {
+        b->hist_host = NULL;
         goto fail;
-    if (b->cfg.awb &&
-        (hipMalloc((void **)&b->hist, (size_t)params->max_frames * CRAW_HIST_WORDS * 4) !=
-             hipSuccess ||
-         hipHostMalloc((void **)&b->hist_host, (size_t)params->max_frames * CRAW_HIST_WORDS * 4,
-                       hipHostMallocDefault) != hipSuccess))
-        goto fail;
-
-    b->tiles_cap = 0;
-    for (int st = 0; st < KSTAGE_NB; st++)
-        b->tiles_cap += stage_tiles_bound(st, params, (int)params->debayer,
-                                          b->form != KFORM_EXACT);
-    b->tiles_cap += NB_LISTS * 64;
-    if (hipMalloc((void **)&b->frames, params->max_frames * sizeof(KFrame)) != hipSuccess ||
-        hipMalloc((void **)&b->tiles, b->tiles_cap * sizeof(uint32_t)) != hipSuccess)
-        goto fail;
-    staging = params->max_frames * sizeof(KFrame) + b->tiles_cap * sizeof(uint32_t);
+    }
+    b->tiles_cap = sz.tiles_cap;
+    b->device_bytes = sz.device();
+    staging = sz.frames + b->tiles_cap * sizeof(uint32_t);
     for (int i = 0; i < RING; i++) {
         if (hipHostMalloc(&b->ring[i].host, staging, hipHostMallocDefault) != hipSuccess) {
             b->ring[i].host = NULL;
@@ -295,6 +340,15 @@ static int batch_plan(CRAWBatch *b, const CRAWBatchFrame *frames, uint32_t nb, B
     for (int k = 0; k < 4; k++)
         bp.key.push_back((uint32_t)b->wg_per_cu[k]);
     return 0;
+}
+
+int craw_batch_check(CRAWBatch *b, const CRAWBatchFrame *frames, uint32_t nb)
+{
+    BatchPlan bp;
+
+    if (!b || !frames)
+        return CRAW_ERROR(EINVAL);
+    return batch_plan(b, frames, nb, bp);
 }
 
 static int grid_for(CRAWBatch *b, int kind, size_t ntiles)

@@ -123,6 +123,13 @@ void craw_profiler_finish(CRAWProfRun &r);
 int craw_profiler_times(CRAWContext *ctx, CRAWProfiler *pr, uint32_t frames_back,
                         CRAWStageTime *out, uint32_t max);
 
+/* Device bytes a batch of `params` allocates (no allocation). */
+int craw_batch_footprint(CRAWContext *ctx, const CRAWBatchParams *params, uint64_t *bytes);
+
+/* Plans `frames` as craw_batch_record() would, and launches nothing: 0 or
+ * the error recording them would return. */
+int craw_batch_check(CRAWBatch *b, const CRAWBatchFrame *frames, uint32_t nb);
+
 /* hipFree and friends on teardown paths, whose errors change nothing. */
 #define CRAW_IGNORE(x) do { hipError_t craw_ignored_ = (x); (void)craw_ignored_; } while (0)
 

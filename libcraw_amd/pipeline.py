@@ -223,3 +223,98 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+def server_params(max_width, max_height, max_batch_pixels, max_batch_frames=16, nb_slots=2,
+                  hbm_budget=64 << 30, debayer=capi.CRAW_DEBAYER_RCD,
+                  out_space=capi.CRAW_CS_XYZ_D65,
+                  out_format=capi.CRAW_FORMAT_R32G32B32A32_SFLOAT, flags=0, use_graphs=False,
+                  max_queue=1024):
+    sp = capi.CRAWServerParams()
+    capi.lib().craw_server_params_init(ctypes.byref(sp))
+    sp.hbm_budget, sp.max_batch_pixels = hbm_budget, max_batch_pixels
+    sp.nb_slots, sp.max_batch_frames = nb_slots, max_batch_frames
+    sp.max_width, sp.max_height = max_width, max_height
+    sp.debayer, sp.out_space, sp.out_format = debayer, out_space, out_format
+    sp.flags, sp.use_graphs, sp.max_queue = flags, 1 if use_graphs else 0, max_queue
+    return sp
+
+
+class Server:
+    """Continuous-batching request loop (include/craw/serve.h)."""
+
+    def __init__(self, ctx, params):
+        self._lib = capi.lib()
+        self.ctx = ctx
+        self.params = params
+        self.handle = ctypes.c_void_p()
+        self._keep = {}
+        _ok(self._lib.craw_server_create(ctx.handle, ctypes.byref(params),
+                                         ctypes.byref(self.handle)), "craw_server_create")
+
+    def open_stream(self, width, height):
+        sid = ctypes.c_int32()
+        _ok(self._lib.craw_server_open_stream(self.handle, width, height, ctypes.byref(sid)),
+            "craw_server_open_stream")
+        return sid.value
+
+    def close_stream(self, sid):
+        _ok(self._lib.craw_server_close_stream(self.handle, sid), "craw_server_close_stream")
+
+    @staticmethod
+    def request(src, dst, fp, stream=-1, tag=0, histogram=None):
+        """A CRAWServeRequest. `histogram`: None or a uint32 array of
+        3 x 128 x 128 words, filled before the completion is posted."""
+        r = capi.CRAWServeRequest()
+        r.src, r.dst, r.params = src, dst, fp
+        r.stream, r.tag = stream, tag
+        if histogram is not None:
+            assert histogram.dtype == np.uint32 and histogram.size == 3 * 128 * 128
+            r.awb_histogram = histogram.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))
+        r._refs = (fp, histogram)
+        return r
+
+    def submit(self, src, dst, fp, stream=-1, tag=0, histogram=None):
+        """Returns the C API's return code."""
+        return self.submit_n([self.request(src, dst, fp, stream, tag, histogram)])
+
+    def submit_n(self, reqs):
+        """Queues requests (from request()) together; the C API's return code."""
+        arr = (capi.CRAWServeRequest * max(len(reqs), 1))()
+        for i, r in enumerate(reqs):
+            arr[i] = r
+        ret = self._lib.craw_server_submit_n(self.handle, arr, len(reqs))
+        if ret == 0:
+            # the profiles and the histograms must outlive the requests
+            for r in reqs:
+                self._keep[r.tag] = r._refs
+        return ret
+
+    def poll(self, max_n=256, timeout_ms=-1):
+        out = (capi.CRAWServeCompletion * max_n)()
+        n = _ok(self._lib.craw_server_poll(self.handle, out, max_n, timeout_ms),
+                "craw_server_poll")
+        res = []
+        for i in range(n):
+            c = out[i]
+            self._keep.pop(c.tag, None)
+            res.append({"tag": c.tag, "status": c.status, "stream": c.stream,
+                        "submit_ms": c.submit_ms, "launch_ms": c.launch_ms,
+                        "done_ms": c.done_ms, "batch_frames": c.batch_frames, "slot": c.slot})
+        return res
+
+    def drain(self):
+        _ok(self._lib.craw_server_drain(self.handle), "craw_server_drain")
+
+    def memory_usage(self):
+        return int(self._lib.craw_server_get_memory_usage(self.handle))
+
+    def close(self):
+        if self.handle.value:
+            self._lib.craw_server_free(ctypes.byref(self.handle))
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

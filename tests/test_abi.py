@@ -41,18 +41,20 @@ def test_struct_sizes_match_the_reference():
     c = os.path.join(src, "abi_sizes.c")
     exe = os.path.join(src, "abi_sizes")
     with open(c, "w") as f:
-        f.write('#include <craw/batch.h>\n#include <stdio.h>\n#include <stddef.h>\n'
-                'int main(void){printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu\\n", '
+        f.write('#include <craw/serve.h>\n#include <stdio.h>\n#include <stddef.h>\n'
+                'int main(void){printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\\n", '
                 'sizeof(CRAWPipelineParams), sizeof(CRAWFrameParams), sizeof(CRAWProfile), '
                 'sizeof(CRAWImage), sizeof(CRAWContextOptions), '
                 'offsetof(CRAWFrameParams, lens), offsetof(CRAWFrameParams, rotation), '
-                'sizeof(CRAWBatchParams), sizeof(CRAWBatchFrame));}\n')
+                'sizeof(CRAWBatchParams), sizeof(CRAWBatchFrame), sizeof(CRAWServerParams), '
+                'sizeof(CRAWServeRequest), sizeof(CRAWServeCompletion));}\n')
     subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), c, "-o", exe], check=True)
     sizes = [int(x) for x in subprocess.run([exe], capture_output=True, text=True,
                                             check=True).stdout.split()]
     assert sizes == [256, 264, 1264, ctypes.sizeof(capi.CRAWImage), 1024,
                      capi.CRAWFrameParams.lens.offset, capi.CRAWFrameParams.rotation.offset,
-                     256, ctypes.sizeof(capi.CRAWBatchFrame)]
+                     256, ctypes.sizeof(capi.CRAWBatchFrame), 256,
+                     ctypes.sizeof(capi.CRAWServeRequest), ctypes.sizeof(capi.CRAWServeCompletion)]
 
 
 def test_headers_compile_as_c_and_cpp():
@@ -60,6 +62,7 @@ def test_headers_compile_as_c_and_cpp():
         path = "/tmp/craw_hdr_test.%s" % ext
         with open(path, "w") as f:
             f.write("#include <craw/pipeline.h>\n#include <craw/batch.h>\n"
+                    "#include <craw/serve.h>\n"
                     "int main(void){return 0;}\n")
         subprocess.run([compiler, "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"),
                         "-c", path, "-o", "/tmp/craw_hdr_test.o"], check=True)

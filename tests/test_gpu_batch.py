@@ -70,7 +70,8 @@ def params_for(prof, i):
     # different strengths per frame: the frame table carries each frame's own
     return frame_params(prof, denoise_strength=0.8 + 0.15 * (i % 3),
                         denoise_chroma_strength=0.5 + 0.1 * (i % 4),
-                        exposure_stops=0.25 * (i % 2))
+                        exposure_stops=0.25 * (i % 2),
+                        noise_override=(2e-4 * (1 + i % 3), 4e-6 * (1 + i % 2)))
 
 
 def new_batch(sizes, flags, debayer=capi.CRAW_DEBAYER_RCD, graphs=False, fmt=F32,
@@ -188,7 +189,8 @@ def test_temporal_streams(gpu_available):
         fr.upload(mos)
         fps = []
         for c, kind in enumerate(kinds):
-            fp = frame_params(prof, denoise_strength=1.0, denoise_temporal_strength=0.6)
+            fp = frame_params(prof, denoise_strength=1.0, denoise_temporal_strength=0.6,
+                              noise_override=(4e-4, 1.6e-5))
             if kind == "reset":
                 fp.flags |= capi.CRAW_FRAME_FLAG_TEMPORAL_RESET
             if kind == "replay":
@@ -226,7 +228,7 @@ def test_limits_and_errors(gpu_available):
     fr = Frames(sizes + [(128, 96)])
     b = new_batch(sizes, capi.CRAW_PIPELINE_FLAG_TEMPORAL_DENOISE, max_frames=2)
     stream = hip.Stream()
-    fp = frame_params(prof, denoise_temporal_strength=0.5)
+    fp = frame_params(prof, denoise_temporal_strength=0.5, noise_override=(2e-4, 4e-6))
     ok = [batch_frame(*fr.images(i), fp) for i in range(2)]
     assert b.record(stream, ok) == 0
     # more frames, more pixels, a larger frame than created for
