@@ -52,7 +52,7 @@ CRAW_API int craw_context_create(CRAWContext **ctx,
 
 /* One 2D image in device memory: `data` points at the first pixel of the
  * first row, rows are `pitch` bytes apart (at least width * the format's
- * pixel size, and a multiple of 16 for the RGBA formats). Replaces
+ * pixel size, and a multiple of it; `data` aligned to it). Replaces
  * CRAWImage of vulkan.h:150-157; there are no layouts to track. */
 typedef struct CRAWImage {
     void            *data;
