@@ -18,12 +18,13 @@
 #include "internal.h"
 
 #define RING 4
-#define NB_LISTS 10     /* 5 denoise modes, 2 debayer targets, blur, up, spare */
+#define NB_LISTS 10     /* 5 denoise modes, 2 debayer targets, blur, up, awb */
 #define L_DN0 0
 #define L_DB_OUT 5
 #define L_DB_INTER 6
 #define L_CB 7
 #define L_CU 8
+#define L_AWB 9
 
 static const int dn_modes[5] = {
     KDN_SPATIAL, KDN_TEMPORAL, KDN_TEMPORAL | KDN_HISTORY, KDN_SPATIAL | KDN_TEMPORAL,
@@ -46,6 +47,7 @@ struct CRAWBatch {
     uint16_t *clean;
     float *inter_rgb, *down, *blur;
     uint32_t *hist;
+    uint2 *awb_samples;
     uint32_t *hist_host;    /* pinned, max_frames histograms */
     uint32_t nb_last;
     uint64_t device_bytes;
@@ -122,6 +124,7 @@ CRAW_API void craw_batch_free(CRAWBatch **pb)
     CRAW_IGNORE(hipFree(b->down));
     CRAW_IGNORE(hipFree(b->blur));
     CRAW_IGNORE(hipFree(b->hist));
+    CRAW_IGNORE(hipFree(b->awb_samples));
     CRAW_IGNORE(hipFree(b->frames));
     CRAW_IGNORE(hipFree(b->tiles));
     if (b->hist_host)
@@ -135,11 +138,12 @@ CRAW_API void craw_batch_free(CRAWBatch **pb)
 
 /* Bytes of each device buffer a batch of `params` allocates. */
 struct BatchSizes {
-    size_t clean, inter_rgb, down, blur, hist, frames, tiles_cap;
+    size_t clean, inter_rgb, down, blur, hist, awb_samples, frames, tiles_cap;
 
     size_t device() const
     {
-        return clean + inter_rgb + down + blur + hist + frames + tiles_cap * sizeof(uint32_t);
+        return clean + inter_rgb + down + blur + hist + awb_samples + frames +
+               tiles_cap * sizeof(uint32_t);
     }
 };
 
@@ -170,8 +174,12 @@ static int batch_sizes(CRAWContext *ctx, const CRAWBatchParams *params, CRAWStag
         sz->down = 3 * qpx * sizeof(float);
         sz->blur = 2 * qpx * sizeof(float);
     }
-    if (cfg->awb)
+    if (cfg->awb) {
         sz->hist = (size_t)params->max_frames * CRAW_HIST_WORDS * 4;
+        /* 8x8 blocks of every frame, a row and a column of rounding each */
+        sz->awb_samples = (px / 64 + (size_t)params->max_frames *
+                           ((params->max_width + params->max_height) / 8 + 2)) * sizeof(uint2);
+    }
     sz->frames = params->max_frames * sizeof(KFrame);
     for (int st = 0; st < KSTAGE_NB; st++)
         sz->tiles_cap += stage_tiles_bound(st, params, (int)params->debayer, fast);
@@ -222,6 +230,7 @@ CRAW_API int craw_batch_create(CRAWContext *ctx, const CRAWBatchParams *params, 
 #define DALLOC(ptr, bytes) ((bytes) && hipMalloc((void **)&(ptr), (bytes)) != hipSuccess)
     if (DALLOC(b->clean, sz.clean) || DALLOC(b->inter_rgb, sz.inter_rgb) ||
         DALLOC(b->down, sz.down) || DALLOC(b->blur, sz.blur) || DALLOC(b->hist, sz.hist) ||
+        DALLOC(b->awb_samples, sz.awb_samples) ||
         DALLOC(b->frames, sz.frames) || DALLOC(b->tiles, sz.tiles_cap * sizeof(uint32_t)))
         goto fail;
 #undef DALLOC
@@ -265,7 +274,7 @@ struct BatchPlan {
 
 static int batch_plan(CRAWBatch *b, const CRAWBatchFrame *frames, uint32_t nb, BatchPlan &bp)
 {
-    size_t off_px = 0, off_q = 0;
+    size_t off_px = 0, off_q = 0, off_awb = 0;
     bool fast = b->form != KFORM_EXACT;
 
     if (!nb || nb > b->params.max_frames)
@@ -288,6 +297,7 @@ static int batch_plan(CRAWBatch *b, const CRAWBatchFrame *frames, uint32_t nb, B
         buf.down = b->down ? b->down + 3 * off_q : NULL;
         buf.blur = b->blur ? b->blur + 2 * off_q : NULL;
         buf.hist = b->hist ? b->hist + (size_t)i * CRAW_HIST_WORDS : NULL;
+        buf.awb_samples = b->awb_samples ? b->awb_samples + off_awb : NULL;
         buf.ts = fr.state;
         ret = craw_frame_plan(b->ctx, b->cfg, buf, w, h, &fr.src, &fr.dst, fr.params,
                               &bp.plans[i]);
@@ -295,6 +305,7 @@ static int batch_plan(CRAWBatch *b, const CRAWBatchFrame *frames, uint32_t nb, B
             return ret;
         off_px += (size_t)w * h;
         off_q += q;
+        off_awb += craw_awb_blocks(w, h);
     }
     /* one temporal state may appear once per recording */
     for (uint32_t i = 0; i < nb; i++)
@@ -325,6 +336,11 @@ static int batch_plan(CRAWBatch *b, const CRAWBatchFrame *frames, uint32_t nb, B
             n = craw_stage_tiles(KSTAGE_CHROMA_UP, W, H, pl.f.method, fast, &ntx);
             for (int t = 0; t < n; t++)
                 bp.lists[L_CU].push_back(KTILE(i, t));
+        }
+        if (b->cfg.awb) {
+            n = craw_stage_tiles(KSTAGE_AWB, W, H, pl.f.method, fast, &ntx);
+            for (int t = 0; t < n; t++)
+                bp.lists[L_AWB].push_back(KTILE(i, t));
         }
         bp.key.push_back((uint32_t)W);
         bp.key.push_back((uint32_t)H);
@@ -403,6 +419,13 @@ static hipError_t batch_launch(CRAWBatch *b, const BatchPlan &bp, const size_t *
         PB("chroma_up_color");
         e = craw_launch_batch_chroma_up(b->frames, b->tiles + list_off[L_CU], (uint32_t)n,
                                         grid_for(b, 3, n), s);
+        PE();
+    }
+    if (e == hipSuccess && bp.lists[L_AWB].size()) {
+        size_t n = bp.lists[L_AWB].size();
+        PB("awb_hist");
+        e = craw_launch_batch_awb_hist(b->frames, b->tiles + list_off[L_AWB], (uint32_t)n,
+                                       grid_for(b, 3, n), s);
         PE();
     }
 #undef PB
@@ -602,7 +625,8 @@ CRAW_API int craw_batch_autotune(CRAWBatch *b, CRAWStream stream, const CRAWBatc
                 int kind = !strncmp(st[j].name, "denoise", 7) ? 0 :
                            !strcmp(st[j].name, "chroma_blur") ? 2 :
                            !strcmp(st[j].name, "chroma_up_color") ? 3 :
-                           !strcmp(st[j].name, "awb_clear") ? -1 : 1;
+                           !strcmp(st[j].name, "awb_clear") ? -1 :
+                           !strcmp(st[j].name, "awb_hist") ? -1 : 1;
                 if (kind >= 0)
                     acc[kind] += st[j].ms;
             }

@@ -19,7 +19,7 @@ struct CRAWContext {
     int cu_count;
 };
 
-#define CRAW_MAX_STAGES 8
+#define CRAW_MAX_STAGES 12
 #define CRAW_HIST_WORDS (3u * CRAW_AWB_HIST_BINS * CRAW_AWB_HIST_BINS)
 
 /* Two sets of (level, variance) for one clip stream and which set holds
@@ -48,6 +48,7 @@ struct CRAWFrameBuffers {
     float *inter_rgb;           /* 3 x W x H, with chroma denoise */
     float *down, *blur;         /* 3 and 2 quarter planes */
     uint32_t *hist;             /* CRAW_HIST_WORDS, with the histogram */
+    uint2 *awb_samples;         /* one per 8x8 block, with the histogram */
     CRAWTemporalState *ts;      /* with the temporal pass, or NULL */
 };
 
@@ -122,6 +123,12 @@ void craw_profiler_end(CRAWProfRun &r);
 void craw_profiler_finish(CRAWProfRun &r);
 int craw_profiler_times(CRAWContext *ctx, CRAWProfiler *pr, uint32_t frames_back,
                         CRAWStageTime *out, uint32_t max);
+
+/* Blocks of the AWB grid of a W x H frame. */
+static inline size_t craw_awb_blocks(uint32_t w, uint32_t h)
+{
+    return (size_t)((w + 7) / 8) * ((h + 7) / 8);
+}
 
 /* Device bytes a batch of `params` allocates (no allocation). */
 int craw_batch_footprint(CRAWContext *ctx, const CRAWBatchParams *params, uint64_t *bytes);

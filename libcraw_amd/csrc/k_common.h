@@ -133,10 +133,19 @@ struct KOut {
     void *dst;           /* RGBA16F or RGBA32F */
     size_t pitch;        /* bytes */
     int half;            /* 1: RGBA16F */
-    /* AWB histogram (NULL: none): count, weight, bright, each 128^2. */
+    /* AWB (hist NULL: none): the finish stage writes one packed sample
+     * per 8x8 block into `samples` (row-major over the frame's
+     * ceil(W/8) x ceil(H/8) blocks); the histogram stage adds them into
+     * `hist` (count, weight, bright, each 128^2). */
     uint32_t *hist;
+    uint2 *samples;
     float awb_wb0, awb_wb2;
 };
+
+/* A block's histogram sample, packed: x = bin | weight << 14 (bin < 2^14,
+ * weight <= 2^15), y = bright (<= 2^16); x = KAWB_NONE for a block the
+ * estimator skips. */
+#define KAWB_NONE 0xffffffffu
 
 KDEV void kstore_rgba(const KOut &o, int x, int y, const float v[3])
 {

@@ -24,41 +24,44 @@
 
 /* ------------------------------------------------------------ finish */
 
-/* AWB histogram of the tile's camera RGB (SoA in LDS) and the colour stage
+/* AWB samples of the tile's camera RGB (SoA in LDS) and the colour stage
  * into the destination. Expects the LDS tile complete (after a barrier).
- * Blocks are summed row by row as lib/awb.c:249-256 sums them. */
+ * Blocks are summed row by row as lib/awb.c:249-256 sums them, one lane of
+ * the first wave per block and channel; each block's packed sample goes
+ * to the frame's sample array for the histogram stage. */
 template <int TW, int TH, int NT>
 KDEV void kfinish(const float (*rgbs)[TH][TW], int x0, int y0, int W, int H,
                   const KColor &col, const KOut &out)
 {
     const int tid = threadIdx.x;
 
-    if (out.hist) {
-        constexpr int NBX = TW / 8, NBY = TH / 8;
+    if (out.hist && tid < 64) {
+        /* lane c * NB + b sums channel c of block b (64 sequential adds);
+         * the channel-0 lane gathers the other two and bins the block */
+        constexpr int NBX = TW / 8, NBY = TH / 8, NB = NBX * NBY;
+        static_assert(3 * NB <= 64, "one wave sums every block channel");
+        const int c = tid / NB, b = tid % NB;
+        const int gx0 = x0 + (b % NBX) * 8, gy0 = y0 + (b / NBX) * 8;
+        const int gx1 = gx0 + 8 < W ? gx0 + 8 : W, gy1 = gy0 + 8 < H ? gy0 + 8 : H;
+        const bool inside = c < 3 && gx0 < W && gy0 < H;
+        float sum = 0.0f, acc[3];
+        CRAWAwbSample smp = { 0u, 0u, 0u };
+        bool valid = false;
 
-        for (int b = tid; b < NBX * NBY; b += NT) {
-            int gx0 = x0 + (b % NBX) * 8, gy0 = y0 + (b / NBX) * 8;
-            int gx1, gy1;
-            float acc[3] = { 0.0f, 0.0f, 0.0f };
-            CRAWAwbSample s;
-
-            if (gx0 >= W || gy0 >= H)
-                continue;
-            gx1 = gx0 + 8 < W ? gx0 + 8 : W;
-            gy1 = gy0 + 8 < H ? gy0 + 8 : H;
-            for (int y = gy0; y < gy1; y++) {
-                for (int x = gx0; x < gx1; x++) {
-                    acc[0] += rgbs[0][y - y0][x - x0];
-                    acc[1] += rgbs[1][y - y0][x - x0];
-                    acc[2] += rgbs[2][y - y0][x - x0];
-                }
-            }
-            if (craw_awb_sample(acc, (float)((gy1 - gy0) * (gx1 - gx0)), out.awb_wb0,
-                                out.awb_wb2, &s)) {
-                atomicAdd(&out.hist[s.bin], 1u);
-                atomicAdd(&out.hist[16384 + s.bin], s.weight);
-                atomicAdd(&out.hist[32768 + s.bin], s.bright);
-            }
+        if (inside) {
+            for (int y = gy0; y < gy1; y++)
+                for (int x = gx0; x < gx1; x++)
+                    sum += rgbs[c][y - y0][x - x0];
+        }
+        acc[0] = sum;
+        acc[1] = __shfl(sum, tid + NB, 64);
+        acc[2] = __shfl(sum, tid + 2 * NB, 64);
+        if (inside && c == 0) {
+            valid = craw_awb_sample(acc, (float)((gy1 - gy0) * (gx1 - gx0)), out.awb_wb0,
+                                    out.awb_wb2, &smp);
+            out.samples[(size_t)(gy0 / 8) * ((W + 7) / 8) + gx0 / 8] =
+                valid ? make_uint2(smp.bin | (smp.weight << 14), smp.bright)
+                      : make_uint2(KAWB_NONE, 0u);
         }
     }
 
@@ -140,10 +143,11 @@ struct SmSimple {
     float rgbs[3][BL_TH][BL_TW];
 };
 
-template <int METHOD, bool INTER>
+template <int METHOD, bool INTER, bool INTERIOR>
 KDEV void dev_debayer_simple(SmSimple &sm, const KFrame &f, int x0, int y0)
 {
-    float (&vs)[BL_TH + 2 * BL_A][BL_TW + 2 * BL_A] = sm.vs;
+    constexpr int RW = BL_TW + 2 * BL_A, RH = BL_TH + 2 * BL_A;
+    float (&vs)[RH][RW] = sm.vs;
     float (&rgbs)[3][BL_TH][BL_TW] = sm.rgbs;
     const uint16_t *__restrict__ src = f.src;
     const int src_pitch = f.src_pitch, W = f.W, H = f.H;
@@ -152,11 +156,19 @@ KDEV void dev_debayer_simple(SmSimple &sm, const KFrame &f, int x0, int y0)
     const int xe = min(x0 + BL_TW, W), ye = min(y0 + BL_TH, H);
     const int rw = xe - x0 + 2 * BL_A, rh = ye - y0 + 2 * BL_A;
 
-    for (int i = tid; i < rw * rh; i += BL_NT) {
-        int lx = i % rw, ly = i / rw;
-        int mx = kmirror(x0 - BL_A + lx, W), my = kmirror(y0 - BL_A + ly, H);
-        int s = (my & 1) * 2 + (mx & 1);
+    /* compile-time strides; in an interior tile no position needs the
+     * mirror */
+    for (int i = tid; i < RW * RH; i += BL_NT) {
+        int lx = i % RW, ly = i / RW;
+        int mx = x0 - BL_A + lx, my = y0 - BL_A + ly, s;
 
+        if (!INTERIOR) {
+            if (lx >= rw || ly >= rh)
+                continue;
+            mx = kmirror(mx, W);
+            my = kmirror(my, H);
+        }
+        s = (my & 1) * 2 + (mx & 1);
         vs[ly][lx] = kprep(src[(size_t)my * src_pitch + mx], s, prep);
     }
     __syncthreads();
@@ -214,6 +226,15 @@ KDEV void dev_debayer_simple(SmSimple &sm, const KFrame &f, int x0, int y0)
         kfinish<BL_TW, BL_TH, BL_NT>(rgbs, x0, y0, W, H, f.col, f.out);
 }
 
+template <int METHOD, bool INTER>
+KDEV void dev_debayer_simple_tile(SmSimple &sm, const KFrame &f, int x0, int y0)
+{
+    if (x0 >= BL_A && y0 >= BL_A && x0 + BL_TW + BL_A <= f.W && y0 + BL_TH + BL_A <= f.H)
+        dev_debayer_simple<METHOD, INTER, true>(sm, f, x0, y0);
+    else
+        dev_debayer_simple<METHOD, INTER, false>(sm, f, x0, y0);
+}
+
 /* --------------------------------------------------------------- RCD */
 
 #define RC_TW 32
@@ -253,7 +274,25 @@ struct SmRcd {
     float scp[RcStage<1>::N];
 };
 
-template <bool INTER>
+/* A stage loop over the full (RC_TW + 2A) x (RC_TH + 2A) region of a
+ * stage with apron A, with compile-time strides. In an interior tile
+ * (every stage's region inside the frame) a position is its own mirrored
+ * position; elsewhere positions past the partial tile's own reach are
+ * skipped and the rest are mirrored. Sets lx, ly (region coordinates) and
+ * mx, my (the mirrored frame position). */
+#define RC_STAGE_BEGIN(A)                                                                  \
+    for (int i_ = tid; i_ < RcStage<A>::N; i_ += RC_NT) {                                  \
+        const int lx = i_ % RcStage<A>::W, ly = i_ / RcStage<A>::W;                        \
+        int mx = x0 - (A) + lx, my = y0 - (A) + ly;                                        \
+        if (!INTERIOR) {                                                                   \
+            if (lx >= tw + 2 * (A) || ly >= th + 2 * (A))                                  \
+                continue;                                                                  \
+            mx = kmirror(mx, W);                                                           \
+            my = kmirror(my, H);                                                           \
+        }
+#define RC_STAGE_END }
+
+template <bool INTER, bool INTERIOR>
 KDEV void dev_debayer_rcd(SmRcd &sm, const KFrame &f, int x0, int y0)
 {
     typedef RcStage<5> S5;
@@ -278,131 +317,109 @@ KDEV void dev_debayer_rcd(SmRcd &sm, const KFrame &f, int x0, int y0)
 #define C(dx, dy) AT(scp, 1, mx + (dx), my + (dy))
 
     /* V: prep, apron 5 */
-    {
-        const int rw = tw + 10, rh = th + 10;
-        for (int i = tid; i < rw * rh; i += RC_NT) {
-            int lx = i % rw, ly = i / rw;
-            int mx = kmirror(x0 - 5 + lx, W), my = kmirror(y0 - 5 + ly, H);
-            int s = (my & 1) * 2 + (mx & 1);
-            sv[ly * S5::W + lx] = kprep(src[(size_t)my * src_pitch + mx], s, prep);
-        }
-    }
+    RC_STAGE_BEGIN(5)
+        int s = (my & 1) * 2 + (mx & 1);
+        sv[ly * S5::W + lx] = kprep(src[(size_t)my * src_pitch + mx], s, prep);
+    RC_STAGE_END
     __syncthreads();
 
     /* Pass A: low pass (apron 4) and the four gradients (apron 3). */
-    {
-        const int rw = tw + 8, rh = th + 8;
-        for (int i = tid; i < rw * rh; i += RC_NT) {
-            int lx = i % rw, ly = i / rw;
-            int vx = x0 - 4 + lx, vy = y0 - 4 + ly;
-            int mx = kmirror(vx, W), my = kmirror(vy, H);
-
-            slp[ly * S4::W + lx] =
-                (4.0f * V(0, 0)
-                 + 2.0f * (V(0, -1) + V(0, 1) + V(-1, 0) + V(1, 0))
-                 + V(-1, -1) + V(1, -1) + V(-1, 1) + V(1, 1)) * (1.0f / 16.0f);
-            if (lx >= 1 && ly >= 1 && lx < rw - 1 && ly < rh - 1) {
-                int j = (ly - 1) * S3::W + (lx - 1);
-                sg[0][j] = fabsf(V(0, -1) - V(0, 1))
-                         + fabsf(2.0f * V(0, 0) - V(0, -2) - V(0, 2));
-                sg[1][j] = fabsf(V(-1, 0) - V(1, 0))
-                         + fabsf(2.0f * V(0, 0) - V(-2, 0) - V(2, 0));
-                sg[2][j] = fabsf(V(-1, 1) - V(1, -1))
-                         + fabsf(2.0f * V(0, 0) - V(-2, 2) - V(2, -2));
-                sg[3][j] = fabsf(V(-1, -1) - V(1, 1))
-                         + fabsf(2.0f * V(0, 0) - V(-2, -2) - V(2, 2));
-            }
+    RC_STAGE_BEGIN(4)
+        slp[ly * S4::W + lx] =
+            (4.0f * V(0, 0)
+             + 2.0f * (V(0, -1) + V(0, 1) + V(-1, 0) + V(1, 0))
+             + V(-1, -1) + V(1, -1) + V(-1, 1) + V(1, 1)) * (1.0f / 16.0f);
+        if (lx >= 1 && ly >= 1 && lx < S4::W - 1 && ly < S4::H - 1 &&
+            (INTERIOR || (lx < tw + 7 && ly < th + 7))) {
+            int j = (ly - 1) * S3::W + (lx - 1);
+            sg[0][j] = fabsf(V(0, -1) - V(0, 1))
+                     + fabsf(2.0f * V(0, 0) - V(0, -2) - V(0, 2));
+            sg[1][j] = fabsf(V(-1, 0) - V(1, 0))
+                     + fabsf(2.0f * V(0, 0) - V(-2, 0) - V(2, 0));
+            sg[2][j] = fabsf(V(-1, 1) - V(1, -1))
+                     + fabsf(2.0f * V(0, 0) - V(-2, 2) - V(2, -2));
+            sg[3][j] = fabsf(V(-1, -1) - V(1, 1))
+                     + fabsf(2.0f * V(0, 0) - V(-2, -2) - V(2, 2));
         }
-    }
+    RC_STAGE_END
     __syncthreads();
 
     /* Pass A: direction weights, wv over apron 2 and wp over apron 1, from
      * 3x3 means of the gradients summed row by row
      * (lib/ref/ref_debayer.c:179-190). */
-    {
-        const int rw = tw + 4, rh = th + 4;
-        for (int i = tid; i < rw * rh; i += RC_NT) {
-            int lx = i % rw, ly = i / rw;
-            int vx = x0 - 2 + lx, vy = y0 - 2 + ly;
-            int mx = kmirror(vx, W), my = kmirror(vy, H);
-            float a0 = 0.0f, a1 = 0.0f;
+    RC_STAGE_BEGIN(2)
+        float a0 = 0.0f, a1 = 0.0f;
 
+#pragma unroll
+        for (int dy = -1; dy <= 1; dy++) {
+#pragma unroll
+            for (int dx = -1; dx <= 1; dx++) {
+                a0 += AT(sg[0], 3, mx + dx, my + dy);
+                a1 += AT(sg[1], 3, mx + dx, my + dy);
+            }
+        }
+        {
+            float vsum = a0 * (1.0f / 9.0f), hsum = a1 * (1.0f / 9.0f);
+            swv[ly * S2::W + lx] = hsum / (hsum + vsum + RCD_EPS);
+        }
+        if (lx >= 1 && ly >= 1 && lx < S2::W - 1 && ly < S2::H - 1 &&
+            (INTERIOR || (lx < tw + 3 && ly < th + 3))) {
+            float b0 = 0.0f, b1 = 0.0f;
+
+#pragma unroll
             for (int dy = -1; dy <= 1; dy++) {
+#pragma unroll
                 for (int dx = -1; dx <= 1; dx++) {
-                    a0 += AT(sg[0], 3, mx + dx, my + dy);
-                    a1 += AT(sg[1], 3, mx + dx, my + dy);
+                    b0 += AT(sg[2], 3, mx + dx, my + dy);
+                    b1 += AT(sg[3], 3, mx + dx, my + dy);
                 }
             }
             {
-                float vsum = a0 * (1.0f / 9.0f), hsum = a1 * (1.0f / 9.0f);
-                swv[ly * S2::W + lx] = hsum / (hsum + vsum + RCD_EPS);
-            }
-            if (lx >= 1 && ly >= 1 && lx < rw - 1 && ly < rh - 1) {
-                float b0 = 0.0f, b1 = 0.0f;
-
-                for (int dy = -1; dy <= 1; dy++) {
-                    for (int dx = -1; dx <= 1; dx++) {
-                        b0 += AT(sg[2], 3, mx + dx, my + dy);
-                        b1 += AT(sg[3], 3, mx + dx, my + dy);
-                    }
-                }
-                {
-                    float ps = b0 * (1.0f / 9.0f), qs = b1 * (1.0f / 9.0f);
-                    swp[(ly - 1) * S1::W + (lx - 1)] = qs / (ps + qs + RCD_EPS);
-                }
+                float ps = b0 * (1.0f / 9.0f), qs = b1 * (1.0f / 9.0f);
+                swp[(ly - 1) * S1::W + (lx - 1)] = qs / (ps + qs + RCD_EPS);
             }
         }
-    }
+    RC_STAGE_END
     __syncthreads();
 
     /* Pass B: green, apron 2. */
-    {
-        const int rw = tw + 4, rh = th + 4;
-        for (int i = tid; i < rw * rh; i += RC_NT) {
-            int lx = i % rw, ly = i / rw;
-            int mx = kmirror(x0 - 2 + lx, W), my = kmirror(y0 - 2 + ly, H);
-            int s = (my & 1) * 2 + (mx & 1);
-            float gval;
+    RC_STAGE_BEGIN(2)
+        int s = (my & 1) * 2 + (mx & 1);
+        float gval;
 
-            if (s == 1 || s == 2) {
-                gval = V(0, 0);
-            } else {
-                float l00 = L(0, 0), lu = L(0, -2), ld = L(0, 2), ll = L(-2, 0), lr = L(2, 0);
-                float gv = 0.5f * (V(0, -1) * (1.0f + rcd_ratio(l00, lu))
-                                 + V(0, 1) * (1.0f + rcd_ratio(l00, ld)));
-                float gh = 0.5f * (V(-1, 0) * (1.0f + rcd_ratio(l00, ll))
-                                 + V(1, 0) * (1.0f + rcd_ratio(l00, lr)));
-                float w = AT(swv, 2, mx, my);
-                gval = w * gv + (1.0f - w) * gh;
-            }
-            sgp[ly * S2::W + lx] = gval;
+        if (s == 1 || s == 2) {
+            gval = V(0, 0);
+        } else {
+            float l00 = L(0, 0), lu = L(0, -2), ld = L(0, 2), ll = L(-2, 0), lr = L(2, 0);
+            float gv = 0.5f * (V(0, -1) * (1.0f + rcd_ratio(l00, lu))
+                             + V(0, 1) * (1.0f + rcd_ratio(l00, ld)));
+            float gh = 0.5f * (V(-1, 0) * (1.0f + rcd_ratio(l00, ll))
+                             + V(1, 0) * (1.0f + rcd_ratio(l00, lr)));
+            float w = AT(swv, 2, mx, my);
+            gval = w * gv + (1.0f - w) * gh;
         }
-    }
+        sgp[ly * S2::W + lx] = gval;
+    RC_STAGE_END
     __syncthreads();
 
     /* Pass C, first half: colour estimate at the red and blue sites,
      * apron 1. */
-    {
-        const int rw = tw + 2, rh = th + 2;
-        for (int i = tid; i < rw * rh; i += RC_NT) {
-            int lx = i % rw, ly = i / rw;
-            int mx = kmirror(x0 - 1 + lx, W), my = kmirror(y0 - 1 + ly, H);
-            int s = (my & 1) * 2 + (mx & 1);
-            float c = 0.0f;
+    RC_STAGE_BEGIN(1)
+        int s = (my & 1) * 2 + (mx & 1);
+        float c = 0.0f;
 
-            if (s == 0 || s == 3) {
-                float g00 = G(0, 0);
-                float dsw = V(-1, 1) - G(-1, 1);
-                float dne = V(1, -1) - G(1, -1);
-                float dnw = V(-1, -1) - G(-1, -1);
-                float dse = V(1, 1) - G(1, 1);
-                float w = AT(swp, 1, mx, my);
+        if (s == 0 || s == 3) {
+            float g00 = G(0, 0);
+            float dsw = V(-1, 1) - G(-1, 1);
+            float dne = V(1, -1) - G(1, -1);
+            float dnw = V(-1, -1) - G(-1, -1);
+            float dse = V(1, 1) - G(1, 1);
+            float w = AT(swp, 1, mx, my);
 
-                c = w * (g00 + 0.5f * (dsw + dne)) + (1.0f - w) * (g00 + 0.5f * (dnw + dse));
-            }
-            scp[ly * S1::W + lx] = c;
+            c = w * (g00 + 0.5f * (dsw + dne)) + (1.0f - w) * (g00 + 0.5f * (dnw + dse));
         }
-    }
+        scp[ly * S1::W + lx] = c;
+    RC_STAGE_END
     __syncthreads();
 
     /* Output: RGB of the tile's own pixels into the gradient space. */
@@ -454,6 +471,23 @@ KDEV void dev_debayer_rcd(SmRcd &sm, const KFrame &f, int x0, int y0)
 #undef G
 #undef C
 }
+#undef RC_STAGE_BEGIN
+#undef RC_STAGE_END
+
+/* Whether every stage region of the tile at (x0, y0) lies in the frame. */
+KDEV bool rcd_interior(const KFrame &f, int x0, int y0)
+{
+    return x0 >= 5 && y0 >= 5 && x0 + RC_TW + 5 <= f.W && y0 + RC_TH + 5 <= f.H;
+}
+
+template <bool INTER>
+KDEV void dev_debayer_rcd_tile(SmRcd &sm, const KFrame &f, int x0, int y0)
+{
+    if (rcd_interior(f, x0, y0))
+        dev_debayer_rcd<INTER, true>(sm, f, x0, y0);
+    else
+        dev_debayer_rcd<INTER, false>(sm, f, x0, y0);
+}
 
 /* ------------------------------------------------------------ chroma */
 
@@ -461,51 +495,75 @@ KDEV void dev_debayer_rcd(SmRcd &sm, const KFrame &f, int x0, int y0)
 #define CB_TY 16
 
 /* lib/ref/ref_chroma.c:86-117: 9x9 joint bilateral on the quarter grid,
- * guided by G, one reflection at the borders, summed row by row. */
-KDEV void dev_chroma_blur(const KFrame &f, int qx0, int qy0)
+ * guided by G, one reflection at the borders, summed row by row. The
+ * tile's three quarter-grid planes are staged in LDS with the 4-sample
+ * reach. */
+#define CB_R 4
+#define CB_LW (CB_TX + 2 * CB_R)
+#define CB_LH (CB_TY + 2 * CB_R)
+
+struct SmCb {
+    float q[3][CB_LH][CB_LW];
+};
+
+KDEV void dev_chroma_blur(SmCb &sm, const KFrame &f, int qx0, int qy0)
 {
     const KInter &it = f.inter;
-    const int qx = qx0 + threadIdx.x % CB_TX;
-    const int qy = qy0 + threadIdx.x / CB_TX;
+    const int tid = threadIdx.x;
+    const int lx = tid % CB_TX, ly = tid / CB_TX;
+    const int qx = qx0 + lx, qy = qy0 + ly;
     const size_t qplane = (size_t)it.wq * it.hq;
-    const float *d0 = it.down, *d1 = it.down + qplane, *d2 = it.down + 2 * qplane;
+    const int wq = it.wq, hq = it.hq;
     float c0, c1, c2, level, inv_range, inv_chroma;
     float acc0 = 0.0f, acc1 = 0.0f, wsum = 0.0f;
 
-    if (qx >= it.wq || qy >= it.hq)
+    /* positions up to the reach past the last valid sample, mirrored once
+     * (wq, hq >= 5) */
+    for (int i = tid; i < CB_LW * CB_LH; i += CB_TX * CB_TY) {
+        const int sx0 = i % CB_LW, sy0 = i / CB_LW;
+        int sx = qx0 - CB_R + sx0, sy = qy0 - CB_R + sy0;
+        size_t si;
+
+        if (sx >= wq + CB_R || sy >= hq + CB_R)
+            continue;
+        sx = sx < 0 ? -sx : (sx >= wq ? 2 * wq - 2 - sx : sx);
+        sy = sy < 0 ? -sy : (sy >= hq ? 2 * hq - 2 - sy : sy);
+        si = (size_t)sy * wq + sx;
+        sm.q[0][sy0][sx0] = it.down[si];
+        sm.q[1][sy0][sx0] = it.down[qplane + si];
+        sm.q[2][sy0][sx0] = it.down[2 * qplane + si];
+    }
+    __syncthreads();
+
+    if (qx >= wq || qy >= hq)
         return;
-    c0 = d0[(size_t)qy * it.wq + qx];
-    c1 = d1[(size_t)qy * it.wq + qx];
-    c2 = d2[(size_t)qy * it.wq + qx];
+    c0 = sm.q[0][ly + CB_R][lx + CB_R];
+    c1 = sm.q[1][ly + CB_R][lx + CB_R];
+    c2 = sm.q[2][ly + CB_R][lx + CB_R];
     level = (c2 > 0.0f ? c2 : 0.0f) + 0.02f;
     inv_range = 1.0f / (0.35f * level);
     inv_chroma = 1.0f / (0.35f * level);
 
-    for (int dy = -4; dy <= 4; dy++) {
-        int sy = qy + dy;
-        sy = sy < 0 ? -sy : (sy >= it.hq ? 2 * it.hq - 2 - sy : sy);
-        for (int dx = -4; dx <= 4; dx++) {
-            int sx = qx + dx;
-            size_t si;
-            float s0, s1, s2, dg, dd0, dd1, wgt;
+#pragma unroll 1
+    for (int dy = -CB_R; dy <= CB_R; dy++) {
+#pragma unroll
+        for (int dx = -CB_R; dx <= CB_R; dx++) {
+            const float s0 = sm.q[0][ly + CB_R + dy][lx + CB_R + dx];
+            const float s1 = sm.q[1][ly + CB_R + dy][lx + CB_R + dx];
+            const float s2 = sm.q[2][ly + CB_R + dy][lx + CB_R + dx];
+            const float dg = (s2 - c2) * inv_range;
+            const float dd0 = (s0 - c0) * inv_chroma;
+            const float dd1 = (s1 - c1) * inv_chroma;
+            const float wgt = craw_dm_expf(-(float)(dx * dx + dy * dy) * 0.125f - dg * dg
+                                           - dd0 * dd0 - dd1 * dd1);
 
-            sx = sx < 0 ? -sx : (sx >= it.wq ? 2 * it.wq - 2 - sx : sx);
-            si = (size_t)sy * it.wq + sx;
-            s0 = d0[si];
-            s1 = d1[si];
-            s2 = d2[si];
-            dg = (s2 - c2) * inv_range;
-            dd0 = (s0 - c0) * inv_chroma;
-            dd1 = (s1 - c1) * inv_chroma;
-            wgt = craw_dm_expf(-(float)(dx * dx + dy * dy) * 0.125f - dg * dg - dd0 * dd0
-                               - dd1 * dd1);
             acc0 += wgt * s0;
             acc1 += wgt * s1;
             wsum += wgt;
         }
     }
-    it.blur[(size_t)qy * it.wq + qx] = acc0 / wsum;
-    it.blur[qplane + (size_t)qy * it.wq + qx] = acc1 / wsum;
+    it.blur[(size_t)qy * wq + qx] = acc0 / wsum;
+    it.blur[qplane + (size_t)qy * wq + qx] = acc1 / wsum;
 }
 
 #define CU_TW 64
@@ -583,25 +641,141 @@ KDEV void dev_chroma_up_finish(SmUp &sm, const KFrame &f, int x0, int y0)
     kfinish<CU_TW, CU_TH, CU_NT>(rgbs, x0, y0, W, H, f.col, f.out);
 }
 
+/* ------------------------------------------------------ AWB histogram */
+
+/* The histogram of one chunk of a frame's block samples, privatised in an
+ * LDS hash of bins: each lane merges the runs of equal bins among its 32
+ * consecutive samples (neighbouring blocks mostly share a bin), inserts
+ * each run into the hash with LDS atomics, and the workgroup then adds
+ * every occupied entry to the frame's histogram with three global
+ * atomics. A run whose bin finds no slot within AWB_PROBES probes goes to
+ * the global histogram directly, so the stage terminates whatever the
+ * data. Integer sums: the result does not depend on the grouping. */
+#define AWB_NT 256
+#define AWB_PER (KG_AWB_CHUNK / AWB_NT)
+#define AWB_HASH 4096
+#define AWB_PROBES 32
+
+struct SmAwb {
+    uint32_t key[AWB_HASH];
+    uint32_t cnt[AWB_HASH];
+    uint32_t w[AWB_HASH];
+    uint32_t b[AWB_HASH];
+};
+
+KDEV void awb_insert(SmAwb &sm, uint32_t *hist, uint32_t bin, uint32_t c, uint32_t w,
+                     uint32_t b)
+{
+    uint32_t h = (bin * 0x9e3779b1u) >> 20;
+
+    for (int probe = 0; probe < AWB_PROBES; probe++) {
+        const uint32_t k = atomicCAS(&sm.key[h], KAWB_NONE, bin);
+
+        if (k == KAWB_NONE || k == bin) {
+            atomicAdd(&sm.cnt[h], c);
+            atomicAdd(&sm.w[h], w);
+            atomicAdd(&sm.b[h], b);
+            return;
+        }
+        h = (h + 1) & (AWB_HASH - 1);
+    }
+    atomicAdd(&hist[bin], c);
+    atomicAdd(&hist[16384 + bin], w);
+    atomicAdd(&hist[32768 + bin], b);
+}
+
+KDEV void dev_awb_hist(SmAwb &sm, const KFrame &f, int chunk)
+{
+    const int tid = threadIdx.x;
+    const int nblocks = ((f.W + 7) / 8) * ((f.H + 7) / 8);
+    const int i0 = chunk * KG_AWB_CHUNK + tid * AWB_PER;
+    const uint2 *__restrict__ smp = f.out.samples;
+    uint32_t *hist = f.out.hist;
+    uint32_t rb = KAWB_NONE, rc = 0, rw = 0, rbr = 0;
+
+    for (int i = tid; i < AWB_HASH; i += AWB_NT) {
+        sm.key[i] = KAWB_NONE;
+        sm.cnt[i] = 0u;
+        sm.w[i] = 0u;
+        sm.b[i] = 0u;
+    }
+    __syncthreads();
+
+    for (int k = 0; k < AWB_PER && i0 + k < nblocks; k++) {
+        const uint2 v = smp[i0 + k];
+        uint32_t bin;
+
+        if (v.x == KAWB_NONE)
+            continue;
+        bin = v.x & 0x3fffu;
+        if (bin == rb) {
+            rc += 1u;
+            rw += v.x >> 14;
+            rbr += v.y;
+        } else {
+            if (rb != KAWB_NONE)
+                awb_insert(sm, hist, rb, rc, rw, rbr);
+            rb = bin;
+            rc = 1u;
+            rw = v.x >> 14;
+            rbr = v.y;
+        }
+    }
+    if (rb != KAWB_NONE)
+        awb_insert(sm, hist, rb, rc, rw, rbr);
+    __syncthreads();
+
+    for (int i = tid; i < AWB_HASH; i += AWB_NT) {
+        const uint32_t k = sm.key[i];
+
+        if (k != KAWB_NONE) {
+            atomicAdd(&hist[k], sm.cnt[i]);
+            atomicAdd(&hist[16384 + k], sm.w[i]);
+            atomicAdd(&hist[32768 + k], sm.b[i]);
+        }
+    }
+}
+
+__global__ __launch_bounds__(AWB_NT) void k_awb_hist(KFrame f)
+{
+    __shared__ SmAwb sm;
+    dev_awb_hist(sm, f, blockIdx.x);
+}
+
+__global__ __launch_bounds__(AWB_NT) void kb_awb_hist(const KFrame *__restrict__ frames,
+                                                      const uint32_t *__restrict__ tiles,
+                                                      uint32_t ntiles)
+{
+    __shared__ SmAwb sm;
+
+    for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const uint32_t d = tiles[t];
+
+        __syncthreads();
+        dev_awb_hist(sm, frames[KTILE_FRAME(d)], KTILE_IDX(d));
+    }
+}
+
 /* ------------------------------------------------- plain and batched */
 
 template <int METHOD, bool INTER>
 __global__ __launch_bounds__(BL_NT) void k_debayer_simple(KFrame f)
 {
     __shared__ SmSimple sm;
-    dev_debayer_simple<METHOD, INTER>(sm, f, blockIdx.x * BL_TW, blockIdx.y * BL_TH);
+    dev_debayer_simple_tile<METHOD, INTER>(sm, f, blockIdx.x * BL_TW, blockIdx.y * BL_TH);
 }
 
 template <bool INTER>
 __global__ __launch_bounds__(RC_NT) void k_debayer_rcd(KFrame f)
 {
     __shared__ SmRcd sm;
-    dev_debayer_rcd<INTER>(sm, f, blockIdx.x * RC_TW, blockIdx.y * RC_TH);
+    dev_debayer_rcd_tile<INTER>(sm, f, blockIdx.x * RC_TW, blockIdx.y * RC_TH);
 }
 
 __global__ __launch_bounds__(CB_TX * CB_TY) void k_chroma_blur(KFrame f)
 {
-    dev_chroma_blur(f, blockIdx.x * CB_TX, blockIdx.y * CB_TY);
+    __shared__ SmCb sm;
+    dev_chroma_blur(sm, f, blockIdx.x * CB_TX, blockIdx.y * CB_TY);
 }
 
 __global__ __launch_bounds__(CU_NT) void k_chroma_up_finish(KFrame f)
@@ -627,7 +801,7 @@ __global__ __launch_bounds__(BL_NT) void kb_debayer_simple(const KFrame *__restr
         const int ntx = (f.W + BL_TW - 1) / BL_TW, i = KTILE_IDX(d);
 
         __syncthreads();
-        dev_debayer_simple<METHOD, INTER>(sm, f, (i % ntx) * BL_TW, (i / ntx) * BL_TH);
+        dev_debayer_simple_tile<METHOD, INTER>(sm, f, (i % ntx) * BL_TW, (i / ntx) * BL_TH);
     }
 }
 
@@ -644,7 +818,7 @@ __global__ __launch_bounds__(RC_NT) void kb_debayer_rcd(const KFrame *__restrict
         const int ntx = (f.W + RC_TW - 1) / RC_TW, i = KTILE_IDX(d);
 
         __syncthreads();
-        dev_debayer_rcd<INTER>(sm, f, (i % ntx) * RC_TW, (i / ntx) * RC_TH);
+        dev_debayer_rcd_tile<INTER>(sm, f, (i % ntx) * RC_TW, (i / ntx) * RC_TH);
     }
 }
 
@@ -652,12 +826,15 @@ __global__ __launch_bounds__(CB_TX * CB_TY) void kb_chroma_blur(const KFrame *__
                                                                 const uint32_t *__restrict__ tiles,
                                                                 uint32_t ntiles)
 {
+    __shared__ SmCb sm;
+
     for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const uint32_t d = tiles[t];
         const KFrame &f = frames[KTILE_FRAME(d)];
         const int ntx = (f.inter.wq + CB_TX - 1) / CB_TX, i = KTILE_IDX(d);
 
-        dev_chroma_blur(f, (i % ntx) * CB_TX, (i / ntx) * CB_TY);
+        __syncthreads();
+        dev_chroma_blur(sm, f, (i % ntx) * CB_TX, (i / ntx) * CB_TY);
     }
 }
 
@@ -771,6 +948,23 @@ hipError_t craw_launch_batch_chroma_blur(const KFrame *frames, const uint32_t *t
         return hipSuccess;
     hipLaunchKernelGGL(kb_chroma_blur, dim3(grid), dim3(CB_TX * CB_TY), 0, s, frames, tiles,
                        ntiles);
+    return hipGetLastError();
+}
+
+hipError_t craw_launch_awb_hist(const KFrame &f, hipStream_t s)
+{
+    int ntx, n = craw_stage_tiles(KSTAGE_AWB, f.W, f.H, 0, false, &ntx);
+
+    hipLaunchKernelGGL(k_awb_hist, dim3(n), dim3(AWB_NT), 0, s, f);
+    return hipGetLastError();
+}
+
+hipError_t craw_launch_batch_awb_hist(const KFrame *frames, const uint32_t *tiles,
+                                      uint32_t ntiles, int grid, hipStream_t s)
+{
+    if (!ntiles)
+        return hipSuccess;
+    hipLaunchKernelGGL(kb_awb_hist, dim3(grid), dim3(AWB_NT), 0, s, frames, tiles, ntiles);
     return hipGetLastError();
 }
 

@@ -90,12 +90,14 @@ struct KFrame {
 #define KG_DN_TH 16
 #define KG_FT_TW 28
 #define KG_FT_TH 64
+#define KG_AWB_CHUNK 8192
 
 enum KStage {
     KSTAGE_DENOISE = 0,
     KSTAGE_DEBAYER,
     KSTAGE_CHROMA_BLUR,
     KSTAGE_CHROMA_UP,
+    KSTAGE_AWB,
     KSTAGE_NB,
 };
 
@@ -126,6 +128,12 @@ static inline void craw_stage_geom(int stage, int method, bool fast, int *tw, in
         *tw = KG_CU_TW;
         *th = KG_CU_TH;
         break;
+    case KSTAGE_AWB:
+        /* the 8x8-block grid in chunks of KG_AWB_CHUNK blocks, one row of
+         * "tiles" per chunk */
+        *div = 8;
+        *tw = KG_AWB_CHUNK;
+        break;
     }
 }
 
@@ -135,6 +143,11 @@ static inline int craw_stage_tiles(int stage, int W, int H, int method, bool fas
 {
     int tw, th, div, planes, w, h;
 
+    if (stage == KSTAGE_AWB) {
+        int nblocks = ((W + 7) / 8) * ((H + 7) / 8);
+        *ntx = (nblocks + KG_AWB_CHUNK - 1) / KG_AWB_CHUNK;
+        return *ntx;
+    }
     craw_stage_geom(stage, method, fast, &tw, &th, &div, &planes);
     w = (W + div - 1) / div;
     h = (H + div - 1) / div;
@@ -148,6 +161,8 @@ hipError_t craw_launch_zero_u32(uint32_t *p, size_t n, hipStream_t s);
 hipError_t craw_launch_debayer(bool to_inter, const KFrame &f, hipStream_t s);
 hipError_t craw_launch_chroma_blur(const KFrame &f, hipStream_t s);
 hipError_t craw_launch_chroma_up_finish(const KFrame &f, hipStream_t s);
+/* The AWB histogram from the finish stage's block samples. */
+hipError_t craw_launch_awb_hist(const KFrame &f, hipStream_t s);
 
 /* mode: KDN_* bits. The FP32 form that reproduces the CPU reference bit
  * for bit. */
@@ -169,6 +184,8 @@ hipError_t craw_launch_batch_debayer(int method, bool to_inter, const KFrame *fr
                                      hipStream_t s);
 hipError_t craw_launch_batch_chroma_blur(const KFrame *frames, const uint32_t *tiles,
                                          uint32_t ntiles, int grid, hipStream_t s);
+hipError_t craw_launch_batch_awb_hist(const KFrame *frames, const uint32_t *tiles,
+                                      uint32_t ntiles, int grid, hipStream_t s);
 hipError_t craw_launch_batch_chroma_up(const KFrame *frames, const uint32_t *tiles,
                                        uint32_t ntiles, int grid, hipStream_t s);
 

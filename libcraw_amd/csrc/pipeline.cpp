@@ -9,10 +9,10 @@
  *
  * A recording plans the frame on the host (craw_frame_plan(), which fails
  * without side effects), then enqueues on the caller's stream: histogram
- * clear, denoise, debayer (fused with the colour stage and the histogram
- * when there is no chroma denoise), the chroma bilateral and the
- * chroma/histogram/colour pass, and the histogram's copy into the next
- * pinned readback slot.
+ * clear, denoise, debayer (fused with the colour stage and the AWB block
+ * samples when there is no chroma denoise), the chroma bilateral and the
+ * chroma/samples/colour pass, the histogram of the block samples, and the
+ * histogram's copy into the next pinned readback slot.
  */
 #include <stdlib.h>
 #include <string.h>
@@ -57,6 +57,7 @@ CRAW_API void craw_pipeline_free(CRAWPipeline **pipe)
     CRAW_IGNORE(hipFree(p->buf.down));
     CRAW_IGNORE(hipFree(p->buf.blur));
     CRAW_IGNORE(hipFree(p->buf.hist));
+    CRAW_IGNORE(hipFree(p->buf.awb_samples));
     craw_temporal_release(&p->ts);
     for (int i = 0; i < 4; i++)
         if (p->hist_host[i])
@@ -148,7 +149,9 @@ CRAW_API int craw_pipeline_create(CRAWContext *ctx, const CRAWPipelineParams *pa
     }
     if (p->cfg.awb) {
         p->nb_slots = params->nb_awb_slots ? params->nb_awb_slots : 1;
-        if ((ret = dmalloc((void **)&p->buf.hist, CRAW_HIST_WORDS * sizeof(uint32_t))) < 0)
+        if ((ret = dmalloc((void **)&p->buf.hist, CRAW_HIST_WORDS * sizeof(uint32_t))) < 0 ||
+            (ret = dmalloc((void **)&p->buf.awb_samples,
+                           craw_awb_blocks(params->width, params->height) * sizeof(uint2))) < 0)
             goto fail;
         for (uint32_t i = 0; i < p->nb_slots; i++) {
             if (hipHostMalloc((void **)&p->hist_host[i], CRAW_HIST_WORDS * sizeof(uint32_t),
@@ -218,6 +221,11 @@ static int pipeline_enqueue(CRAWPipeline *p, const CRAWFramePlan *plan, hipStrea
             e = craw_launch_chroma_up_finish(f, stream);
             craw_profiler_end(pr);
         }
+    }
+    if (e == hipSuccess && p->cfg.awb) {
+        craw_profiler_begin(pr, "awb_hist");
+        e = craw_launch_awb_hist(f, stream);
+        craw_profiler_end(pr);
     }
     if (e == hipSuccess && p->cfg.awb) {
         uint32_t *slot = p->hist_host[p->nb_recorded % p->nb_slots];

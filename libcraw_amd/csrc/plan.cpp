@@ -281,6 +281,9 @@ int craw_frame_plan(CRAWContext *ctx, const CRAWStageConfig &cfg, const CRAWFram
     plan->f.out.pitch = dst->pitch;
     plan->f.out.half = dst->format == CRAW_FORMAT_R16G16B16A16_SFLOAT;
     plan->f.out.hist = cfg.awb ? buf.hist : NULL;
+    plan->f.out.samples = cfg.awb ? buf.awb_samples : NULL;
+    if (cfg.awb && (!buf.hist || !buf.awb_samples))
+        return CRAW_ERROR(EINVAL);
     plan->f.out.awb_wb0 = frame.wb[0];
     plan->f.out.awb_wb2 = frame.wb[2];
 
