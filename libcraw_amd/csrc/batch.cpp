@@ -75,12 +75,12 @@ struct CRAWBatch {
  * w x h grid samples has at most (w/tw + 1)(h/th + 1) tiles per plane, and
  * w <= W/div + 1, h <= H/div + 1; summed over at most max_frames frames of
  * at most max_pixels pixels together. */
-static size_t stage_tiles_bound(int stage, const CRAWBatchParams *p, int method, bool fast)
+static size_t stage_tiles_bound(int stage, const CRAWBatchParams *p, int method, int dn_geom)
 {
     int tw, th, div, planes;
     double F = p->max_frames, wm, hm, area, n;
 
-    craw_stage_geom(stage, method, fast, &tw, &th, &div, &planes);
+    craw_stage_geom(stage, method, dn_geom, &tw, &th, &div, &planes);
     wm = (double)p->max_width / div + 1.0;
     hm = (double)p->max_height / div + 1.0;
     area = (double)p->max_pixels / ((double)div * div) + F * (wm + hm);
@@ -182,7 +182,12 @@ static int batch_sizes(CRAWContext *ctx, const CRAWBatchParams *params, CRAWStag
     }
     sz->frames = params->max_frames * sizeof(KFrame);
     for (int st = 0; st < KSTAGE_NB; st++)
-        sz->tiles_cap += stage_tiles_bound(st, params, (int)params->debayer, fast);
+        sz->tiles_cap += stage_tiles_bound(st, params, (int)params->debayer,
+                                           fast ? KDN_GEOM_FAST : KDN_GEOM_DIRECT);
+    /* exact batches may also hold frames with the pairwise-weight tiles */
+    if (!fast)
+        sz->tiles_cap += stage_tiles_bound(KSTAGE_DENOISE, params, (int)params->debayer,
+                                           KDN_GEOM_SYM);
     sz->tiles_cap += NB_LISTS * 64;
     return 0;
 }
@@ -322,23 +327,24 @@ static int batch_plan(CRAWBatch *b, const CRAWBatchFrame *frames, uint32_t nb, B
             for (int m = 0; m < 5; m++)
                 if (dn_modes[m] == pl.dn_mode)
                     li = L_DN0 + m;
-            n = craw_stage_tiles(KSTAGE_DENOISE, W, H, pl.f.method, fast, &ntx);
+            n = craw_stage_tiles(KSTAGE_DENOISE, W, H, pl.f.method,
+                                 craw_dn_geom(b->form, pl.dn_mode), &ntx);
             for (int t = 0; t < n; t++)
                 bp.lists[li].push_back(KTILE(i, t));
         }
-        n = craw_stage_tiles(KSTAGE_DEBAYER, W, H, pl.f.method, fast, &ntx);
+        n = craw_stage_tiles(KSTAGE_DEBAYER, W, H, pl.f.method, 0, &ntx);
         for (int t = 0; t < n; t++)
             bp.lists[pl.chroma ? L_DB_INTER : L_DB_OUT].push_back(KTILE(i, t));
         if (pl.chroma) {
-            n = craw_stage_tiles(KSTAGE_CHROMA_BLUR, W, H, pl.f.method, fast, &ntx);
+            n = craw_stage_tiles(KSTAGE_CHROMA_BLUR, W, H, pl.f.method, 0, &ntx);
             for (int t = 0; t < n; t++)
                 bp.lists[L_CB].push_back(KTILE(i, t));
-            n = craw_stage_tiles(KSTAGE_CHROMA_UP, W, H, pl.f.method, fast, &ntx);
+            n = craw_stage_tiles(KSTAGE_CHROMA_UP, W, H, pl.f.method, 0, &ntx);
             for (int t = 0; t < n; t++)
                 bp.lists[L_CU].push_back(KTILE(i, t));
         }
         if (b->cfg.awb) {
-            n = craw_stage_tiles(KSTAGE_AWB, W, H, pl.f.method, fast, &ntx);
+            n = craw_stage_tiles(KSTAGE_AWB, W, H, pl.f.method, 0, &ntx);
             for (int t = 0; t < n; t++)
                 bp.lists[L_AWB].push_back(KTILE(i, t));
         }

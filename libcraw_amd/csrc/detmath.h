@@ -45,6 +45,14 @@
 #define CRAW_DM_FN static inline
 #endif
 
+/* lo <= x <= hi for a number; any value in that range for a NaN (which
+ * callers replace). The device form is one max and one min. */
+#if defined(__HIP_DEVICE_COMPILE__)
+#define CRAW_DM_CLAMP(x, lo, hi) __builtin_fmin(__builtin_fmax((x), (lo)), (hi))
+#else
+#define CRAW_DM_CLAMP(x, lo, hi) ((x) < (lo) ? (lo) : ((x) > (hi) ? (hi) : (x)))
+#endif
+
 CRAW_DM_FN uint64_t craw_dm_bits(double x)
 {
     uint64_t u;
@@ -68,7 +76,7 @@ CRAW_DM_FN double craw_dm_exp_d(double x)
     double k = __builtin_rint(x * inv_ln2);
     double r = __builtin_fma(-k, ln2_hi, x);
     double p;
-    int64_t ki;
+    int32_t ki;
 
     r = __builtin_fma(-k, ln2_lo, r);
 
@@ -87,9 +95,15 @@ CRAW_DM_FN double craw_dm_exp_d(double x)
     p = __builtin_fma(p, r, 1.0);
     p = __builtin_fma(p, r, 1.0);
 
-    /* p in [0.70, 1.42]; 2^k is a normal double for |k| <= 1010. */
-    ki = (int64_t)k;
-    return p * craw_dm_from_bits((uint64_t)(ki + 1023) << 52);
+    /* p in [0.70, 1.42]; 2^k is a normal double for |k| <= 1010, so the
+     * scaling is exact (one ldexp on the device, a multiply by the power
+     * built in the exponent field on the host: the same value). */
+    ki = (int32_t)k;
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_ldexp(p, ki);
+#else
+    return p * craw_dm_from_bits((uint64_t)(uint32_t)(ki + 1023) << 52);
+#endif
 }
 
 /* log(x) in double, for finite x > 0 (double normal). */
@@ -135,16 +149,12 @@ CRAW_DM_FN double craw_dm_log_d(double x)
  * rounding of the double result. */
 CRAW_DM_FN float craw_dm_expf(float x)
 {
-    double xd;
+    float r;
 
-    if (x != x)
-        return x + x;
-
-    xd = (double)x;
-    xd = xd < -200.0 ? -200.0 : xd;
-    xd = xd > 200.0 ? 200.0 : xd;
-
-    return (float)craw_dm_exp_d(xd);
+    /* branch-free: a NaN is clamped like any argument, and its result
+     * replaced at the end */
+    r = (float)craw_dm_exp_d(CRAW_DM_CLAMP((double)x, -200.0, 200.0));
+    return x != x ? x + x : r;
 }
 
 /* logf: NaN for NaN and negatives, -inf at zero, +inf at +inf. Float

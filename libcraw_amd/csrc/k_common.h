@@ -82,31 +82,39 @@ struct KColor {
     uint32_t flags;
 };
 
-/* lib/gamut.c:66-92: ACES 1.3 RGC of one AP1 triple. */
+/* lib/gamut.c:66-92: ACES 1.3 RGC of one AP1 triple. Each channel's
+ * distance comes from the input triple; the compression of a channel past
+ * its threshold is one loop body (not unrolled), so its two powf
+ * evaluations appear once in the code. */
 KDEV void kgamut(const KColor &c, float &r, float &g, float &b)
 {
-    float ach = r > (g > b ? g : b) ? r : (g > b ? g : b);
-    float v[3] = { r, g, b };
-    float mag;
+    const float ach = r > (g > b ? g : b) ? r : (g > b ? g : b);
+    float mag, d0, d1, d2;
 
     if (ach == 0.0f)
         return;
     mag = fabsf(ach);
-#pragma unroll
+    d0 = (ach - r) / mag;
+    d1 = (ach - g) / mag;
+    d2 = (ach - b) / mag;
+#pragma unroll 1
     for (int i = 0; i < 3; i++) {
-        float d = (ach - v[i]) / mag;
-        float nd, p;
+        const float d = i == 0 ? d0 : (i == 1 ? d1 : d2);
+        const float thr = c.thr[i], scl = c.scl[i];
+        float nd, p, v;
 
-        if (d < c.thr[i])
+        if (d < thr)
             continue;
-        nd = (d - c.thr[i]) / c.scl[i];
+        nd = (d - thr) / scl;
         p = craw_dm_powf(nd, c.pwr);
-        d = c.thr[i] + c.scl[i] * nd / craw_dm_powf(1.0f + p, c.inv_pwr);
-        v[i] = ach - d * mag;
+        v = ach - (thr + scl * nd / craw_dm_powf(1.0f + p, c.inv_pwr)) * mag;
+        if (i == 0)
+            r = v;
+        else if (i == 1)
+            g = v;
+        else
+            b = v;
     }
-    r = v[0];
-    g = v[1];
-    b = v[2];
 }
 
 KDEV void kcolor(const KColor &c, float cr, float cg, float cb, float o[3])

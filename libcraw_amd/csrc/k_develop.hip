@@ -65,7 +65,7 @@ KDEV void kfinish(const float (*rgbs)[TH][TW], int x0, int y0, int W, int H,
         }
     }
 
-#pragma unroll
+#pragma unroll 1
     for (int k = 0; k < TW * TH / NT; k++) {
         int p = tid + k * NT, px = p % TW, py = p / TW;
         int x = x0 + px, y = y0 + py;
@@ -143,7 +143,7 @@ struct SmSimple {
     float rgbs[3][BL_TH][BL_TW];
 };
 
-template <int METHOD, bool INTER, bool INTERIOR>
+template <int METHOD, bool INTERIOR>
 KDEV void dev_debayer_simple(SmSimple &sm, const KFrame &f, int x0, int y0)
 {
     constexpr int RW = BL_TW + 2 * BL_A, RH = BL_TH + 2 * BL_A;
@@ -218,21 +218,20 @@ KDEV void dev_debayer_simple(SmSimple &sm, const KFrame &f, int x0, int y0)
         rgbs[2][py][px] = b;
     }
 #undef V
-    __syncthreads();
-
-    if (INTER)
-        kstore_inter<BL_TW, BL_TH, BL_NT>(rgbs, x0, y0, W, H, f.inter);
-    else
-        kfinish<BL_TW, BL_TH, BL_NT>(rgbs, x0, y0, W, H, f.col, f.out);
 }
 
 template <int METHOD, bool INTER>
 KDEV void dev_debayer_simple_tile(SmSimple &sm, const KFrame &f, int x0, int y0)
 {
     if (x0 >= BL_A && y0 >= BL_A && x0 + BL_TW + BL_A <= f.W && y0 + BL_TH + BL_A <= f.H)
-        dev_debayer_simple<METHOD, INTER, true>(sm, f, x0, y0);
+        dev_debayer_simple<METHOD, true>(sm, f, x0, y0);
     else
-        dev_debayer_simple<METHOD, INTER, false>(sm, f, x0, y0);
+        dev_debayer_simple<METHOD, false>(sm, f, x0, y0);
+    __syncthreads();
+    if (INTER)
+        kstore_inter<BL_TW, BL_TH, BL_NT>(sm.rgbs, x0, y0, f.W, f.H, f.inter);
+    else
+        kfinish<BL_TW, BL_TH, BL_NT>(sm.rgbs, x0, y0, f.W, f.H, f.col, f.out);
 }
 
 /* --------------------------------------------------------------- RCD */
@@ -292,7 +291,7 @@ struct SmRcd {
         }
 #define RC_STAGE_END }
 
-template <bool INTER, bool INTERIOR>
+template <bool INTERIOR>
 KDEV void dev_debayer_rcd(SmRcd &sm, const KFrame &f, int x0, int y0)
 {
     typedef RcStage<5> S5;
@@ -458,12 +457,6 @@ KDEV void dev_debayer_rcd(SmRcd &sm, const KFrame &f, int x0, int y0)
             rgbs[1][py][px] = g;
             rgbs[2][py][px] = b;
         }
-        __syncthreads();
-
-        if (INTER)
-            kstore_inter<RC_TW, RC_TH, RC_NT>(rgbs, x0, y0, W, H, f.inter);
-        else
-            kfinish<RC_TW, RC_TH, RC_NT>(rgbs, x0, y0, W, H, f.col, f.out);
     }
 #undef AT
 #undef V
@@ -480,13 +473,22 @@ KDEV bool rcd_interior(const KFrame &f, int x0, int y0)
     return x0 >= 5 && y0 >= 5 && x0 + RC_TW + 5 <= f.W && y0 + RC_TH + 5 <= f.H;
 }
 
+/* One RCD tile: the stages (with or without the mirror) leave the tile's
+ * camera RGB in LDS, then one copy of the tail stores it. */
 template <bool INTER>
 KDEV void dev_debayer_rcd_tile(SmRcd &sm, const KFrame &f, int x0, int y0)
 {
+    const float (*rgbs)[RC_TH][RC_TW] = (const float (*)[RC_TH][RC_TW])&sm.sg[0][0];
+
     if (rcd_interior(f, x0, y0))
-        dev_debayer_rcd<INTER, true>(sm, f, x0, y0);
+        dev_debayer_rcd<true>(sm, f, x0, y0);
     else
-        dev_debayer_rcd<INTER, false>(sm, f, x0, y0);
+        dev_debayer_rcd<false>(sm, f, x0, y0);
+    __syncthreads();
+    if (INTER)
+        kstore_inter<RC_TW, RC_TH, RC_NT>(rgbs, x0, y0, f.W, f.H, f.inter);
+    else
+        kfinish<RC_TW, RC_TH, RC_NT>(rgbs, x0, y0, f.W, f.H, f.col, f.out);
 }
 
 /* ------------------------------------------------------------ chroma */
@@ -953,7 +955,7 @@ hipError_t craw_launch_batch_chroma_blur(const KFrame *frames, const uint32_t *t
 
 hipError_t craw_launch_awb_hist(const KFrame &f, hipStream_t s)
 {
-    int ntx, n = craw_stage_tiles(KSTAGE_AWB, f.W, f.H, 0, false, &ntx);
+    int ntx, n = craw_stage_tiles(KSTAGE_AWB, f.W, f.H, 0, 0, &ntx);
 
     hipLaunchKernelGGL(k_awb_hist, dim3(n), dim3(AWB_NT), 0, s, f);
     return hipGetLastError();

@@ -91,6 +91,8 @@ struct KFrame {
 #define KG_FT_TW 28
 #define KG_FT_TH 64
 #define KG_AWB_CHUNK 8192
+#define KG_XS_TW 224
+#define KG_XS_TH 64
 
 enum KStage {
     KSTAGE_DENOISE = 0,
@@ -101,18 +103,40 @@ enum KStage {
     KSTAGE_NB,
 };
 
+/* NLM forms: bit-exact FP32; FP32 reordered with the hardware exp2;
+ * packed FP16 distances and weight arguments. */
+#define KFORM_EXACT 0
+#define KFORM_FAST  1
+#define KFORM_FP16  2
+
+/* Which denoise kernel's tiles: the exact kernel of the temporal-history
+ * modes (32x16), the fast forms' (28x64), the exact kernel with pairwise
+ * weights of the modes without history (224x64). */
+enum KDnGeom {
+    KDN_GEOM_DIRECT = 0,
+    KDN_GEOM_FAST,
+    KDN_GEOM_SYM,
+};
+
+static inline int craw_dn_geom(int form, int mode)
+{
+    if (form != KFORM_EXACT)
+        return KDN_GEOM_FAST;
+    return ((mode & KDN_SPATIAL) && !(mode & KDN_HISTORY)) ? KDN_GEOM_SYM : KDN_GEOM_DIRECT;
+}
+
 /* Tile geometry of `stage`: tw x th samples of a grid `div` times coarser
  * than the frame (2: the four planes of the mosaic; 4: the quarter grid of
  * the chroma stage), `planes` grids per frame. `method` picks the
- * debayer's tile, `fast` the fast denoise's. */
-static inline void craw_stage_geom(int stage, int method, bool fast, int *tw, int *th, int *div,
-                                   int *planes)
+ * debayer's tile, `dn_geom` (KDnGeom) the denoise's. */
+static inline void craw_stage_geom(int stage, int method, int dn_geom, int *tw, int *th,
+                                   int *div, int *planes)
 {
     *tw = *th = *div = *planes = 1;
     switch (stage) {
     case KSTAGE_DENOISE:
-        *tw = fast ? KG_FT_TW : KG_DN_TW;
-        *th = fast ? KG_FT_TH : KG_DN_TH;
+        *tw = dn_geom == KDN_GEOM_FAST ? KG_FT_TW : (dn_geom == KDN_GEOM_SYM ? KG_XS_TW : KG_DN_TW);
+        *th = dn_geom == KDN_GEOM_FAST ? KG_FT_TH : (dn_geom == KDN_GEOM_SYM ? KG_XS_TH : KG_DN_TH);
         *div = 2;
         *planes = 4;
         break;
@@ -139,7 +163,7 @@ static inline void craw_stage_geom(int stage, int method, bool fast, int *tw, in
 
 /* Number of tiles of `stage` for a frame of W x H (even W and H; the
  * quarter grid rounds up). */
-static inline int craw_stage_tiles(int stage, int W, int H, int method, bool fast, int *ntx)
+static inline int craw_stage_tiles(int stage, int W, int H, int method, int dn_geom, int *ntx)
 {
     int tw, th, div, planes, w, h;
 
@@ -148,7 +172,7 @@ static inline int craw_stage_tiles(int stage, int W, int H, int method, bool fas
         *ntx = (nblocks + KG_AWB_CHUNK - 1) / KG_AWB_CHUNK;
         return *ntx;
     }
-    craw_stage_geom(stage, method, fast, &tw, &th, &div, &planes);
+    craw_stage_geom(stage, method, dn_geom, &tw, &th, &div, &planes);
     w = (W + div - 1) / div;
     h = (H + div - 1) / div;
     *ntx = (w + tw - 1) / tw;
@@ -172,6 +196,11 @@ hipError_t craw_launch_denoise(int mode, const KFrame &f, hipStream_t s);
  * with fp16 the packed-FP16 distances and weight arguments. */
 hipError_t craw_launch_denoise_fast(int mode, bool fp16, const KFrame &f, hipStream_t s);
 
+/* The exact form with pairwise weights (modes without a history). */
+hipError_t craw_launch_denoise_sym(int mode, const KFrame &f, hipStream_t s);
+hipError_t craw_launch_batch_denoise_sym(int mode, const KFrame *frames, const uint32_t *tiles,
+                                         uint32_t ntiles, int grid, hipStream_t s);
+
 /* Batched form: `frames` is a device table, `tiles` a device work-list of
  * `ntiles` KTILE entries for the stage, `grid` the number of persistent
  * workgroups. All frames share the method, the denoise mode and whether
@@ -188,10 +217,5 @@ hipError_t craw_launch_batch_awb_hist(const KFrame *frames, const uint32_t *tile
                                       uint32_t ntiles, int grid, hipStream_t s);
 hipError_t craw_launch_batch_chroma_up(const KFrame *frames, const uint32_t *tiles,
                                        uint32_t ntiles, int grid, hipStream_t s);
-
-/* denoise forms for the batched launcher */
-#define KFORM_EXACT 0
-#define KFORM_FAST  1
-#define KFORM_FP16  2
 
 #endif /* CRAW_KERNELS_H */

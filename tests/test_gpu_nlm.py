@@ -95,6 +95,20 @@ def test_exact_form_equals_oracle(gpu_available, a, b):
     assert np.array_equal(g[0], o[0])
 
 
+@pytest.mark.parametrize("size", [(912, 272), (1000, 300), (458, 140)])
+def test_exact_form_large_tiles(gpu_available, size):
+    """Planes with whole tiles away from every border (no mirror in the
+    staging), partial tiles at the right and bottom, and sizes that are
+    not multiples of the tile."""
+    a, b = 2e-4, 4e-6
+    w, h = size
+    frames = [scene.make_frame(w, h, seed=21, a=a, b=b)]
+    g = gpu_sequence("exact", frames, 1.0, 0.0, a, b)
+    o = oracle_sequence(oracle.f32d, frames, 1.0, 0.0, a, b)
+    d = g[0] != o[0]
+    assert not d.any(), "%d samples differ, first at %s" % (int(d.sum()), np.argwhere(d)[0])
+
+
 CLIP = 4095.0 - 512.0      # the clip level above black, in codes
 
 
