@@ -73,6 +73,9 @@ def parse():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--plain-only", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=3.0)
+    # for hardware-counter runs: copies of a known size, to scale the
+    # counters' bytes by what they read for a plain copy
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0)
     return ap.parse_args()
 
 
@@ -517,6 +520,16 @@ def main():
     }
     pipe.close()
     print("bench: gpu timing done", file=sys.stderr, flush=True)
+    if args.calibrate_copy_mib:
+        nbytes = args.calibrate_copy_mib << 20
+        a, b = hip.DeviceBuffer(nbytes), hip.DeviceBuffer(nbytes)
+        a.zero()
+        for _ in range(3):
+            b.copy_from(a, nbytes)
+        hip.synchronize()
+        out["calibration_copy_bytes"] = nbytes
+        a.close()
+        b.close()
     if not args.plain_only:
         import libcraw_amd.pipeline as P
         # the plain form with the packed-FP16 NLM, same frame

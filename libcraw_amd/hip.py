@@ -100,6 +100,11 @@ class DeviceBuffer:
               "hipMemcpy D2H")
         return out
 
+    def copy_from(self, other, nbytes):
+        """Device-to-device copy (HIP's own copy kernel)."""
+        assert nbytes <= self.nbytes and nbytes <= other.nbytes
+        check(hip().hipMemcpy(self.ptr, other.ptr, nbytes, D2D), "hipMemcpy D2D")
+
     def zero(self):
         check(hip().hipMemset(self.ptr, 0, self.nbytes), "hipMemset")
 
