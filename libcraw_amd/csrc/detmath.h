@@ -157,6 +157,16 @@ CRAW_DM_FN float craw_dm_expf(float x)
     return x != x ? x + x : r;
 }
 
+/* craw_dm_expf for an argument that is <= 0 and not a NaN (the weights of
+ * the denoise and chroma filters): the same value, without the checks
+ * that cannot fire there. */
+CRAW_DM_FN float craw_dm_expf_nonpos(float x)
+{
+    const double xd = (double)x;
+
+    return (float)craw_dm_exp_d(xd < -200.0 ? -200.0 : xd);
+}
+
 /* logf: NaN for NaN and negatives, -inf at zero, +inf at +inf. Float
  * subnormals are normal doubles, so they need no special path. */
 CRAW_DM_FN float craw_dm_logf(float x)

@@ -556,8 +556,8 @@ KDEV void dev_chroma_blur(SmCb &sm, const KFrame &f, int qx0, int qy0)
             const float dg = (s2 - c2) * inv_range;
             const float dd0 = (s0 - c0) * inv_chroma;
             const float dd1 = (s1 - c1) * inv_chroma;
-            const float wgt = craw_dm_expf(-(float)(dx * dx + dy * dy) * 0.125f - dg * dg
-                                           - dd0 * dd0 - dd1 * dd1);
+            const float wgt = craw_dm_expf_nonpos(-(float)(dx * dx + dy * dy) * 0.125f
+                                                  - dg * dg - dd0 * dd0 - dd1 * dd1);
 
             acc0 += wgt * s0;
             acc1 += wgt * s1;
@@ -575,8 +575,14 @@ KDEV void dev_chroma_blur(SmCb &sm, const KFrame &f, int qx0, int qy0)
 /* lib/ref/ref_chroma.c:119-155: the difference of the filtered and the
  * unfiltered band, bilinearly up to every pixel and blended by the
  * strength, into R and B; then the histogram and the colour stage. */
+/* The quarter-grid cells a 64x16 tile's bilinear taps can reach: one
+ * before and one after, clamped to the grid as the reference clamps. */
+#define CU_QW (CU_TW / 4 + 2)
+#define CU_QH (CU_TH / 4 + 2)
+
 struct SmUp {
     float rgbs[3][CU_TH][CU_TW];
+    float q[4][CU_QH][CU_QW];     /* down R-G, down B-G, blur R-G, blur B-G */
 };
 
 KDEV void dev_chroma_up_finish(SmUp &sm, const KFrame &f, int x0, int y0)
@@ -587,14 +593,27 @@ KDEV void dev_chroma_up_finish(SmUp &sm, const KFrame &f, int x0, int y0)
     const int W = f.W, H = f.H;
     const int tid = threadIdx.x;
     const size_t plane = (size_t)W * H, qplane = (size_t)it.wq * it.hq;
+    const int qbx = x0 / 4 - 1, qby = y0 / 4 - 1;
+
+    /* cell (j, i) of the stage holds grid cell clamp(qbx + i, qby + j) */
+    for (int i = tid; i < 4 * CU_QH * CU_QW; i += CU_NT) {
+        const int c = i / (CU_QH * CU_QW), r = i % (CU_QH * CU_QW);
+        const int lj = r / CU_QW, li = r % CU_QW;
+        int gx = qbx + li, gy = qby + lj;
+        const float *srcp = (c < 2 ? it.down : it.blur) + (c & 1) * qplane;
+
+        gx = gx > 0 ? (gx < it.wq - 1 ? gx : it.wq - 1) : 0;
+        gy = gy > 0 ? (gy < it.hq - 1 ? gy : it.hq - 1) : 0;
+        sm.q[c][lj][li] = srcp[(size_t)gy * it.wq + gx];
+    }
+    __syncthreads();
 
 #pragma unroll
     for (int k = 0; k < CU_TW * CU_TH / CU_NT; k++) {
         int p = tid + k * CU_NT, px = p % CU_TW, py = p / CU_TW;
         int x = x0 + px, y = y0 + py;
         float fy, fly, ty, fx, flx, tx, w00, w10, w01, w11, r, g, b, d[2];
-        int qy0, qy1, qx0, qx1;
-        size_t i00, i10, i01, i11;
+        int ly0, ly1, lx0, lx1;
 
         if (x >= W || y >= H)
             continue;
@@ -602,36 +621,32 @@ KDEV void dev_chroma_up_finish(SmUp &sm, const KFrame &f, int x0, int y0)
         g = it.rgb[plane + (size_t)y * W + x];
         b = it.rgb[2 * plane + (size_t)y * W + x];
 
+        /* lib/ref/ref_chroma.c:128-140: the taps' weights and (clamped)
+         * cells; the clamp is in the staged cells */
         fy = ((float)y + 0.5f) * 0.25f - 0.5f;
         fly = floorf(fy);
         ty = fy - fly;
-        qy0 = (int)fly;
-        qy0 = qy0 > 0 ? (qy0 < it.hq - 1 ? qy0 : it.hq - 1) : 0;
-        qy1 = (int)fly + 1;
-        qy1 = qy1 > 0 ? (qy1 < it.hq - 1 ? qy1 : it.hq - 1) : 0;
+        ly0 = (int)fly - qby;
+        ly1 = ly0 + 1;
         fx = ((float)x + 0.5f) * 0.25f - 0.5f;
         flx = floorf(fx);
         tx = fx - flx;
-        qx0 = (int)flx;
-        qx0 = qx0 > 0 ? (qx0 < it.wq - 1 ? qx0 : it.wq - 1) : 0;
-        qx1 = (int)flx + 1;
-        qx1 = qx1 > 0 ? (qx1 < it.wq - 1 ? qx1 : it.wq - 1) : 0;
+        lx0 = (int)flx - qbx;
+        lx1 = lx0 + 1;
         w00 = (1.0f - tx) * (1.0f - ty);
         w10 = tx * (1.0f - ty);
         w01 = (1.0f - tx) * ty;
         w11 = tx * ty;
-        i00 = (size_t)qy0 * it.wq + qx0;
-        i10 = (size_t)qy0 * it.wq + qx1;
-        i01 = (size_t)qy1 * it.wq + qx0;
-        i11 = (size_t)qy1 * it.wq + qx1;
 
         d[0] = r - g;
         d[1] = b - g;
 #pragma unroll
         for (int c = 0; c < 2; c++) {
-            const float *dn = it.down + c * qplane, *bl = it.blur + c * qplane;
-            float lo = w00 * dn[i00] + w10 * dn[i10] + w01 * dn[i01] + w11 * dn[i11];
-            float lf = w00 * bl[i00] + w10 * bl[i10] + w01 * bl[i01] + w11 * bl[i11];
+            const float (&dn)[CU_QH][CU_QW] = sm.q[c], (&bl)[CU_QH][CU_QW] = sm.q[2 + c];
+            float lo = w00 * dn[ly0][lx0] + w10 * dn[ly0][lx1] + w01 * dn[ly1][lx0] +
+                       w11 * dn[ly1][lx1];
+            float lf = w00 * bl[ly0][lx0] + w10 * bl[ly0][lx1] + w01 * bl[ly1][lx0] +
+                       w11 * bl[ly1][lx1];
             d[c] += strength * (lf - lo);
         }
         rgbs[0][py][px] = g + d[0];

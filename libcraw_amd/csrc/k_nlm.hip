@@ -132,7 +132,7 @@ KDEV void dev_denoise(SmDn<MODE> &sm, const KFrame &f, int x0, int y0, int pl)
                 }
                 {
                     float arg = d2 * (1.0f / 9.0f) * p.inv_h2 - 2.0f;
-                    a = p.alpha_max * craw_dm_expf(-(arg > 0.0f ? arg : 0.0f));
+                    a = p.alpha_max * craw_dm_expf_nonpos(-(arg > 0.0f ? arg : 0.0f));
                 }
                 t = scur[cy * CW + cx] * (1.0f - a) + stp[cy * CW + cx] * a;
                 pv = kvar_decode(st.prev_var[pl * psize + (size_t)my * w2 + mx]);
@@ -211,10 +211,10 @@ KDEV void dev_denoise(SmDn<MODE> &sm, const KFrame &f, int x0, int y0, int pl)
                     if (HISTORY) {
                         float v2 = vc + svar[n];
                         arg = d2 * (1.0f / 9.0f) * p.inv_h2 - v2;
-                        wgt = craw_dm_expf(-(arg > 0.0f ? arg : 0.0f) / vc);
+                        wgt = craw_dm_expf_nonpos(-(arg > 0.0f ? arg : 0.0f) / vc);
                     } else {
                         arg = d2 * (1.0f / 9.0f) * p.inv_h2 - 2.0f;
-                        wgt = craw_dm_expf(-(arg > 0.0f ? arg : 0.0f));
+                        wgt = craw_dm_expf_nonpos(-(arg > 0.0f ? arg : 0.0f));
                     }
                     wsum += wgt;
                     acc += wgt * splane[n];

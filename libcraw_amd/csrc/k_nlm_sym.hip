@@ -13,7 +13,7 @@
  * W_o(p - o), offset o is W_o(p). The sums are then the reference's, with
  * half the distances and half the exps.
  *
- * Layout: a workgroup's four waves stand side by side over a band of the
+ * Layout: a workgroup's four waves stand side by side over a 224x56 band of the
  * plane, each lane one column, and walk down it a row at a time. A lane
  * keeps, per offset, the squared differences of three rows and the weights
  * of the last four rows in registers; the patch sums across columns come
@@ -32,11 +32,11 @@
 #define XS_NT    (64 * XS_NW)
 #define XS_OUTW  56                    /* output columns per wave */
 #define XS_TW    (XS_OUTW * XS_NW)     /* 224 */
-#define XS_TH    64
+#define XS_TH    56
 #define XS_AX    7
 #define XS_AY    4
 #define XS_LW    (XS_TW + 2 * XS_AX)   /* 238 */
-#define XS_LH    (XS_TH + 2 * XS_AY)   /* 72 */
+#define XS_LH    (XS_TH + 2 * XS_AY)   /* 64 */
 #define XS_NO    24                    /* offsets after the centre */
 
 #define DPP_WAVE_SHR1 0x138
@@ -90,7 +90,7 @@ struct SmXs {
 KDEV float xs_weight(float d2, float inv_h2)
 {
     float arg = d2 * (1.0f / 9.0f) * inv_h2 - 2.0f;
-    return craw_dm_expf(-(arg > 0.0f ? arg : 0.0f));
+    return craw_dm_expf_nonpos(-(arg > 0.0f ? arg : 0.0f));
 }
 
 template <int MODE>

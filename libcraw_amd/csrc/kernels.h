@@ -92,7 +92,7 @@ struct KFrame {
 #define KG_FT_TH 64
 #define KG_AWB_CHUNK 8192
 #define KG_XS_TW 224
-#define KG_XS_TH 64
+#define KG_XS_TH 56
 
 enum KStage {
     KSTAGE_DENOISE = 0,
@@ -111,7 +111,7 @@ enum KStage {
 
 /* Which denoise kernel's tiles: the exact kernel of the temporal-history
  * modes (32x16), the fast forms' (28x64), the exact kernel with pairwise
- * weights of the modes without history (224x64). */
+ * weights of the modes without history (224x56). */
 enum KDnGeom {
     KDN_GEOM_DIRECT = 0,
     KDN_GEOM_FAST,
