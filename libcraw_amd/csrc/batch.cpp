@@ -18,13 +18,14 @@
 #include "internal.h"
 
 #define RING 4
-#define NB_LISTS 10     /* 5 denoise modes, 2 debayer targets, blur, up, awb */
+#define NB_LISTS 11     /* 5 denoise modes, 2 debayer targets, blur, up, awb, RCD stream */
 #define L_DN0 0
 #define L_DB_OUT 5
 #define L_DB_INTER 6
 #define L_CB 7
 #define L_CU 8
 #define L_AWB 9
+#define L_RS 10
 
 static const int dn_modes[5] = {
     KDN_SPATIAL, KDN_TEMPORAL, KDN_TEMPORAL | KDN_HISTORY, KDN_SPATIAL | KDN_TEMPORAL,
@@ -79,6 +80,15 @@ static size_t stage_tiles_bound(int stage, const CRAWBatchParams *p, int method,
 {
     int tw, th, div, planes;
     double F = p->max_frames, wm, hm, area, n;
+
+    if (stage == KSTAGE_RCD_RING)       /* at most every tile of the frame */
+        return stage_tiles_bound(KSTAGE_DEBAYER, p, 2, dn_geom);
+    if (stage == KSTAGE_RCD_STREAM) {   /* strips of the interior, in groups */
+        n = (double)p->max_pixels / ((double)KG_RS_OUT * KG_RS_R * KG_RS_NW) +
+            F * (((double)p->max_width / KG_RS_OUT + (double)p->max_height / KG_RS_R + 1.0) /
+                 KG_RS_NW + 1.0);
+        return (size_t)n + 64;
+    }
 
     craw_stage_geom(stage, method, dn_geom, &tw, &th, &div, &planes);
     wm = (double)p->max_width / div + 1.0;
@@ -332,9 +342,19 @@ static int batch_plan(CRAWBatch *b, const CRAWBatchFrame *frames, uint32_t nb, B
             for (int t = 0; t < n; t++)
                 bp.lists[li].push_back(KTILE(i, t));
         }
-        n = craw_stage_tiles(KSTAGE_DEBAYER, W, H, pl.f.method, 0, &ntx);
-        for (int t = 0; t < n; t++)
-            bp.lists[pl.chroma ? L_DB_INTER : L_DB_OUT].push_back(KTILE(i, t));
+        if (pl.chroma && pl.f.method == 2) {
+            /* RCD into the chroma stage: ring tiles and interior strips */
+            n = craw_stage_tiles(KSTAGE_RCD_RING, W, H, 2, 0, &ntx);
+            for (int t = 0; t < n; t++)
+                bp.lists[L_DB_INTER].push_back(KTILE(i, t));
+            n = craw_stage_tiles(KSTAGE_RCD_STREAM, W, H, 2, 0, &ntx);
+            for (int t = 0; t < n; t++)
+                bp.lists[L_RS].push_back(KTILE(i, t));
+        } else {
+            n = craw_stage_tiles(KSTAGE_DEBAYER, W, H, pl.f.method, 0, &ntx);
+            for (int t = 0; t < n; t++)
+                bp.lists[pl.chroma ? L_DB_INTER : L_DB_OUT].push_back(KTILE(i, t));
+        }
         if (pl.chroma) {
             n = craw_stage_tiles(KSTAGE_CHROMA_BLUR, W, H, pl.f.method, 0, &ntx);
             for (int t = 0; t < n; t++)
@@ -411,6 +431,13 @@ static hipError_t batch_launch(CRAWBatch *b, const BatchPlan &bp, const size_t *
         PB(names[b->cfg.debayer]);
         e = craw_launch_batch_debayer((int)b->cfg.debayer, l == L_DB_INTER, b->frames,
                                       b->tiles + list_off[l], (uint32_t)n, grid_for(b, 1, n), s);
+        PE();
+    }
+    if (e == hipSuccess && bp.lists[L_RS].size()) {
+        size_t n = bp.lists[L_RS].size();
+        PB("rcd");
+        e = craw_launch_batch_rcd_stream(b->frames, b->tiles + list_off[L_RS], (uint32_t)n,
+                                         grid_for(b, 1, n), s);
         PE();
     }
     if (e == hipSuccess && bp.lists[L_CB].size()) {

@@ -786,7 +786,16 @@ template <bool INTER>
 __global__ __launch_bounds__(RC_NT) void k_debayer_rcd(KFrame f)
 {
     __shared__ SmRcd sm;
-    dev_debayer_rcd_tile<INTER>(sm, f, blockIdx.x * RC_TW, blockIdx.y * RC_TH);
+
+    if (INTER) {
+        /* into the chroma stage: the ring around the streamed interior,
+         * blockIdx.x a ring tile */
+        int tx, ty;
+        craw_rcd_ring_tile(craw_rcd_split(f.W, f.H), blockIdx.x, &tx, &ty);
+        dev_debayer_rcd_tile<INTER>(sm, f, tx * RC_TW, ty * RC_TH);
+    } else {
+        dev_debayer_rcd_tile<INTER>(sm, f, blockIdx.x * RC_TW, blockIdx.y * RC_TH);
+    }
 }
 
 __global__ __launch_bounds__(CB_TX * CB_TY) void k_chroma_blur(KFrame f)
@@ -833,9 +842,13 @@ __global__ __launch_bounds__(RC_NT) void kb_debayer_rcd(const KFrame *__restrict
         const uint32_t d = tiles[t];
         const KFrame &f = frames[KTILE_FRAME(d)];
         const int ntx = (f.W + RC_TW - 1) / RC_TW, i = KTILE_IDX(d);
+        int tx = i % ntx, ty = i / ntx;
 
+        /* into the chroma stage, entries are ring tiles */
+        if (INTER)
+            craw_rcd_ring_tile(craw_rcd_split(f.W, f.H), i, &tx, &ty);
         __syncthreads();
-        dev_debayer_rcd_tile<INTER>(sm, f, (i % ntx) * RC_TW, (i / ntx) * RC_TH);
+        dev_debayer_rcd_tile<INTER>(sm, f, tx * RC_TW, ty * RC_TH);
     }
 }
 
@@ -895,10 +908,16 @@ hipError_t craw_launch_debayer(bool to_inter, const KFrame &f, hipStream_t s)
 {
     if (f.method == 2) {
         dim3 grid((f.W + RC_TW - 1) / RC_TW, (f.H + RC_TH - 1) / RC_TH);
-        if (to_inter)
-            hipLaunchKernelGGL(k_debayer_rcd<true>, grid, dim3(RC_NT), 0, s, f);
-        else
-            hipLaunchKernelGGL(k_debayer_rcd<false>, grid, dim3(RC_NT), 0, s, f);
+        if (to_inter) {
+            int ntx, nring = craw_stage_tiles(KSTAGE_RCD_RING, f.W, f.H, 2, 0, &ntx);
+            hipError_t e;
+
+            hipLaunchKernelGGL(k_debayer_rcd<true>, dim3(nring), dim3(RC_NT), 0, s, f);
+            if ((e = hipGetLastError()) != hipSuccess)
+                return e;
+            return craw_launch_rcd_stream(f, s);
+        }
+        hipLaunchKernelGGL(k_debayer_rcd<false>, grid, dim3(RC_NT), 0, s, f);
     } else {
         dim3 grid((f.W + BL_TW - 1) / BL_TW, (f.H + BL_TH - 1) / BL_TH);
         if (f.method == 0 && to_inter)

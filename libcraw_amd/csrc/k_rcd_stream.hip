@@ -1,0 +1,368 @@
+/*
+ * libcraw for MI355X: RCD over the frame's interior, streamed down columns.
+ *
+ * Same values as k_develop.hip's tiled RCD (lib/ref/ref_debayer.c:192-395
+ * of cyanreg/libcraw), computed without LDS: a wave owns a strip of the
+ * frame's interior, each lane a pair of adjacent columns (one green and
+ * one red or blue site in every row), and walks down the strip a row at a
+ * time. Every stage keeps the rows its consumers reach in registers, so
+ * each value is computed once; neighbouring columns come from the
+ * neighbouring lanes through wave-wide DPP shifts (row_shr/shl would stop
+ * at 16 lanes). Because a pair always holds one site of each kind, the
+ * site-dependent stages (green, colour estimate, output) are uniform
+ * branches on the row's parity.
+ *
+ * Per step the stages run at fixed lags behind the newest row of V:
+ * low pass 1, gradients 2, direction weights and green 3, colour estimate
+ * 4, output 5. A strip's V reach is 5 columns and 5 rows past its outputs,
+ * all inside the frame (the interior is chosen so), so nothing here needs
+ * the mirror; the tiled kernel develops the ring of tiles around it.
+ *
+ * Output: the chroma stage's inputs (the three camera RGB planes and the
+ * 4x4 block means of R - G, B - G, G, summed in raster order).
+ */
+#include "k_common.h"
+#include "kernels.h"
+
+#define RS_NW 4
+#define RS_NT (64 * RS_NW)
+#define RS_EPS 1e-5f
+
+static_assert(KG_RS_OUT == 116 && KG_RS_R % 4 == 0, "strip geometry");
+
+#define DPP_WAVE_SHR1 0x138
+#define DPP_WAVE_SHL1 0x130
+
+/* The value of the lane to the left / right (0 past the wave's ends). */
+KDEV float rs_shr(float v)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v),
+                                                                 DPP_WAVE_SHR1, 0xf, 0xf, true));
+}
+
+KDEV float rs_shl(float v)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v),
+                                                                 DPP_WAVE_SHL1, 0xf, 0xf, true));
+}
+
+/* lib/ref/ref_debayer.c:195-202 */
+KDEV float rs_ratio(float a, float b)
+{
+    a = a > 0.0f ? a : 0.0f;
+    b = b > 0.0f ? b : 0.0f;
+    return (a - b) / (a + b + RS_EPS);
+}
+
+/* A row of a stage: the lane's two columns. */
+struct RsRow {
+    float c[2];
+};
+
+/* Column C's left / right neighbour of row r, for the lane's column C. */
+template <int C> KDEV float rs_l1(const RsRow &r) { return C == 0 ? rs_shr(r.c[1]) : r.c[0]; }
+template <int C> KDEV float rs_r1(const RsRow &r) { return C == 0 ? r.c[1] : rs_shl(r.c[0]); }
+template <int C> KDEV float rs_l2(const RsRow &r) { return rs_shr(r.c[C]); }
+template <int C> KDEV float rs_r2(const RsRow &r) { return rs_shl(r.c[C]); }
+
+struct RsState {
+    RsRow v[7];                 /* V rows vt - 6 .. vt */
+    RsRow lp[5];                /* LP rows vt - 5 .. vt - 1 */
+    RsRow g0[3], g1[3], g2[3], g3[3];   /* gradient rows vt - 4 .. vt - 2 */
+    RsRow wv[3];                /* rows vt - 5 .. vt - 3 */
+    RsRow wp[2];                /* rows vt - 4 .. vt - 3 */
+    RsRow G[4];                 /* rows vt - 6 .. vt - 3 */
+    RsRow C[3];                 /* rows vt - 6 .. vt - 4 */
+    float acc[3];               /* 4x4 block sums */
+};
+
+template <int N>
+KDEV void rs_shift(RsRow (&a)[N])
+{
+#pragma unroll
+    for (int k = 0; k + 1 < N; k++)
+        a[k] = a[k + 1];
+}
+
+/* Gradients of column C at the centre row v[4] (rows v[2] .. v[6]). */
+template <int C>
+KDEV void rs_gradients(RsState &s)
+{
+    const RsRow *v = s.v;
+    const float c2 = 2.0f * v[4].c[C];
+
+    s.g0[2].c[C] = fabsf(v[3].c[C] - v[5].c[C]) + fabsf(c2 - v[2].c[C] - v[6].c[C]);
+    s.g1[2].c[C] = fabsf(rs_l1<C>(v[4]) - rs_r1<C>(v[4])) +
+                   fabsf(c2 - rs_l2<C>(v[4]) - rs_r2<C>(v[4]));
+    s.g2[2].c[C] = fabsf(rs_l1<C>(v[5]) - rs_r1<C>(v[3])) +
+                   fabsf(c2 - rs_l2<C>(v[6]) - rs_r2<C>(v[2]));
+    s.g3[2].c[C] = fabsf(rs_l1<C>(v[3]) - rs_r1<C>(v[5])) +
+                   fabsf(c2 - rs_l2<C>(v[2]) - rs_r2<C>(v[6]));
+}
+
+/* 3x3 sum of gradient rows g[0..2] around column C, in raster order. */
+template <int C>
+KDEV float rs_box(const RsRow (&g)[3])
+{
+    float a = 0.0f;
+
+#pragma unroll
+    for (int dy = 0; dy < 3; dy++) {
+        a += rs_l1<C>(g[dy]);
+        a += g[dy].c[C];
+        a += rs_r1<C>(g[dy]);
+    }
+    return a;
+}
+
+template <int C>
+KDEV void rs_weights(RsState &s)
+{
+    const float a0 = rs_box<C>(s.g0), a1 = rs_box<C>(s.g1);
+    const float b0 = rs_box<C>(s.g2), b1 = rs_box<C>(s.g3);
+    const float vsum = a0 * (1.0f / 9.0f), hsum = a1 * (1.0f / 9.0f);
+    const float ps = b0 * (1.0f / 9.0f), qs = b1 * (1.0f / 9.0f);
+
+    s.wv[2].c[C] = hsum / (hsum + vsum + RS_EPS);
+    s.wp[1].c[C] = qs / (ps + qs + RS_EPS);
+}
+
+/* Green of the red/blue column N of the centre row v[3]. */
+template <int N>
+KDEV float rs_green(const RsState &s)
+{
+    const float l00 = s.lp[2].c[N], lu = s.lp[0].c[N], ld = s.lp[4].c[N];
+    const float ll = rs_l2<N>(s.lp[2]), lr = rs_r2<N>(s.lp[2]);
+    const float gv = 0.5f * (s.v[2].c[N] * (1.0f + rs_ratio(l00, lu)) +
+                             s.v[4].c[N] * (1.0f + rs_ratio(l00, ld)));
+    const float gh = 0.5f * (rs_l1<N>(s.v[3]) * (1.0f + rs_ratio(l00, ll)) +
+                             rs_r1<N>(s.v[3]) * (1.0f + rs_ratio(l00, lr)));
+    const float w = s.wv[2].c[N];
+
+    return w * gv + (1.0f - w) * gh;
+}
+
+/* Colour estimate of the red/blue column N of the centre row v[2]. */
+template <int N>
+KDEV float rs_colour(const RsState &s)
+{
+    const float g00 = s.G[2].c[N];
+    const float dsw = rs_l1<N>(s.v[3]) - rs_l1<N>(s.G[3]);
+    const float dne = rs_r1<N>(s.v[1]) - rs_r1<N>(s.G[1]);
+    const float dnw = rs_l1<N>(s.v[1]) - rs_l1<N>(s.G[1]);
+    const float dse = rs_r1<N>(s.v[3]) - rs_r1<N>(s.G[3]);
+    const float w = s.wp[0].c[N];
+
+    return w * (g00 + 0.5f * (dsw + dne)) + (1.0f - w) * (g00 + 0.5f * (dnw + dse));
+}
+
+/* One step: P is the parity of the step (the newest V row is odd for
+ * P = 0, the strip starting on an even row). */
+template <int P>
+KDEV void rs_step(RsState &s, const KFrame &f, int vt, int ld0, bool out_row, bool out_cols,
+                  int X0)
+{
+    /* V row vt */
+    rs_shift(s.v);
+    {
+        const uint16_t *row = f.src + (size_t)vt * f.src_pitch;
+        const int sb = (vt & 1) * 2;
+        s.v[6].c[0] = kprep(row[ld0], sb, f.prep);
+        s.v[6].c[1] = kprep(row[ld0 + 1], sb + 1, f.prep);
+    }
+
+    /* low pass, row vt - 1 */
+    rs_shift(s.lp);
+#pragma unroll
+    for (int c = 0; c < 2; c++) {
+        const RsRow *v = s.v;
+        const float w_ = c == 0 ? rs_shr(v[5].c[1]) : v[5].c[0];
+        const float e_ = c == 0 ? v[5].c[1] : rs_shl(v[5].c[0]);
+        const float nw = c == 0 ? rs_shr(v[4].c[1]) : v[4].c[0];
+        const float ne = c == 0 ? v[4].c[1] : rs_shl(v[4].c[0]);
+        const float sw = c == 0 ? rs_shr(v[6].c[1]) : v[6].c[0];
+        const float se = c == 0 ? v[6].c[1] : rs_shl(v[6].c[0]);
+
+        s.lp[4].c[c] = (4.0f * v[5].c[c] + 2.0f * (v[4].c[c] + v[6].c[c] + w_ + e_) + nw + ne +
+                        sw + se) * (1.0f / 16.0f);
+    }
+
+    /* gradients, row vt - 2 */
+    rs_shift(s.g0);
+    rs_shift(s.g1);
+    rs_shift(s.g2);
+    rs_shift(s.g3);
+    rs_gradients<0>(s);
+    rs_gradients<1>(s);
+
+    /* direction weights, row vt - 3 */
+    rs_shift(s.wv);
+    rs_shift(s.wp);
+    rs_weights<0>(s);
+    rs_weights<1>(s);
+
+    /* green, row vt - 3: the red/blue site is column P there */
+    rs_shift(s.G);
+    s.G[3].c[P] = rs_green<P>(s);
+    s.G[3].c[1 - P] = s.v[3].c[1 - P];
+
+    /* colour estimate, row vt - 4: red/blue column 1 - P */
+    rs_shift(s.C);
+    s.C[2].c[1 - P] = rs_colour<1 - P>(s);
+    s.C[2].c[P] = 0.0f;
+
+    /* output, row vt - 5: green column 1 - P, red/blue column P */
+    if (out_row) {
+        constexpr int CG = 1 - P, CN = P;
+        const KInter &it = f.inter;
+        const size_t plane = (size_t)f.W * f.H;
+        const int oy = vt - 5;
+        float r[2], g[2], b[2];
+
+        {
+            const float c = s.C[1].c[CN], gg = s.G[1].c[CN], vv = s.v[1].c[CN];
+            g[CN] = gg;
+            r[CN] = P == 0 ? vv : c;
+            b[CN] = P == 0 ? c : vv;
+        }
+        {
+            const float g00 = s.G[1].c[CG], w = s.wv[0].c[CG];
+            const float dh = 0.5f * ((rs_l1<CG>(s.v[1]) - rs_l1<CG>(s.G[1])) +
+                                     (rs_r1<CG>(s.v[1]) - rs_r1<CG>(s.G[1])));
+            const float ev = 0.5f * ((s.C[0].c[CG] - s.G[0].c[CG]) + (s.C[2].c[CG] - s.G[2].c[CG]));
+            const float ch = w * (g00 + ev) + (1.0f - w) * (g00 + dh);
+            const float dv = 0.5f * ((s.v[0].c[CG] - s.G[0].c[CG]) + (s.v[2].c[CG] - s.G[2].c[CG]));
+            const float eh = 0.5f * ((rs_l1<CG>(s.C[1]) - rs_l1<CG>(s.G[1])) +
+                                     (rs_r1<CG>(s.C[1]) - rs_r1<CG>(s.G[1])));
+            const float cv = w * (g00 + dv) + (1.0f - w) * (g00 + eh);
+
+            g[CG] = s.v[1].c[CG];
+            r[CG] = P == 0 ? ch : cv;
+            b[CG] = P == 0 ? cv : ch;
+        }
+
+        if (out_cols) {
+            const size_t o = (size_t)oy * f.W + X0;
+            *(float2 *)&it.rgb[o] = make_float2(r[0], r[1]);
+            *(float2 *)&it.rgb[plane + o] = make_float2(g[0], g[1]);
+            *(float2 *)&it.rgb[2 * plane + o] = make_float2(b[0], b[1]);
+        }
+
+        /* 4x4 block sums in raster order: the block's first lane (column
+         * X0 % 4 == 0) adds its two columns and then its right neighbour's
+         * (lib/ref/ref_chroma.c:59-82) */
+        {
+            const float d0[2] = { r[0] - g[0], r[1] - g[1] };
+            const float d1[2] = { b[0] - g[0], b[1] - g[1] };
+            const float n00 = rs_shl(d0[0]), n01 = rs_shl(d0[1]);
+            const float n10 = rs_shl(d1[0]), n11 = rs_shl(d1[1]);
+            const float n20 = rs_shl(g[0]), n21 = rs_shl(g[1]);
+            const bool first_row = (oy & 3) == 0;
+
+            s.acc[0] = (first_row ? 0.0f : s.acc[0]) + d0[0];
+            s.acc[0] = s.acc[0] + d0[1];
+            s.acc[0] = s.acc[0] + n00;
+            s.acc[0] = s.acc[0] + n01;
+            s.acc[1] = (first_row ? 0.0f : s.acc[1]) + d1[0];
+            s.acc[1] = s.acc[1] + d1[1];
+            s.acc[1] = s.acc[1] + n10;
+            s.acc[1] = s.acc[1] + n11;
+            s.acc[2] = (first_row ? 0.0f : s.acc[2]) + g[0];
+            s.acc[2] = s.acc[2] + g[1];
+            s.acc[2] = s.acc[2] + n20;
+            s.acc[2] = s.acc[2] + n21;
+            if ((oy & 3) == 3 && out_cols && (X0 & 3) == 0) {
+                const size_t q = (size_t)(oy / 4) * it.wq + X0 / 4;
+                const size_t qplane = (size_t)it.wq * it.hq;
+
+                it.down[q] = s.acc[0] / 16.0f;
+                it.down[qplane + q] = s.acc[1] / 16.0f;
+                it.down[2 * qplane + q] = s.acc[2] / 16.0f;
+            }
+        }
+    }
+}
+
+/* One strip: output columns [xs, xe), rows [ys, ye) of the interior. */
+KDEV void dev_rcd_stream(const KFrame &f, int strip)
+{
+    const KRcdSplit sp = craw_rcd_split(f.W, f.H);
+    const int nsx = (sp.x_hi - sp.x_lo + KG_RS_OUT - 1) / KG_RS_OUT;
+    const int nsy = (sp.y_hi - sp.y_lo + KG_RS_R - 1) / KG_RS_R;
+    const int lane = threadIdx.x & 63;
+
+    if (strip >= nsx * nsy)
+        return;
+    {
+        const int sx = strip % nsx, sy = strip / nsx;
+        const int xs = sp.x_lo + sx * KG_RS_OUT, ys = sp.y_lo + sy * KG_RS_R;
+        const int xe = min(xs + KG_RS_OUT, sp.x_hi), ye = min(ys + KG_RS_R, sp.y_hi);
+        const int X0 = xs - 6 + 2 * lane;
+        const bool out_cols = X0 >= xs && X0 < xe;
+        /* the pair's load column, kept inside the frame for lanes past the
+         * strip's reach (their values are never used) */
+        const int ld0 = X0 < f.W - 2 ? X0 : f.W - 2;
+        const int nsteps = (ye - ys) + 10;
+        RsState s;
+
+#pragma unroll
+        for (int k = 0; k < 7; k++)
+            s.v[k].c[0] = s.v[k].c[1] = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 5; k++)
+            s.lp[k].c[0] = s.lp[k].c[1] = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            s.g0[k].c[0] = s.g0[k].c[1] = s.g1[k].c[0] = s.g1[k].c[1] = 0.0f;
+            s.g2[k].c[0] = s.g2[k].c[1] = s.g3[k].c[0] = s.g3[k].c[1] = 0.0f;
+            s.wv[k].c[0] = s.wv[k].c[1] = s.C[k].c[0] = s.C[k].c[1] = 0.0f;
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+            s.G[k].c[0] = s.G[k].c[1] = 0.0f;
+        s.wp[0].c[0] = s.wp[0].c[1] = s.wp[1].c[0] = s.wp[1].c[1] = 0.0f;
+        s.acc[0] = s.acc[1] = s.acc[2] = 0.0f;
+
+        /* ys is even and nsteps is even: steps alternate parity 0, 1 */
+#pragma unroll 1
+        for (int t = 0; t < nsteps; t += 2) {
+            rs_step<0>(s, f, ys - 5 + t, ld0, t >= 10, out_cols, X0);
+            rs_step<1>(s, f, ys - 4 + t, ld0, t + 1 >= 10, out_cols, X0);
+        }
+    }
+}
+
+__global__ __launch_bounds__(RS_NT) void k_rcd_stream(KFrame f)
+{
+    dev_rcd_stream(f, blockIdx.x * RS_NW + (threadIdx.x >> 6));
+}
+
+/* Persistent form: work-list entries are groups of RS_NW strips. */
+__global__ __launch_bounds__(RS_NT) void kb_rcd_stream(const KFrame *__restrict__ frames,
+                                                       const uint32_t *__restrict__ tiles,
+                                                       uint32_t ntiles)
+{
+    for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const uint32_t d = tiles[t];
+        dev_rcd_stream(frames[KTILE_FRAME(d)], KTILE_IDX(d) * RS_NW + (threadIdx.x >> 6));
+    }
+}
+
+hipError_t craw_launch_rcd_stream(const KFrame &f, hipStream_t s)
+{
+    int ntx, n = craw_stage_tiles(KSTAGE_RCD_STREAM, f.W, f.H, 2, 0, &ntx);
+
+    if (!n)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_rcd_stream, dim3(n), dim3(RS_NT), 0, s, f);
+    return hipGetLastError();
+}
+
+hipError_t craw_launch_batch_rcd_stream(const KFrame *frames, const uint32_t *tiles,
+                                        uint32_t ntiles, int grid, hipStream_t s)
+{
+    if (!ntiles)
+        return hipSuccess;
+    hipLaunchKernelGGL(kb_rcd_stream, dim3(grid), dim3(RS_NT), 0, s, frames, tiles, ntiles);
+    return hipGetLastError();
+}

@@ -93,6 +93,9 @@ struct KFrame {
 #define KG_AWB_CHUNK 8192
 #define KG_XS_TW 224
 #define KG_XS_TH 56
+#define KG_RS_OUT 116           /* streamed RCD: output columns per strip */
+#define KG_RS_R   64            /* and rows */
+#define KG_RS_NW  4             /* strips per workgroup */
 
 enum KStage {
     KSTAGE_DENOISE = 0,
@@ -100,6 +103,8 @@ enum KStage {
     KSTAGE_CHROMA_BLUR,
     KSTAGE_CHROMA_UP,
     KSTAGE_AWB,
+    KSTAGE_RCD_RING,            /* tiled RCD tiles around the streamed interior */
+    KSTAGE_RCD_STREAM,          /* groups of KG_RS_NW interior strips */
     KSTAGE_NB,
 };
 
@@ -161,12 +166,78 @@ static inline void craw_stage_geom(int stage, int method, int dn_geom, int *tw, 
     }
 }
 
+/* The split of a frame for RCD into the chroma stage: whole 32x16 tiles at
+ * least 5 pixels (the filter's reach) from every border form the interior,
+ * developed by the streamed kernel; the ring of tiles around it by the
+ * tiled one. Tile units [itx0, itx1) x [ity0, ity1), empty for a frame too
+ * small to have one. */
+struct KRcdSplit {
+    int ntx, nty;
+    int itx0, itx1, ity0, ity1;
+    int x_lo, x_hi, y_lo, y_hi;
+};
+
+#ifdef __HIPCC__
+__host__ __device__
+#endif
+static inline KRcdSplit craw_rcd_split(int W, int H)
+{
+    KRcdSplit s;
+
+    s.ntx = (W + KG_RCD_TW - 1) / KG_RCD_TW;
+    s.nty = (H + KG_RCD_TH - 1) / KG_RCD_TH;
+    s.itx0 = 1;
+    s.itx1 = W - KG_RCD_TW * (s.ntx - 1) >= 5 ? s.ntx - 1 : s.ntx - 2;
+    s.ity0 = 1;
+    s.ity1 = H - KG_RCD_TH * (s.nty - 1) >= 5 ? s.nty - 1 : s.nty - 2;
+    if (s.itx1 <= s.itx0 || s.ity1 <= s.ity0)
+        s.itx0 = s.itx1 = s.ity0 = s.ity1 = 0;
+    s.x_lo = KG_RCD_TW * s.itx0;
+    s.x_hi = KG_RCD_TW * s.itx1;
+    s.y_lo = KG_RCD_TH * s.ity0;
+    s.y_hi = KG_RCD_TH * s.ity1;
+    return s;
+}
+
+/* Ring tile i: the rows above the interior, the columns either side of it,
+ * the rows below; all tiles when the interior is empty. */
+#ifdef __HIPCC__
+__host__ __device__
+#endif
+static inline void craw_rcd_ring_tile(const KRcdSplit &s, int i, int *tx, int *ty)
+{
+    const int top = s.ity0 * s.ntx, per = s.ntx - (s.itx1 - s.itx0);
+    const int mid = (s.ity1 - s.ity0) * per;
+
+    if (i < top) {
+        *ty = i / s.ntx;
+        *tx = i % s.ntx;
+    } else if (i < top + mid) {
+        const int j = (i - top) % per;
+        *ty = s.ity0 + (i - top) / per;
+        *tx = j < s.itx0 ? j : s.itx1 + (j - s.itx0);
+    } else {
+        *ty = s.ity1 + (i - top - mid) / s.ntx;
+        *tx = (i - top - mid) % s.ntx;
+    }
+}
+
 /* Number of tiles of `stage` for a frame of W x H (even W and H; the
  * quarter grid rounds up). */
 static inline int craw_stage_tiles(int stage, int W, int H, int method, int dn_geom, int *ntx)
 {
     int tw, th, div, planes, w, h;
 
+    if (stage == KSTAGE_RCD_RING || stage == KSTAGE_RCD_STREAM) {
+        const KRcdSplit sp = craw_rcd_split(W, H);
+        const int nstrips = ((sp.x_hi - sp.x_lo + KG_RS_OUT - 1) / KG_RS_OUT) *
+                            ((sp.y_hi - sp.y_lo + KG_RS_R - 1) / KG_RS_R);
+
+        *ntx = 1;
+        if (stage == KSTAGE_RCD_RING)
+            return sp.ntx * sp.nty - (sp.itx1 - sp.itx0) * (sp.ity1 - sp.ity0);
+        return (nstrips + KG_RS_NW - 1) / KG_RS_NW;
+    }
     if (stage == KSTAGE_AWB) {
         int nblocks = ((W + 7) / 8) * ((H + 7) / 8);
         *ntx = (nblocks + KG_AWB_CHUNK - 1) / KG_AWB_CHUNK;
@@ -185,6 +256,12 @@ hipError_t craw_launch_zero_u32(uint32_t *p, size_t n, hipStream_t s);
 hipError_t craw_launch_debayer(bool to_inter, const KFrame &f, hipStream_t s);
 hipError_t craw_launch_chroma_blur(const KFrame &f, hipStream_t s);
 hipError_t craw_launch_chroma_up_finish(const KFrame &f, hipStream_t s);
+/* RCD into the chroma stage over the interior of craw_rcd_split() (the
+ * tiled launcher does the ring). */
+hipError_t craw_launch_rcd_stream(const KFrame &f, hipStream_t s);
+hipError_t craw_launch_batch_rcd_stream(const KFrame *frames, const uint32_t *tiles,
+                                        uint32_t ntiles, int grid, hipStream_t s);
+
 /* The AWB histogram from the finish stage's block samples. */
 hipError_t craw_launch_awb_hist(const KFrame &f, hipStream_t s);
 
