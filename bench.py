@@ -58,10 +58,13 @@ ALGO_BYTES_PER_PX = {
     "chroma_up_color": 12 + 20.0 / 16 + 16,  # RGB planes + quarter planes in, RGBA32F out
     "awb_clear": 0.0,
 }
-# VALU operations per sample of the FP32 NLM (7x7 search, 3x3 patch):
-# 49 x (9 differences, 9 squares, 8 adds for the distance; 4 for the
-# weight's argument; ~22 for the deterministic exp; 3 to accumulate).
-NLM_OPS_PER_SAMPLE = 49 * (9 + 9 + 8 + 4 + 22 + 3)
+# VALU lane operations per sample of the exact NLM (7x7 search, 3x3 patch)
+# as the pairwise-weight kernel evaluates it: 24 weights a sample (a
+# difference and a square, 8 adds for the raster-order distance, 5 for
+# the argument and its clamp, 21 for the double-precision exp), 48
+# neighbours accumulated (3 each) with their weights fetched across lanes
+# (1 each), and the centre (3).
+NLM_OPS_PER_SAMPLE = 24 * (2 + 8 + 5 + 21) + 48 * (3 + 1) + 3
 
 
 def parse():

@@ -74,8 +74,9 @@ CRAW_API int craw_batch_get_awb_histogram(CRAWBatch *batch, uint32_t index,
  * craw_pipeline_get_stage_times(). */
 CRAW_API int craw_batch_get_stage_times(CRAWBatch *batch, CRAWStageTime *out, uint32_t max);
 
-/* Picks the persistent launch shape (workgroups per CU, per stage kind) by
- * timing each candidate on `frames` (developing them
+/* Picks the launch shape of each stage kind (one workgroup per tile, or a
+ * persistent 1 to 16 workgroups per CU) by timing each candidate on
+ * `frames` (developing them
  * `iterations` times per candidate on `stream`, synchronously), and keeps
  * the fastest for later recordings. Returns the chosen configuration's
  * time per recording in microseconds. Outputs are the same whichever

@@ -18,7 +18,7 @@
 #include "internal.h"
 
 #define RING 4
-#define NB_LISTS 11     /* 5 denoise modes, 2 debayer targets, blur, up, awb, RCD stream */
+#define NB_LISTS 12     /* 5 denoise modes, 2 debayer targets, blur, up, awb, 2 RCD streams */
 #define L_DN0 0
 #define L_DB_OUT 5
 #define L_DB_INTER 6
@@ -26,6 +26,7 @@
 #define L_CU 8
 #define L_AWB 9
 #define L_RS 10
+#define L_RS_F 11
 
 static const int dn_modes[5] = {
     KDN_SPATIAL, KDN_TEMPORAL, KDN_TEMPORAL | KDN_HISTORY, KDN_SPATIAL | KDN_TEMPORAL,
@@ -60,8 +61,8 @@ struct CRAWBatch {
     Staging ring[RING];
     uint32_t ring_pos;
 
-    /* persistent workgroups per CU for each list kind: denoise, debayer,
-     * blur, up */
+    /* persistent workgroups per CU for each list kind (denoise, debayer,
+     * blur, up); 0: one workgroup per tile */
     int wg_per_cu[4];
 
     /* graph of the last captured shape */
@@ -83,9 +84,11 @@ static size_t stage_tiles_bound(int stage, const CRAWBatchParams *p, int method,
 
     if (stage == KSTAGE_RCD_RING)       /* at most every tile of the frame */
         return stage_tiles_bound(KSTAGE_DEBAYER, p, 2, dn_geom);
-    if (stage == KSTAGE_RCD_STREAM) {   /* strips of the interior, in groups */
-        n = (double)p->max_pixels / ((double)KG_RS_OUT * KG_RS_R * KG_RS_NW) +
-            F * (((double)p->max_width / KG_RS_OUT + (double)p->max_height / KG_RS_R + 1.0) /
+    if (stage == KSTAGE_RCD_STREAM || stage == KSTAGE_RCD_STREAM_F) {
+        /* strips of the interior, in groups */
+        const double outw = stage == KSTAGE_RCD_STREAM ? KG_RS_OUT : KG_RS_OUT_F;
+        n = (double)p->max_pixels / (outw * KG_RS_R * KG_RS_NW) +
+            F * (((double)p->max_width / outw + (double)p->max_height / KG_RS_R + 1.0) /
                  KG_RS_NW + 1.0);
         return (size_t)n + 64;
     }
@@ -235,7 +238,7 @@ CRAW_API int craw_batch_create(CRAWContext *ctx, const CRAWBatchParams *params, 
     }
     b->form = b->cfg.fp16 ? KFORM_FP16 : (b->cfg.fast ? KFORM_FAST : KFORM_EXACT);
     for (int i = 0; i < 4; i++)
-        b->wg_per_cu[i] = 4;
+        b->wg_per_cu[i] = 0;
     if (hipSetDevice(ctx->device) != hipSuccess) {
         delete b;
         return CRAW_ERROR(ENODEV);
@@ -342,14 +345,15 @@ static int batch_plan(CRAWBatch *b, const CRAWBatchFrame *frames, uint32_t nb, B
             for (int t = 0; t < n; t++)
                 bp.lists[li].push_back(KTILE(i, t));
         }
-        if (pl.chroma && pl.f.method == 2) {
-            /* RCD into the chroma stage: ring tiles and interior strips */
+        if (pl.f.method == 2) {
+            /* RCD: ring tiles and interior strips */
             n = craw_stage_tiles(KSTAGE_RCD_RING, W, H, 2, 0, &ntx);
             for (int t = 0; t < n; t++)
-                bp.lists[L_DB_INTER].push_back(KTILE(i, t));
-            n = craw_stage_tiles(KSTAGE_RCD_STREAM, W, H, 2, 0, &ntx);
+                bp.lists[pl.chroma ? L_DB_INTER : L_DB_OUT].push_back(KTILE(i, t));
+            n = craw_stage_tiles(pl.chroma ? KSTAGE_RCD_STREAM : KSTAGE_RCD_STREAM_F, W, H, 2, 0,
+                                 &ntx);
             for (int t = 0; t < n; t++)
-                bp.lists[L_RS].push_back(KTILE(i, t));
+                bp.lists[pl.chroma ? L_RS : L_RS_F].push_back(KTILE(i, t));
         } else {
             n = craw_stage_tiles(KSTAGE_DEBAYER, W, H, pl.f.method, 0, &ntx);
             for (int t = 0; t < n; t++)
@@ -393,10 +397,15 @@ int craw_batch_check(CRAWBatch *b, const CRAWBatchFrame *frames, uint32_t nb)
     return batch_plan(b, frames, nb, bp);
 }
 
+/* Workgroups of a persistent launch over `ntiles` entries: wg_per_cu per
+ * CU, or with 0 one per entry (the hardware's own scheduling). */
 static int grid_for(CRAWBatch *b, int kind, size_t ntiles)
 {
     size_t g = (size_t)b->wg_per_cu[kind] * (size_t)b->ctx->cu_count;
-    return (int)(ntiles < g ? ntiles : g);
+
+    if (!b->wg_per_cu[kind] || ntiles < g)
+        return (int)ntiles;
+    return (int)g;
 }
 
 /* The stage launches of a planned recording, in order. */
@@ -433,11 +442,13 @@ static hipError_t batch_launch(CRAWBatch *b, const BatchPlan &bp, const size_t *
                                       b->tiles + list_off[l], (uint32_t)n, grid_for(b, 1, n), s);
         PE();
     }
-    if (e == hipSuccess && bp.lists[L_RS].size()) {
-        size_t n = bp.lists[L_RS].size();
+    for (int l = L_RS; l <= L_RS_F && e == hipSuccess; l++) {
+        size_t n = bp.lists[l].size();
+        if (!n)
+            continue;
         PB("rcd");
-        e = craw_launch_batch_rcd_stream(b->frames, b->tiles + list_off[L_RS], (uint32_t)n,
-                                         grid_for(b, 1, n), s);
+        e = craw_launch_batch_rcd_stream(l == L_RS, b->frames, b->tiles + list_off[l],
+                                         (uint32_t)n, grid_for(b, 1, n), s);
         PE();
     }
     if (e == hipSuccess && bp.lists[L_CB].size()) {
@@ -603,16 +614,17 @@ CRAW_API int craw_batch_get_stage_times(CRAWBatch *b, CRAWStageTime *out, uint32
     return craw_profiler_times(b->ctx, &b->prof, 0, out, max);
 }
 
-/* Times every stage of `iterations` recordings for each candidate number
- * of persistent workgroups per CU and keeps, per stage kind, the fastest.
- * The temporal states of the frames are left as they were. */
+/* Times every stage of `iterations` recordings for each candidate launch
+ * shape (one workgroup per tile, or 1 to 16 persistent workgroups per CU)
+ * and keeps, per stage kind, the fastest. The temporal states of the
+ * frames are left as they were. */
 CRAW_API int craw_batch_autotune(CRAWBatch *b, CRAWStream stream, const CRAWBatchFrame *frames,
                                  uint32_t nb, uint32_t iterations)
 {
-    static const int cand[] = { 1, 2, 4, 8, 16 };
+    static const int cand[] = { 0, 1, 2, 4, 8, 16 };
     const int ncand = (int)(sizeof(cand) / sizeof(cand[0]));
     double best[4] = { 1e30, 1e30, 1e30, 1e30 }, total = 0.0;
-    int pick[4] = { 4, 4, 4, 4 };
+    int pick[4] = { 0, 0, 0, 0 };
     hipStream_t s = (hipStream_t)stream;
     CRAWProfiler saved;
     std::vector<int> prev(nb), cur(nb);

@@ -787,15 +787,10 @@ __global__ __launch_bounds__(RC_NT) void k_debayer_rcd(KFrame f)
 {
     __shared__ SmRcd sm;
 
-    if (INTER) {
-        /* into the chroma stage: the ring around the streamed interior,
-         * blockIdx.x a ring tile */
-        int tx, ty;
-        craw_rcd_ring_tile(craw_rcd_split(f.W, f.H), blockIdx.x, &tx, &ty);
-        dev_debayer_rcd_tile<INTER>(sm, f, tx * RC_TW, ty * RC_TH);
-    } else {
-        dev_debayer_rcd_tile<INTER>(sm, f, blockIdx.x * RC_TW, blockIdx.y * RC_TH);
-    }
+    /* the ring around the streamed interior, blockIdx.x a ring tile */
+    int tx, ty;
+    craw_rcd_ring_tile(craw_rcd_split(f.W, f.H), blockIdx.x, &tx, &ty);
+    dev_debayer_rcd_tile<INTER>(sm, f, tx * RC_TW, ty * RC_TH);
 }
 
 __global__ __launch_bounds__(CB_TX * CB_TY) void k_chroma_blur(KFrame f)
@@ -842,11 +837,11 @@ __global__ __launch_bounds__(RC_NT) void kb_debayer_rcd(const KFrame *__restrict
         const uint32_t d = tiles[t];
         const KFrame &f = frames[KTILE_FRAME(d)];
         const int ntx = (f.W + RC_TW - 1) / RC_TW, i = KTILE_IDX(d);
-        int tx = i % ntx, ty = i / ntx;
+        int tx, ty;
 
-        /* into the chroma stage, entries are ring tiles */
-        if (INTER)
-            craw_rcd_ring_tile(craw_rcd_split(f.W, f.H), i, &tx, &ty);
+        /* entries are ring tiles */
+        (void)ntx;
+        craw_rcd_ring_tile(craw_rcd_split(f.W, f.H), i, &tx, &ty);
         __syncthreads();
         dev_debayer_rcd_tile<INTER>(sm, f, tx * RC_TW, ty * RC_TH);
     }
@@ -907,17 +902,17 @@ hipError_t craw_launch_zero_u32(uint32_t *p, size_t n, hipStream_t s)
 hipError_t craw_launch_debayer(bool to_inter, const KFrame &f, hipStream_t s)
 {
     if (f.method == 2) {
-        dim3 grid((f.W + RC_TW - 1) / RC_TW, (f.H + RC_TH - 1) / RC_TH);
-        if (to_inter) {
-            int ntx, nring = craw_stage_tiles(KSTAGE_RCD_RING, f.W, f.H, 2, 0, &ntx);
-            hipError_t e;
+        /* the ring of tiles, then the streamed interior */
+        int ntx, nring = craw_stage_tiles(KSTAGE_RCD_RING, f.W, f.H, 2, 0, &ntx);
+        hipError_t e;
 
+        if (to_inter)
             hipLaunchKernelGGL(k_debayer_rcd<true>, dim3(nring), dim3(RC_NT), 0, s, f);
-            if ((e = hipGetLastError()) != hipSuccess)
-                return e;
-            return craw_launch_rcd_stream(f, s);
-        }
-        hipLaunchKernelGGL(k_debayer_rcd<false>, grid, dim3(RC_NT), 0, s, f);
+        else
+            hipLaunchKernelGGL(k_debayer_rcd<false>, dim3(nring), dim3(RC_NT), 0, s, f);
+        if ((e = hipGetLastError()) != hipSuccess)
+            return e;
+        return craw_launch_rcd_stream(to_inter, f, s);
     } else {
         dim3 grid((f.W + BL_TW - 1) / BL_TW, (f.H + BL_TH - 1) / BL_TH);
         if (f.method == 0 && to_inter)

@@ -19,7 +19,11 @@
  * the mirror; the tiled kernel develops the ring of tiles around it.
  *
  * Output: the chroma stage's inputs (the three camera RGB planes and the
- * 4x4 block means of R - G, B - G, G, summed in raster order).
+ * 4x4 block means of R - G, B - G, G, summed in raster order), or without
+ * the chroma stage the finished frame: the AWB block samples (8x8 sums in
+ * raster order) and the colour stage into the destination. Strips are 116
+ * columns (29 4x4 blocks) into the chroma stage, 112 (14 8x8 blocks) to
+ * the output, so no block straddles two strips.
  */
 #include "k_common.h"
 #include "kernels.h"
@@ -28,7 +32,7 @@
 #define RS_NT (64 * RS_NW)
 #define RS_EPS 1e-5f
 
-static_assert(KG_RS_OUT == 116 && KG_RS_R % 4 == 0, "strip geometry");
+static_assert(KG_RS_OUT == 116 && KG_RS_OUT_F == 112 && KG_RS_R % 8 == 0, "strip geometry");
 
 #define DPP_WAVE_SHR1 0x138
 #define DPP_WAVE_SHL1 0x130
@@ -156,9 +160,57 @@ KDEV float rs_colour(const RsState &s)
     return w * (g00 + 0.5f * (dsw + dne)) + (1.0f - w) * (g00 + 0.5f * (dnw + dse));
 }
 
+/* The output of row oy without the chroma stage: each 8x8 block's sums in
+ * raster order (the block's first lane, column X0 % 8 == 0, adds its two
+ * columns and then its three right neighbours'; lib/awb.c:249-256), its
+ * sample on the block's last row, and the colour stage into the
+ * destination (lib/ref/ref_color.c:10-76). */
+KDEV void rs_finish(RsState &s, const KFrame &f, int oy, bool out_cols, int X0,
+                    const float r[2], const float g[2], const float b[2])
+{
+    if (f.out.hist) {
+        const bool first_row = (oy & 7) == 0;
+        const float *chan[3] = { r, g, b };
+
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const float n10 = rs_shl(chan[c][0]), n11 = rs_shl(chan[c][1]);
+            const float n20 = rs_shl(n10), n21 = rs_shl(n11);
+            const float n30 = rs_shl(n20), n31 = rs_shl(n21);
+            float a = (first_row ? 0.0f : s.acc[c]) + chan[c][0];
+
+            a = a + chan[c][1];
+            a = a + n10;
+            a = a + n11;
+            a = a + n20;
+            a = a + n21;
+            a = a + n30;
+            a = a + n31;
+            s.acc[c] = a;
+        }
+        if ((oy & 7) == 7 && out_cols && (X0 & 7) == 0) {
+            CRAWAwbSample smp;
+            const bool valid = craw_awb_sample(s.acc, 64.0f, f.out.awb_wb0, f.out.awb_wb2, &smp);
+
+            f.out.samples[(size_t)(oy / 8) * ((f.W + 7) / 8) + X0 / 8] =
+                valid ? make_uint2(smp.bin | (smp.weight << 14), smp.bright)
+                      : make_uint2(KAWB_NONE, 0u);
+        }
+    }
+    if (out_cols) {
+#pragma unroll 1
+        for (int c = 0; c < 2; c++) {
+            float o[3];
+
+            kcolor(f.col, r[c], g[c], b[c], o);
+            kstore_rgba(f.out, X0 + c, oy, o);
+        }
+    }
+}
+
 /* One step: P is the parity of the step (the newest V row is odd for
  * P = 0, the strip starting on an even row). */
-template <int P>
+template <bool INTER, int P>
 KDEV void rs_step(RsState &s, const KFrame &f, int vt, int ld0, bool out_row, bool out_cols,
                   int X0)
 {
@@ -241,6 +293,11 @@ KDEV void rs_step(RsState &s, const KFrame &f, int vt, int ld0, bool out_row, bo
             b[CG] = P == 0 ? cv : ch;
         }
 
+        if (!INTER) {
+            rs_finish(s, f, oy, out_cols, X0, r, g, b);
+            return;
+        }
+
         if (out_cols) {
             const size_t o = (size_t)oy * f.W + X0;
             *(float2 *)&it.rgb[o] = make_float2(r[0], r[1]);
@@ -284,10 +341,12 @@ KDEV void rs_step(RsState &s, const KFrame &f, int vt, int ld0, bool out_row, bo
 }
 
 /* One strip: output columns [xs, xe), rows [ys, ye) of the interior. */
+template <bool INTER>
 KDEV void dev_rcd_stream(const KFrame &f, int strip)
 {
+    constexpr int OUTW = INTER ? KG_RS_OUT : KG_RS_OUT_F;
     const KRcdSplit sp = craw_rcd_split(f.W, f.H);
-    const int nsx = (sp.x_hi - sp.x_lo + KG_RS_OUT - 1) / KG_RS_OUT;
+    const int nsx = (sp.x_hi - sp.x_lo + OUTW - 1) / OUTW;
     const int nsy = (sp.y_hi - sp.y_lo + KG_RS_R - 1) / KG_RS_R;
     const int lane = threadIdx.x & 63;
 
@@ -295,8 +354,8 @@ KDEV void dev_rcd_stream(const KFrame &f, int strip)
         return;
     {
         const int sx = strip % nsx, sy = strip / nsx;
-        const int xs = sp.x_lo + sx * KG_RS_OUT, ys = sp.y_lo + sy * KG_RS_R;
-        const int xe = min(xs + KG_RS_OUT, sp.x_hi), ye = min(ys + KG_RS_R, sp.y_hi);
+        const int xs = sp.x_lo + sx * OUTW, ys = sp.y_lo + sy * KG_RS_R;
+        const int xe = min(xs + OUTW, sp.x_hi), ye = min(ys + KG_RS_R, sp.y_hi);
         const int X0 = xs - 6 + 2 * lane;
         const bool out_cols = X0 >= xs && X0 < xe;
         /* the pair's load column, kept inside the frame for lanes past the
@@ -326,43 +385,55 @@ KDEV void dev_rcd_stream(const KFrame &f, int strip)
         /* ys is even and nsteps is even: steps alternate parity 0, 1 */
 #pragma unroll 1
         for (int t = 0; t < nsteps; t += 2) {
-            rs_step<0>(s, f, ys - 5 + t, ld0, t >= 10, out_cols, X0);
-            rs_step<1>(s, f, ys - 4 + t, ld0, t + 1 >= 10, out_cols, X0);
+            rs_step<INTER, 0>(s, f, ys - 5 + t, ld0, t >= 10, out_cols, X0);
+            rs_step<INTER, 1>(s, f, ys - 4 + t, ld0, t + 1 >= 10, out_cols, X0);
         }
     }
 }
 
+template <bool INTER>
 __global__ __launch_bounds__(RS_NT) void k_rcd_stream(KFrame f)
 {
-    dev_rcd_stream(f, blockIdx.x * RS_NW + (threadIdx.x >> 6));
+    dev_rcd_stream<INTER>(f, blockIdx.x * RS_NW + (threadIdx.x >> 6));
 }
 
 /* Persistent form: work-list entries are groups of RS_NW strips. */
+template <bool INTER>
 __global__ __launch_bounds__(RS_NT) void kb_rcd_stream(const KFrame *__restrict__ frames,
                                                        const uint32_t *__restrict__ tiles,
                                                        uint32_t ntiles)
 {
     for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const uint32_t d = tiles[t];
-        dev_rcd_stream(frames[KTILE_FRAME(d)], KTILE_IDX(d) * RS_NW + (threadIdx.x >> 6));
+        dev_rcd_stream<INTER>(frames[KTILE_FRAME(d)], KTILE_IDX(d) * RS_NW + (threadIdx.x >> 6));
     }
 }
 
-hipError_t craw_launch_rcd_stream(const KFrame &f, hipStream_t s)
+hipError_t craw_launch_rcd_stream(bool to_inter, const KFrame &f, hipStream_t s)
 {
-    int ntx, n = craw_stage_tiles(KSTAGE_RCD_STREAM, f.W, f.H, 2, 0, &ntx);
+    int ntx, n = craw_stage_tiles(to_inter ? KSTAGE_RCD_STREAM : KSTAGE_RCD_STREAM_F, f.W, f.H,
+                                  2, 0, &ntx);
 
     if (!n)
         return hipSuccess;
-    hipLaunchKernelGGL(k_rcd_stream, dim3(n), dim3(RS_NT), 0, s, f);
+    if (to_inter)
+        hipLaunchKernelGGL(k_rcd_stream<true>, dim3(n), dim3(RS_NT), 0, s, f);
+    else
+        hipLaunchKernelGGL(k_rcd_stream<false>, dim3(n), dim3(RS_NT), 0, s, f);
     return hipGetLastError();
 }
 
-hipError_t craw_launch_batch_rcd_stream(const KFrame *frames, const uint32_t *tiles,
-                                        uint32_t ntiles, int grid, hipStream_t s)
+hipError_t craw_launch_batch_rcd_stream(bool to_inter, const KFrame *frames,
+                                        const uint32_t *tiles, uint32_t ntiles, int grid,
+                                        hipStream_t s)
 {
     if (!ntiles)
         return hipSuccess;
-    hipLaunchKernelGGL(kb_rcd_stream, dim3(grid), dim3(RS_NT), 0, s, frames, tiles, ntiles);
+    if (to_inter)
+        hipLaunchKernelGGL(kb_rcd_stream<true>, dim3(grid), dim3(RS_NT), 0, s, frames, tiles,
+                           ntiles);
+    else
+        hipLaunchKernelGGL(kb_rcd_stream<false>, dim3(grid), dim3(RS_NT), 0, s, frames, tiles,
+                           ntiles);
     return hipGetLastError();
 }

@@ -94,6 +94,7 @@ struct KFrame {
 #define KG_XS_TW 224
 #define KG_XS_TH 56
 #define KG_RS_OUT 116           /* streamed RCD: output columns per strip */
+#define KG_RS_OUT_F 112         /* the same without the chroma stage */
 #define KG_RS_R   64            /* and rows */
 #define KG_RS_NW  4             /* strips per workgroup */
 
@@ -105,6 +106,7 @@ enum KStage {
     KSTAGE_AWB,
     KSTAGE_RCD_RING,            /* tiled RCD tiles around the streamed interior */
     KSTAGE_RCD_STREAM,          /* groups of KG_RS_NW interior strips */
+    KSTAGE_RCD_STREAM_F,        /* the same, strips to the output */
     KSTAGE_NB,
 };
 
@@ -166,7 +168,7 @@ static inline void craw_stage_geom(int stage, int method, int dn_geom, int *tw, 
     }
 }
 
-/* The split of a frame for RCD into the chroma stage: whole 32x16 tiles at
+/* The split of a frame for RCD: whole 32x16 tiles at
  * least 5 pixels (the filter's reach) from every border form the interior,
  * developed by the streamed kernel; the ring of tiles around it by the
  * tiled one. Tile units [itx0, itx1) x [ity0, ity1), empty for a frame too
@@ -228,9 +230,11 @@ static inline int craw_stage_tiles(int stage, int W, int H, int method, int dn_g
 {
     int tw, th, div, planes, w, h;
 
-    if (stage == KSTAGE_RCD_RING || stage == KSTAGE_RCD_STREAM) {
+    if (stage == KSTAGE_RCD_RING || stage == KSTAGE_RCD_STREAM ||
+        stage == KSTAGE_RCD_STREAM_F) {
         const KRcdSplit sp = craw_rcd_split(W, H);
-        const int nstrips = ((sp.x_hi - sp.x_lo + KG_RS_OUT - 1) / KG_RS_OUT) *
+        const int outw = stage == KSTAGE_RCD_STREAM_F ? KG_RS_OUT_F : KG_RS_OUT;
+        const int nstrips = ((sp.x_hi - sp.x_lo + outw - 1) / outw) *
                             ((sp.y_hi - sp.y_lo + KG_RS_R - 1) / KG_RS_R);
 
         *ntx = 1;
@@ -256,11 +260,12 @@ hipError_t craw_launch_zero_u32(uint32_t *p, size_t n, hipStream_t s);
 hipError_t craw_launch_debayer(bool to_inter, const KFrame &f, hipStream_t s);
 hipError_t craw_launch_chroma_blur(const KFrame &f, hipStream_t s);
 hipError_t craw_launch_chroma_up_finish(const KFrame &f, hipStream_t s);
-/* RCD into the chroma stage over the interior of craw_rcd_split() (the
- * tiled launcher does the ring). */
-hipError_t craw_launch_rcd_stream(const KFrame &f, hipStream_t s);
-hipError_t craw_launch_batch_rcd_stream(const KFrame *frames, const uint32_t *tiles,
-                                        uint32_t ntiles, int grid, hipStream_t s);
+/* RCD over the interior of craw_rcd_split(), into the chroma stage or to
+ * the output (the tiled launcher does the ring). */
+hipError_t craw_launch_rcd_stream(bool to_inter, const KFrame &f, hipStream_t s);
+hipError_t craw_launch_batch_rcd_stream(bool to_inter, const KFrame *frames,
+                                        const uint32_t *tiles, uint32_t ntiles, int grid,
+                                        hipStream_t s);
 
 /* The AWB histogram from the finish stage's block samples. */
 hipError_t craw_launch_awb_hist(const KFrame &f, hipStream_t s);

@@ -102,21 +102,25 @@ def test_chroma(gpu_available):
     _check(run, bayer, frame_params(prof, denoise_chroma_strength=0.5), "chroma", hist=True)
 
 
+@pytest.mark.parametrize("chroma", [True, False])
 @pytest.mark.parametrize("size", [(150, 98), (520, 336), (610, 298), (96, 40), (166, 70), (64, 32)])
-def test_rcd_into_chroma_sizes(gpu_available, size):
-    """RCD into the chroma stage develops the frame's interior with the
-    streamed kernel and the ring of tiles around it with the tiled one:
-    interiors of several strips (partial strips at the right and bottom),
-    last tiles narrower than the filter's reach (the interior then stops a
-    tile earlier), and frames too small to have an interior."""
+def test_rcd_interior_and_ring(gpu_available, size, chroma):
+    """RCD develops the frame's interior with the streamed kernel and the
+    ring of tiles around it with the tiled one, into the chroma stage or to
+    the output (AWB samples and colour): interiors of several strips
+    (partial strips at the right and bottom), last tiles narrower than the
+    filter's reach (the interior then stops a tile earlier), and frames too
+    small to have an interior."""
     w, h = size
-    run = Runner(w, h, debayer=capi.CRAW_DEBAYER_RCD, out_space=capi.CRAW_CS_BT709_LINEAR,
-                 flags=capi.CRAW_PIPELINE_FLAG_CHROMA_DENOISE |
-                 capi.CRAW_PIPELINE_FLAG_AWB_HISTOGRAM)
+    flags = capi.CRAW_PIPELINE_FLAG_AWB_HISTOGRAM
+    if chroma:
+        flags |= capi.CRAW_PIPELINE_FLAG_CHROMA_DENOISE
+    run = Runner(w, h, debayer=capi.CRAW_DEBAYER_RCD, out_space=capi.CRAW_CS_ACES_AP1_LINEAR,
+                 flags=flags)
     prof = scene.make_profile()
     bayer = scene.make_frame(w, h, seed=33 + w)
-    _check(run, bayer, frame_params(prof, denoise_chroma_strength=0.7), "rcd chroma %dx%d" % size,
-           hist=True)
+    _check(run, bayer, frame_params(prof, denoise_chroma_strength=0.7),
+           "rcd %dx%d chroma %d" % (w, h, chroma), hist=True)
 
 
 def test_temporal_sequence(gpu_available):
