@@ -18,6 +18,8 @@ sample and on all cores on the full frame, both on the same seeded scene.
 
 `forms` measures the other forms on the same GPU (omitted with --plain-only):
   - plain_fp16: the plain form with the packed-FP16 NLM;
+  - temporal_clip: 60 frames of a 5760x3240 clip through one pipeline,
+    temporal 0.8 + NLM 1.0 (FP32 exact, with the history), MHC, BT.2020;
   - batched_5p8k: the 5.8K full pipeline, 4 frames per batch recording, FP32
     and packed-FP16 NLM, hipGraph replay, autotuned launch shapes;
   - batched_mix: 64 frames of uneven sizes per recording (16 each at
@@ -364,6 +366,45 @@ def time_plain_trace(ctx, capi, hip, P, mos, sched, sizes, fps_stream, fp_still)
             "frames": len(sched)}
 
 
+def scene_profile():
+    from libcraw_amd import scene
+    return scene.make_profile()
+
+
+def time_temporal_clip(ctx, capi, hip, P, mos, frames):
+    """The temporal form of the plain pipeline: a clip of `frames` frames at
+    5760x3240 through one pipeline (its temporal state threaded through),
+    temporal 0.8 + NLM 1.0, MHC, BT.2020, RGBA32F, FP32 exact NLM. The
+    frames repeat one seeded mosaic: the work per frame does not depend on
+    the content."""
+    fmt = capi.CRAW_FORMAT_R32G32B32A32_SFLOAT
+    flags = (capi.CRAW_PIPELINE_FLAG_DENOISE | capi.CRAW_PIPELINE_FLAG_TEMPORAL_DENOISE |
+             capi.CRAW_PIPELINE_FLAG_PROFILE)
+    pipe = P.Pipeline(ctx, P.pipeline_params(W, H, debayer=capi.CRAW_DEBAYER_MHC,
+                                             out_space=capi.CRAW_CS_BT2020_LINEAR,
+                                             out_format=fmt, flags=flags))
+    dst = hip.DeviceBuffer(W * H * 16)
+    fp = P.frame_params(scene_profile(), denoise_strength=1.0, denoise_temporal_strength=0.8,
+                        noise_override=(4e-4, 1.6e-5))
+    si = P.image(mos.buf[(W, H)], W, H, capi.CRAW_FORMAT_R16_UNORM)
+    di = P.image(dst, W, H, fmt)
+    stream = hip.Stream()
+    assert pipe.record(stream, si, di, fp) == 0        # first frame: no history
+    stream.synchronize()
+    e0, e1 = hip.Event(), hip.Event()
+    e0.record(stream)
+    for _ in range(frames):
+        assert pipe.record(stream, si, di, fp) == 0
+    e1.record(stream)
+    stream.synchronize()
+    ms = e0.elapsed_ms(e1) / frames
+    st = {n: round(t / c, 4) for n, c, t in pipe.stage_times(0)}
+    pipe.close()
+    dst.close()
+    return {"mpix_s": round(W * H / 1e6 / (ms / 1e3), 1), "ms_per_frame": round(ms, 4),
+            "frames": frames, "stage_ms": st}
+
+
 def measure_forms(args, ctx, capi, hip, P, scene, plain_value):
     forms = {}
     rng = np.random.Generator(np.random.Philox(2024))
@@ -383,6 +424,10 @@ def measure_forms(args, ctx, capi, hip, P, scene, plain_value):
         print("bench: batched 5.8K %s %.0f Mpix/s" % (name, r["mpix_s"]), file=sys.stderr,
               flush=True)
     forms["batched_5p8k"] = b58
+
+    forms["temporal_clip"] = time_temporal_clip(ctx, capi, hip, P, mos, 60)
+    print("bench: temporal clip %.0f Mpix/s" % forms["temporal_clip"]["mpix_s"],
+          file=sys.stderr, flush=True)
 
     sizes = [sz for sz in MIX_SIZES for _ in range(16)]
     profiles = []
