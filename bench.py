@@ -8,11 +8,15 @@ out; FP32 throughout, the plain form: one craw_pipeline_record() per frame
 on one stream. The input frame is resident in HBM before timing starts.
 That is `value`.
 
-Prints one JSON line. `roofline` is the dominant kernel's: algorithmic
-bytes per launch over its average launch time, taken from the library's
-own hipEvent pair around each launch (CRAW_PIPELINE_FLAG_PROFILE, on the
-stream the kernels run on) over the last recordings of the timed region;
-`kernels` gives the same for every kernel. `cpu_baseline` times the oracle
+Prints one JSON line. `value` and `ms_per_step` come from hipEvents on the
+launch stream around K recordings of a pipeline without profiling. The
+same K frames are then recorded through a pipeline created with
+CRAW_PIPELINE_FLAG_PROFILE, whose per-stage hipEvents ride on the stages'
+kernel dispatches (hipExtLaunchKernel) on the same stream: `roofline` is
+the dominant kernel's algorithmic bytes per launch over its average launch
+time from those events, `kernels` the same for every kernel, and
+`ms_per_step_profiled` that run's frame time (the dispatch timestamps add
+a few microseconds per kernel). `cpu_baseline` times the oracle
 (the CPU restatement of lib/ref) on the host: single-threaded on a bounded
 sample and on all cores on the full frame, both on the same seeded scene.
 
@@ -59,14 +63,17 @@ ALGO_BYTES_PER_PX = {
     "chroma_blur": 5 * 4 / 16.0,             # 3 quarter planes in, 2 out
     "chroma_up_color": 12 + 20.0 / 16 + 16,  # RGB planes + quarter planes in, RGBA32F out
     "awb_clear": 0.0,
+    "awb_export": 0.0,
 }
 # VALU lane operations per sample of the exact NLM (7x7 search, 3x3 patch)
 # as the pairwise-weight kernel evaluates it: 24 weights a sample (a
-# difference and a square, 8 adds for the raster-order distance, 5 for
-# the argument and its clamp, 21 for the double-precision exp), 48
-# neighbours accumulated (3 each) with their weights fetched across lanes
-# (1 each), and the centre (3).
-NLM_OPS_PER_SAMPLE = 24 * (2 + 8 + 5 + 21) + 48 * (3 + 1) + 3
+# difference and a square, 8 adds for the raster-order distance, 3 for the
+# argument and 1 for its clamp, 18 for the double-precision exp: convert,
+# reduce in 4, a degree-10 polynomial in 10 FMAs, scale, round to float),
+# 48 neighbours accumulated (3 each: weight sum, product, sum) and the
+# centre (3). The weights of the neighbours before the centre come from
+# other lanes through ds_bpermute, an LDS instruction.
+NLM_OPS_PER_SAMPLE = 24 * (2 + 8 + 3 + 1 + 18) + 48 * 3 + 3
 
 
 def parse():
@@ -376,33 +383,38 @@ def time_temporal_clip(ctx, capi, hip, P, mos, frames):
     5760x3240 through one pipeline (its temporal state threaded through),
     temporal 0.8 + NLM 1.0, MHC, BT.2020, RGBA32F, FP32 exact NLM. The
     frames repeat one seeded mosaic: the work per frame does not depend on
-    the content."""
+    the content. Timed without profiling; the stage times come from four
+    more frames through a profiled pipeline."""
     fmt = capi.CRAW_FORMAT_R32G32B32A32_SFLOAT
-    flags = (capi.CRAW_PIPELINE_FLAG_DENOISE | capi.CRAW_PIPELINE_FLAG_TEMPORAL_DENOISE |
-             capi.CRAW_PIPELINE_FLAG_PROFILE)
-    pipe = P.Pipeline(ctx, P.pipeline_params(W, H, debayer=capi.CRAW_DEBAYER_MHC,
-                                             out_space=capi.CRAW_CS_BT2020_LINEAR,
-                                             out_format=fmt, flags=flags))
+    flags = capi.CRAW_PIPELINE_FLAG_DENOISE | capi.CRAW_PIPELINE_FLAG_TEMPORAL_DENOISE
     dst = hip.DeviceBuffer(W * H * 16)
     fp = P.frame_params(scene_profile(), denoise_strength=1.0, denoise_temporal_strength=0.8,
                         noise_override=(4e-4, 1.6e-5))
     si = P.image(mos.buf[(W, H)], W, H, capi.CRAW_FORMAT_R16_UNORM)
     di = P.image(dst, W, H, fmt)
     stream = hip.Stream()
-    assert pipe.record(stream, si, di, fp) == 0        # first frame: no history
-    stream.synchronize()
     e0, e1 = hip.Event(), hip.Event()
-    e0.record(stream)
-    for _ in range(frames):
-        assert pipe.record(stream, si, di, fp) == 0
-    e1.record(stream)
-    stream.synchronize()
-    ms = e0.elapsed_ms(e1) / frames
-    st = {n: round(t / c, 4) for n, c, t in pipe.stage_times(0)}
-    pipe.close()
+    out = {}
+    for extra, n in ((0, frames), (capi.CRAW_PIPELINE_FLAG_PROFILE, 4)):
+        pipe = P.Pipeline(ctx, P.pipeline_params(W, H, debayer=capi.CRAW_DEBAYER_MHC,
+                                                 out_space=capi.CRAW_CS_BT2020_LINEAR,
+                                                 out_format=fmt, flags=flags | extra))
+        assert pipe.record(stream, si, di, fp) == 0        # first frame: no history
+        stream.synchronize()
+        e0.record(stream)
+        for _ in range(n):
+            assert pipe.record(stream, si, di, fp) == 0
+        e1.record(stream)
+        stream.synchronize()
+        if extra:
+            out["stage_ms"] = {k: round(t / c, 4) for k, c, t in pipe.stage_times(0)}
+        else:
+            ms = e0.elapsed_ms(e1) / n
+            out.update({"mpix_s": round(W * H / 1e6 / (ms / 1e3), 1),
+                        "ms_per_frame": round(ms, 4), "frames": n})
+        pipe.close()
     dst.close()
-    return {"mpix_s": round(W * H / 1e6 / (ms / 1e3), 1), "ms_per_frame": round(ms, 4),
-            "frames": frames, "stage_ms": st}
+    return out
 
 
 def measure_forms(args, ctx, capi, hip, P, scene, plain_value):
@@ -475,11 +487,15 @@ def main():
     bayer = scene.make_frame(W, H, seed=1, a=NOISE[0], b=NOISE[1])
     fmt = capi.CRAW_FORMAT_R32G32B32A32_SFLOAT
     flags = (capi.CRAW_PIPELINE_FLAG_DENOISE | capi.CRAW_PIPELINE_FLAG_CHROMA_DENOISE |
-             capi.CRAW_PIPELINE_FLAG_AWB_HISTOGRAM | capi.CRAW_PIPELINE_FLAG_PROFILE)
+             capi.CRAW_PIPELINE_FLAG_AWB_HISTOGRAM)
     ctx = Context(0)
-    pp = pipeline_params(W, H, debayer=capi.CRAW_DEBAYER_RCD,
-                         out_space=capi.CRAW_CS_ACES_AP1_LINEAR, out_format=fmt, flags=flags)
-    pipe = Pipeline(ctx, pp)
+
+    def make(extra):
+        return Pipeline(ctx, pipeline_params(W, H, debayer=capi.CRAW_DEBAYER_RCD,
+                                             out_space=capi.CRAW_CS_ACES_AP1_LINEAR,
+                                             out_format=fmt, flags=flags | extra))
+
+    pipe = make(0)
     src, dst = hip.DeviceBuffer(W * H * 2), hip.DeviceBuffer(W * H * 16)
     src.upload(bayer)
     stream = hip.Stream()
@@ -487,34 +503,48 @@ def main():
                       noise_override=NOISE)
     si = image(src, W, H, capi.CRAW_FORMAT_R16_UNORM)
     di = image(dst, W, H, fmt)
-
-    for _ in range(args.warmup):
-        assert pipe.record(stream, si, di, fp) == 0
-    stream.synchronize()
-    print("bench: warmup done", file=sys.stderr, flush=True)
-
     e0, e1 = hip.Event(), hip.Event()
-    hip.synchronize()
-    t0 = time.perf_counter()
-    e0.record(stream)
-    for _ in range(args.steps):
-        assert pipe.record(stream, si, di, fp) == 0
-    e1.record(stream)
-    stream.synchronize()
-    hip.synchronize()
-    wall = time.perf_counter() - t0
-    gpu_ms = e0.elapsed_ms(e1)
+
+    def timed(p, steps, warmup):
+        """GPU ms over `steps` recordings bracketed by hipEvents on the
+        launch stream, after `warmup` untimed ones; host ms spent
+        recording; wall seconds."""
+        for _ in range(warmup):
+            assert p.record(stream, si, di, fp) == 0
+        stream.synchronize()
+        hip.synchronize()
+        t0 = time.perf_counter()
+        e0.record(stream)
+        for _ in range(steps):
+            assert p.record(stream, si, di, fp) == 0
+        t_enq = time.perf_counter() - t0
+        e1.record(stream)
+        stream.synchronize()
+        hip.synchronize()
+        return e0.elapsed_ms(e1), t_enq * 1e3, time.perf_counter() - t0
+
+    # The timed region: the product as a caller runs it (no profiling).
+    gpu_ms, enq_ms, wall = timed(pipe, args.steps, args.warmup)
+    print("bench: timed region done", file=sys.stderr, flush=True)
     ms_per_step = gpu_ms / args.steps
     mpix = W * H / 1e6
     value = mpix * args.steps / (gpu_ms / 1e3)
 
-    # per-kernel averages over the last recordings of the timed region
+    # The same K frames again through a pipeline created with
+    # CRAW_PIPELINE_FLAG_PROFILE: the library's per-stage hipEvents, bound
+    # to the stage's kernel dispatches on the launch stream, give each
+    # kernel's average launch duration (the timestamps cost the dispatches
+    # a few microseconds each, so the throughput above is measured
+    # without them).
+    pipe_prof = make(capi.CRAW_PIPELINE_FLAG_PROFILE)
+    gpu_ms_prof, _, _ = timed(pipe_prof, args.steps, 2)
     stages = {}
     for back in range(min(args.steps, capi_slots())):
-        for name, count, ms in pipe.stage_times(back):
+        for name, count, ms in pipe_prof.stage_times(back):
             s = stages.setdefault(name, [0, 0.0])
             s[0] += count
             s[1] += ms
+    pipe_prof.close()
     stage_ms = {k: v[1] / v[0] for k, v in stages.items()}
     dom = max(stage_ms, key=stage_ms.get)
     algo = ALGO_BYTES_PER_PX.get(dom, 0.0) * W * H
@@ -559,7 +589,9 @@ def main():
                                "histogram, ACES AP1 + RGC, RGBA32F",
                    "width": W, "height": H, "noise": list(NOISE), "form": "plain"},
         "wall_s": round(wall, 3),
+        "host_record_ms": round(enq_ms / args.steps, 4),
         "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()},
+        "ms_per_step_profiled": round(gpu_ms_prof / args.steps, 4),
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 4),
                        "p99": round(float(np.percentile(lat, 99)), 4), "n": int(lat.size),
                        "form": "plain"},
@@ -581,18 +613,11 @@ def main():
     if not args.plain_only:
         import libcraw_amd.pipeline as P
         # the plain form with the packed-FP16 NLM, same frame
-        pp16 = pipeline_params(W, H, debayer=capi.CRAW_DEBAYER_RCD,
-                               out_space=capi.CRAW_CS_ACES_AP1_LINEAR, out_format=fmt,
-                               flags=flags | capi.CRAW_PIPELINE_FLAG_NLM_FP16)
-        p16 = Pipeline(ctx, pp16)
-        for _ in range(2):
-            assert p16.record(stream, si, di, fp) == 0
-        e0.record(stream)
-        for _ in range(args.steps):
-            assert p16.record(stream, si, di, fp) == 0
-        e1.record(stream)
-        stream.synchronize()
-        ms16 = e0.elapsed_ms(e1) / args.steps
+        p16 = make(capi.CRAW_PIPELINE_FLAG_NLM_FP16)
+        ms16 = timed(p16, args.steps, 2)[0] / args.steps
+        p16.close()
+        p16 = make(capi.CRAW_PIPELINE_FLAG_NLM_FP16 | capi.CRAW_PIPELINE_FLAG_PROFILE)
+        timed(p16, 4, 2)
         st16 = {n: round(ms / c, 4) for n, c, ms in p16.stage_times(0)}
         p16.close()
         out["forms"] = {"plain_fp16": {"mpix_s": round(mpix / (ms16 / 1e3), 1),

@@ -550,7 +550,7 @@ static int batch_record(CRAWBatch *b, hipStream_t s, const CRAWBatchFrame *frame
         }
         if (b->prof.on) {
             CRAWProfRun pr = craw_profiler_start(&b->prof, s);
-            craw_profiler_begin(pr, "batch_graph");
+            craw_profiler_begin(pr, "batch_graph", false);
             e = hipGraphLaunch(b->exec, s);
             craw_profiler_end(pr);
             craw_profiler_finish(pr);

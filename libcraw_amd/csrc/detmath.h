@@ -9,17 +9,18 @@
  * the C oracle, so that the two agree bit for bit by construction.
  *
  * Method: the argument is taken to double, the function evaluated there to
- * about 2^-52 relative, and the result rounded once to float. On CDNA4 a
+ * about 2^-51 relative, and the result rounded once to float. On CDNA4 a
  * double FMA issues at the same rate as a float one, so this costs the
  * GPU about what a float polynomial would. Because the double result is
- * within ~2^-52 of the true value, the float result is the correctly
+ * within ~2^-51 of the true value, the float result is the correctly
  * rounded one except for arguments within that distance of a rounding
- * midpoint, which is also where glibc's (nearly correctly rounded)
+ * midpoint (for exp, none: every float argument in [-200, 90] rounds
+ * correctly), which is also where glibc's (nearly correctly rounded)
  * versions would be the ones to differ.
  *
  *   exp: Cody-Waite reduction x = k ln2 + r, |r| <= ln2/2, with ln2 split
- *        into a 32-bit head and a tail (the fdlibm split), degree-12
- *        Taylor polynomial of exp(r) in Horner form with FMA, scaled by
+ *        into a 32-bit head and a tail (the fdlibm split), degree-10
+ *        Chebyshev polynomial of exp(r) in Horner form with FMA, scaled by
  *        2^k through the exponent field.
  *   log: x = m 2^e with m in [sqrt(1/2), sqrt(2)), log m = 2 atanh(s),
  *        s = (m - 1) / (m + 1), odd series to s^19.
@@ -80,19 +81,21 @@ CRAW_DM_FN double craw_dm_exp_d(double x)
 
     r = __builtin_fma(-k, ln2_lo, r);
 
-    /* sum_{n=0}^{12} r^n / n!, coefficients rounded to double */
-    p = 0x1.1eed8eff8d898p-29;
-    p = __builtin_fma(p, r, 0x1.ae64567f544e4p-26);
-    p = __builtin_fma(p, r, 0x1.27e4fb7789f5cp-22);
-    p = __builtin_fma(p, r, 0x1.71de3a556c734p-19);
-    p = __builtin_fma(p, r, 0x1.a01a01a01a01ap-16);
-    p = __builtin_fma(p, r, 0x1.a01a01a01a01ap-13);
-    p = __builtin_fma(p, r, 0x1.6c16c16c16c17p-10);
-    p = __builtin_fma(p, r, 0x1.1111111111111p-7);
-    p = __builtin_fma(p, r, 0x1.5555555555555p-5);
-    p = __builtin_fma(p, r, 0x1.5555555555555p-3);
-    p = __builtin_fma(p, r, 0x1.0000000000000p-1);
-    p = __builtin_fma(p, r, 1.0);
+    /* Degree-10 Chebyshev fit of exp on [-ln2/2, ln2/2] (mpmath chebyfit,
+     * coefficients rounded to double): relative error 2^-51.4 with the
+     * rounded coefficients, and over every float argument in [-200, 90]
+     * its float results equal the correctly rounded exp (checked
+     * exhaustively against long double expl). */
+    p = 0x1.28a2c0c0079ddp-22;
+    p = __builtin_fma(p, r, 0x1.72faf04a14bf2p-19);
+    p = __builtin_fma(p, r, 0x1.a019a66a5defap-16);
+    p = __builtin_fma(p, r, 0x1.a01978c697102p-13);
+    p = __builtin_fma(p, r, 0x1.6c16c17f43ad0p-10);
+    p = __builtin_fma(p, r, 0x1.1111112dd687ap-7);
+    p = __builtin_fma(p, r, 0x1.55555555520afp-5);
+    p = __builtin_fma(p, r, 0x1.555555554b756p-3);
+    p = __builtin_fma(p, r, 0x1.0000000000005p-1);
+    p = __builtin_fma(p, r, 0x1.000000000001ep+0);
     p = __builtin_fma(p, r, 1.0);
 
     /* p in [0.70, 1.42]; 2^k is a normal double for |k| <= 1010, so the
@@ -165,6 +168,22 @@ CRAW_DM_FN float craw_dm_expf_nonpos(float x)
     const double xd = (double)x;
 
     return (float)craw_dm_exp_d(xd < -200.0 ? -200.0 : xd);
+}
+
+/* exp(-max(a, 0)), the weight of the denoise filters, for any float a:
+ * equal to craw_dm_expf(-(a > 0 ? a : 0)) for every a, a NaN included
+ * (it gives 1). The argument is clamped at 200 in float (exp(-200) rounds
+ * to 0 like any smaller value), which the device does with one max and
+ * one min. */
+CRAW_DM_FN float craw_dm_expf_negmax0(float a)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    const float c = __builtin_fminf(__builtin_fmaxf(a, 0.0f), 200.0f);
+#else
+    const float c = a > 0.0f ? (a < 200.0f ? a : 200.0f) : 0.0f;
+#endif
+
+    return (float)craw_dm_exp_d(-(double)c);
 }
 
 /* logf: NaN for NaN and negatives, -inf at zero, +inf at +inf. Float

@@ -77,6 +77,7 @@ struct CRAWProfRun {
     CRAWProfiler *pr;
     hipStream_t s;
     uint32_t slot, n;
+    bool kernels;               /* the open stage's events ride on its launches */
 };
 
 struct CRAWPipeline {
@@ -85,7 +86,9 @@ struct CRAWPipeline {
     CRAWStageConfig cfg;
     CRAWFrameBuffers buf;
     CRAWTemporalState ts;
-    uint32_t *hist_host[4];     /* pinned readback slots */
+    uint32_t *hist_host[4];     /* pinned readback slots (coherent) */
+    uint32_t *hist_dev[4];      /* the same, as the device addresses them */
+    bool hist_dirty;            /* a failed recording may have left counts */
     uint32_t nb_slots;
     uint64_t nb_recorded;
     CRAWProfiler prof;
@@ -118,7 +121,10 @@ void craw_temporal_release(CRAWTemporalState *ts);
 int craw_profiler_init(CRAWProfiler *pr);
 void craw_profiler_free(CRAWProfiler *pr);
 CRAWProfRun craw_profiler_start(CRAWProfiler *pr, hipStream_t s);
-void craw_profiler_begin(CRAWProfRun &r, const char *name);
+/* Opens a stage. kernels: the stage is kernel launches (KLAUNCH), timed from
+ * the first one's start to the last one's end; otherwise (a graph launch,
+ * a copy) event markers are recorded around it on the stream. */
+void craw_profiler_begin(CRAWProfRun &r, const char *name, bool kernels = true);
 void craw_profiler_end(CRAWProfRun &r);
 void craw_profiler_finish(CRAWProfRun &r);
 int craw_profiler_times(CRAWContext *ctx, CRAWProfiler *pr, uint32_t frames_back,

@@ -886,6 +886,18 @@ __global__ void k_zero_u32(uint32_t *p, size_t n)
         p[i] = 0u;
 }
 
+/* A frame's histogram to its host slot (pinned, coherent), leaving the
+ * device copy zero for the next frame: one 16-byte word a thread. */
+__global__ void k_hist_export(uint4 *__restrict__ hist, uint4 *__restrict__ host, uint32_t n4)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+
+    if (i < n4) {
+        host[i] = hist[i];
+        hist[i] = make_uint4(0u, 0u, 0u, 0u);
+    }
+}
+
 /* ---------------------------------------------------------- launchers */
 
 static_assert(BL_TW == KG_SIMPLE_TW && BL_TH == KG_SIMPLE_TH, "tile geometry");
@@ -895,7 +907,18 @@ static_assert(CU_TW == KG_CU_TW && CU_TH == KG_CU_TH, "tile geometry");
 
 hipError_t craw_launch_zero_u32(uint32_t *p, size_t n, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_zero_u32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n);
+    KLAUNCH(k_zero_u32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n);
+    return hipGetLastError();
+}
+
+hipError_t craw_launch_hist_export(uint32_t *hist, uint32_t *host, size_t n, hipStream_t s)
+{
+    const uint32_t n4 = (uint32_t)(n / 4);
+
+    if (n % 4)
+        return hipErrorInvalidValue;
+    KLAUNCH(k_hist_export, dim3((n4 + 255) / 256), dim3(256), 0, s, (uint4 *)hist,
+            (uint4 *)host, n4);
     return hipGetLastError();
 }
 
@@ -907,22 +930,22 @@ hipError_t craw_launch_debayer(bool to_inter, const KFrame &f, hipStream_t s)
         hipError_t e;
 
         if (to_inter)
-            hipLaunchKernelGGL(k_debayer_rcd<true>, dim3(nring), dim3(RC_NT), 0, s, f);
+            KLAUNCH(k_debayer_rcd<true>, dim3(nring), dim3(RC_NT), 0, s, f);
         else
-            hipLaunchKernelGGL(k_debayer_rcd<false>, dim3(nring), dim3(RC_NT), 0, s, f);
+            KLAUNCH(k_debayer_rcd<false>, dim3(nring), dim3(RC_NT), 0, s, f);
         if ((e = hipGetLastError()) != hipSuccess)
             return e;
         return craw_launch_rcd_stream(to_inter, f, s);
     } else {
         dim3 grid((f.W + BL_TW - 1) / BL_TW, (f.H + BL_TH - 1) / BL_TH);
         if (f.method == 0 && to_inter)
-            hipLaunchKernelGGL((k_debayer_simple<0, true>), grid, dim3(BL_NT), 0, s, f);
+            KLAUNCH((k_debayer_simple<0, true>), grid, dim3(BL_NT), 0, s, f);
         else if (f.method == 0)
-            hipLaunchKernelGGL((k_debayer_simple<0, false>), grid, dim3(BL_NT), 0, s, f);
+            KLAUNCH((k_debayer_simple<0, false>), grid, dim3(BL_NT), 0, s, f);
         else if (to_inter)
-            hipLaunchKernelGGL((k_debayer_simple<1, true>), grid, dim3(BL_NT), 0, s, f);
+            KLAUNCH((k_debayer_simple<1, true>), grid, dim3(BL_NT), 0, s, f);
         else
-            hipLaunchKernelGGL((k_debayer_simple<1, false>), grid, dim3(BL_NT), 0, s, f);
+            KLAUNCH((k_debayer_simple<1, false>), grid, dim3(BL_NT), 0, s, f);
     }
     return hipGetLastError();
 }
@@ -930,14 +953,14 @@ hipError_t craw_launch_debayer(bool to_inter, const KFrame &f, hipStream_t s)
 hipError_t craw_launch_chroma_blur(const KFrame &f, hipStream_t s)
 {
     dim3 grid((f.inter.wq + CB_TX - 1) / CB_TX, (f.inter.hq + CB_TY - 1) / CB_TY);
-    hipLaunchKernelGGL(k_chroma_blur, grid, dim3(CB_TX * CB_TY), 0, s, f);
+    KLAUNCH(k_chroma_blur, grid, dim3(CB_TX * CB_TY), 0, s, f);
     return hipGetLastError();
 }
 
 hipError_t craw_launch_chroma_up_finish(const KFrame &f, hipStream_t s)
 {
     dim3 grid((f.W + CU_TW - 1) / CU_TW, (f.H + CU_TH - 1) / CU_TH);
-    hipLaunchKernelGGL(k_chroma_up_finish, grid, dim3(CU_NT), 0, s, f);
+    KLAUNCH(k_chroma_up_finish, grid, dim3(CU_NT), 0, s, f);
     return hipGetLastError();
 }
 
@@ -949,24 +972,24 @@ hipError_t craw_launch_batch_debayer(int method, bool to_inter, const KFrame *fr
         return hipSuccess;
     if (method == 2) {
         if (to_inter)
-            hipLaunchKernelGGL(kb_debayer_rcd<true>, dim3(grid), dim3(RC_NT), 0, s, frames, tiles,
+            KLAUNCH(kb_debayer_rcd<true>, dim3(grid), dim3(RC_NT), 0, s, frames, tiles,
                                ntiles);
         else
-            hipLaunchKernelGGL(kb_debayer_rcd<false>, dim3(grid), dim3(RC_NT), 0, s, frames,
+            KLAUNCH(kb_debayer_rcd<false>, dim3(grid), dim3(RC_NT), 0, s, frames,
                                tiles, ntiles);
     } else if (method == 0) {
         if (to_inter)
-            hipLaunchKernelGGL((kb_debayer_simple<0, true>), dim3(grid), dim3(BL_NT), 0, s,
+            KLAUNCH((kb_debayer_simple<0, true>), dim3(grid), dim3(BL_NT), 0, s,
                                frames, tiles, ntiles);
         else
-            hipLaunchKernelGGL((kb_debayer_simple<0, false>), dim3(grid), dim3(BL_NT), 0, s,
+            KLAUNCH((kb_debayer_simple<0, false>), dim3(grid), dim3(BL_NT), 0, s,
                                frames, tiles, ntiles);
     } else {
         if (to_inter)
-            hipLaunchKernelGGL((kb_debayer_simple<1, true>), dim3(grid), dim3(BL_NT), 0, s,
+            KLAUNCH((kb_debayer_simple<1, true>), dim3(grid), dim3(BL_NT), 0, s,
                                frames, tiles, ntiles);
         else
-            hipLaunchKernelGGL((kb_debayer_simple<1, false>), dim3(grid), dim3(BL_NT), 0, s,
+            KLAUNCH((kb_debayer_simple<1, false>), dim3(grid), dim3(BL_NT), 0, s,
                                frames, tiles, ntiles);
     }
     return hipGetLastError();
@@ -977,7 +1000,7 @@ hipError_t craw_launch_batch_chroma_blur(const KFrame *frames, const uint32_t *t
 {
     if (!ntiles)
         return hipSuccess;
-    hipLaunchKernelGGL(kb_chroma_blur, dim3(grid), dim3(CB_TX * CB_TY), 0, s, frames, tiles,
+    KLAUNCH(kb_chroma_blur, dim3(grid), dim3(CB_TX * CB_TY), 0, s, frames, tiles,
                        ntiles);
     return hipGetLastError();
 }
@@ -986,7 +1009,7 @@ hipError_t craw_launch_awb_hist(const KFrame &f, hipStream_t s)
 {
     int ntx, n = craw_stage_tiles(KSTAGE_AWB, f.W, f.H, 0, 0, &ntx);
 
-    hipLaunchKernelGGL(k_awb_hist, dim3(n), dim3(AWB_NT), 0, s, f);
+    KLAUNCH(k_awb_hist, dim3(n), dim3(AWB_NT), 0, s, f);
     return hipGetLastError();
 }
 
@@ -995,7 +1018,7 @@ hipError_t craw_launch_batch_awb_hist(const KFrame *frames, const uint32_t *tile
 {
     if (!ntiles)
         return hipSuccess;
-    hipLaunchKernelGGL(kb_awb_hist, dim3(grid), dim3(AWB_NT), 0, s, frames, tiles, ntiles);
+    KLAUNCH(kb_awb_hist, dim3(grid), dim3(AWB_NT), 0, s, frames, tiles, ntiles);
     return hipGetLastError();
 }
 
@@ -1004,7 +1027,7 @@ hipError_t craw_launch_batch_chroma_up(const KFrame *frames, const uint32_t *til
 {
     if (!ntiles)
         return hipSuccess;
-    hipLaunchKernelGGL(kb_chroma_up_finish, dim3(grid), dim3(CU_NT), 0, s, frames, tiles,
+    KLAUNCH(kb_chroma_up_finish, dim3(grid), dim3(CU_NT), 0, s, frames, tiles,
                        ntiles);
     return hipGetLastError();
 }

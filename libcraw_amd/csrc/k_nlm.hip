@@ -283,7 +283,7 @@ hipError_t craw_launch_denoise(int mode, const KFrame &f, hipStream_t s)
     switch (mode) {
 #define X(M)                                                                               \
     case M:                                                                                \
-        hipLaunchKernelGGL(k_denoise<M>, grid, dim3(DN_NT), 0, s, f);                      \
+        KLAUNCH(k_denoise<M>, grid, dim3(DN_NT), 0, s, f);                      \
         break;
     DN_MODES(X)
 #undef X
@@ -303,7 +303,7 @@ hipError_t craw_launch_batch_denoise_exact(int mode, const KFrame *frames, const
     switch (mode) {
 #define X(M)                                                                               \
     case M:                                                                                \
-        hipLaunchKernelGGL(kb_denoise<M>, dim3(grid), dim3(DN_NT), 0, s, frames, tiles,     \
+        KLAUNCH(kb_denoise<M>, dim3(grid), dim3(DN_NT), 0, s, frames, tiles,     \
                            ntiles);                                                        \
         break;
     DN_MODES(X)

@@ -354,7 +354,7 @@ static hipError_t launch_fast(int mode, const KFrame &f, hipStream_t s)
     switch (mode) {
 #define X(M)                                                                               \
     case M:                                                                                \
-        hipLaunchKernelGGL((k_denoise_fast<M, FP16>), grid, dim3(FT_NT), 0, s, f);         \
+        KLAUNCH((k_denoise_fast<M, FP16>), grid, dim3(FT_NT), 0, s, f);         \
         break;
     FT_MODES(X)
 #undef X
@@ -371,7 +371,7 @@ static hipError_t launch_batch_fast(int mode, const KFrame *frames, const uint32
     switch (mode) {
 #define X(M)                                                                               \
     case M:                                                                                \
-        hipLaunchKernelGGL((kb_denoise_fast<M, FP16>), dim3(grid), dim3(FT_NT), 0, s, frames, \
+        KLAUNCH((kb_denoise_fast<M, FP16>), dim3(grid), dim3(FT_NT), 0, s, frames, \
                            tiles, ntiles);                                                 \
         break;
     FT_MODES(X)

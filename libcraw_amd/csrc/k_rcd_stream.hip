@@ -417,9 +417,9 @@ hipError_t craw_launch_rcd_stream(bool to_inter, const KFrame &f, hipStream_t s)
     if (!n)
         return hipSuccess;
     if (to_inter)
-        hipLaunchKernelGGL(k_rcd_stream<true>, dim3(n), dim3(RS_NT), 0, s, f);
+        KLAUNCH(k_rcd_stream<true>, dim3(n), dim3(RS_NT), 0, s, f);
     else
-        hipLaunchKernelGGL(k_rcd_stream<false>, dim3(n), dim3(RS_NT), 0, s, f);
+        KLAUNCH(k_rcd_stream<false>, dim3(n), dim3(RS_NT), 0, s, f);
     return hipGetLastError();
 }
 
@@ -430,10 +430,10 @@ hipError_t craw_launch_batch_rcd_stream(bool to_inter, const KFrame *frames,
     if (!ntiles)
         return hipSuccess;
     if (to_inter)
-        hipLaunchKernelGGL(kb_rcd_stream<true>, dim3(grid), dim3(RS_NT), 0, s, frames, tiles,
+        KLAUNCH(kb_rcd_stream<true>, dim3(grid), dim3(RS_NT), 0, s, frames, tiles,
                            ntiles);
     else
-        hipLaunchKernelGGL(kb_rcd_stream<false>, dim3(grid), dim3(RS_NT), 0, s, frames, tiles,
+        KLAUNCH(kb_rcd_stream<false>, dim3(grid), dim3(RS_NT), 0, s, frames, tiles,
                            ntiles);
     return hipGetLastError();
 }

@@ -12,9 +12,29 @@
 #define CRAW_KERNELS_H
 
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 #include <stdint.h>
 
 #include "k_common.h"
+
+/* Stage timing without markers between the launches. While a profiled
+ * stage is open on this thread, craw_launch_events() hands its launches
+ * the stage's events: the start event to the first launch and the stop
+ * event to every launch, so the stage runs from its first kernel's start
+ * to its last kernel's end, timestamped by the dispatches themselves
+ * (hipExtLaunchKernel) rather than by event packets between them. Both
+ * are NULL outside a profiled stage, and the launch is a plain one. */
+void craw_launch_events(hipEvent_t *start, hipEvent_t *stop);
+
+#define KLAUNCH(K, G, B, SHM, S, ...)                                                    \
+    do {                                                                                \
+        hipEvent_t kl_e0_, kl_e1_;                                                      \
+        craw_launch_events(&kl_e0_, &kl_e1_);                                           \
+        if (kl_e1_)                                                                     \
+            hipExtLaunchKernelGGL(K, G, B, SHM, S, kl_e0_, kl_e1_, 0, __VA_ARGS__);     \
+        else                                                                            \
+            hipLaunchKernelGGL(K, G, B, SHM, S, __VA_ARGS__);                           \
+    } while (0)
 
 /* The chroma stage's buffers: camera RGB as three float planes of the
  * frame, the quarter grid (ceil(W/4) x ceil(H/4)) of block means as three
@@ -92,7 +112,7 @@ struct KFrame {
 #define KG_FT_TH 64
 #define KG_AWB_CHUNK 8192
 #define KG_XS_TW 224
-#define KG_XS_TH 56
+#define KG_XS_TH 57
 #define KG_RS_OUT 116           /* streamed RCD: output columns per strip */
 #define KG_RS_OUT_F 112         /* the same without the chroma stage */
 #define KG_RS_R   64            /* and rows */
@@ -118,7 +138,7 @@ enum KStage {
 
 /* Which denoise kernel's tiles: the exact kernel of the temporal-history
  * modes (32x16), the fast forms' (28x64), the exact kernel with pairwise
- * weights of the modes without history (224x56). */
+ * weights of the modes without history (224x57). */
 enum KDnGeom {
     KDN_GEOM_DIRECT = 0,
     KDN_GEOM_FAST,
@@ -255,6 +275,9 @@ static inline int craw_stage_tiles(int stage, int W, int H, int method, int dn_g
 }
 
 hipError_t craw_launch_zero_u32(uint32_t *p, size_t n, hipStream_t s);
+/* n words (a multiple of 4, 16-byte aligned) of hist to the device-visible
+ * host buffer host, zeroing hist. */
+hipError_t craw_launch_hist_export(uint32_t *hist, uint32_t *host, size_t n, hipStream_t s);
 
 /* Plain form: one frame, `f` by value. */
 hipError_t craw_launch_debayer(bool to_inter, const KFrame &f, hipStream_t s);

@@ -154,13 +154,26 @@ CRAW_API int craw_pipeline_create(CRAWContext *ctx, const CRAWPipelineParams *pa
                            craw_awb_blocks(params->width, params->height) * sizeof(uint2))) < 0)
             goto fail;
         for (uint32_t i = 0; i < p->nb_slots; i++) {
+            /* coherent: the export kernel writes it directly */
             if (hipHostMalloc((void **)&p->hist_host[i], CRAW_HIST_WORDS * sizeof(uint32_t),
-                              hipHostMallocDefault) != hipSuccess) {
+                              hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess) {
                 p->hist_host[i] = NULL;
                 ret = CRAW_ERROR(ENOMEM);
                 goto fail;
             }
             memset(p->hist_host[i], 0, CRAW_HIST_WORDS * sizeof(uint32_t));
+            if (hipHostGetDevicePointer((void **)&p->hist_dev[i], p->hist_host[i], 0) !=
+                hipSuccess) {
+                ret = CRAW_ERROR(ENOMEM);
+                goto fail;
+            }
+        }
+        /* the device histogram starts at zero; each recording's export
+         * leaves it zero again */
+        if (hipMemset(p->buf.hist, 0, CRAW_HIST_WORDS * sizeof(uint32_t)) != hipSuccess ||
+            hipStreamSynchronize(nullptr) != hipSuccess) {
+            ret = CRAW_ERROR(EIO);
+            goto fail;
         }
     }
     if ((params->flags & CRAW_PIPELINE_FLAG_PROFILE) && (ret = craw_profiler_init(&p->prof)) < 0)
@@ -190,7 +203,9 @@ static int pipeline_enqueue(CRAWPipeline *p, const CRAWFramePlan *plan, hipStrea
     if (hipSetDevice(p->ctx->device) != hipSuccess)
         return CRAW_ERROR(ENODEV);
 
-    if (p->cfg.awb) {
+    /* after a recording that failed part way the histogram may hold
+     * counts */
+    if (p->cfg.awb && p->hist_dirty) {
         craw_profiler_begin(pr, "awb_clear");
         e = craw_launch_zero_u32(p->buf.hist, CRAW_HIST_WORDS, stream);
         craw_profiler_end(pr);
@@ -228,14 +243,17 @@ static int pipeline_enqueue(CRAWPipeline *p, const CRAWFramePlan *plan, hipStrea
         craw_profiler_end(pr);
     }
     if (e == hipSuccess && p->cfg.awb) {
-        uint32_t *slot = p->hist_host[p->nb_recorded % p->nb_slots];
-        e = hipMemcpyAsync(slot, p->buf.hist, CRAW_HIST_WORDS * sizeof(uint32_t),
-                           hipMemcpyDeviceToHost, stream);
+        craw_profiler_begin(pr, "awb_export");
+        e = craw_launch_hist_export(p->buf.hist, p->hist_dev[p->nb_recorded % p->nb_slots],
+                                    CRAW_HIST_WORDS, stream);
+        craw_profiler_end(pr);
     }
     if (e != hipSuccess) {
         craw_log(p->ctx, CRAW_LOG_ERROR, "Launch failed: %s\n", hipGetErrorString(e));
+        p->hist_dirty = true;
         return CRAW_ERROR(EIO);
     }
+    p->hist_dirty = false;
     craw_profiler_finish(pr);
     return 0;
 }

@@ -426,19 +426,50 @@ CRAWProfRun craw_profiler_start(CRAWProfiler *pr, hipStream_t s)
     return r;
 }
 
-void craw_profiler_begin(CRAWProfRun &r, const char *name)
+/* The open stage's events, handed to its kernel launches (KLAUNCH). */
+static thread_local hipEvent_t tl_start, tl_stop;
+static thread_local bool tl_used;
+
+void craw_launch_events(hipEvent_t *start, hipEvent_t *stop)
+{
+    *start = tl_start;
+    *stop = tl_stop;
+    if (tl_stop) {
+        tl_used = true;
+        tl_start = NULL;
+    }
+}
+
+void craw_profiler_begin(CRAWProfRun &r, const char *name, bool kernels)
 {
     if (!r.pr->on || r.n >= CRAW_MAX_STAGES)
         return;
     r.pr->name[r.slot][r.n] = name;
-    CRAW_IGNORE(hipEventRecord(r.pr->ev[r.slot][r.n][0], r.s));
+    if (kernels) {
+        tl_start = r.pr->ev[r.slot][r.n][0];
+        tl_stop = r.pr->ev[r.slot][r.n][1];
+        tl_used = false;
+    } else {
+        CRAW_IGNORE(hipEventRecord(r.pr->ev[r.slot][r.n][0], r.s));
+    }
+    r.kernels = kernels;
 }
 
 void craw_profiler_end(CRAWProfRun &r)
 {
     if (!r.pr->on || r.n >= CRAW_MAX_STAGES)
         return;
-    CRAW_IGNORE(hipEventRecord(r.pr->ev[r.slot][r.n][1], r.s));
+    if (r.kernels) {
+        /* a stage that launched nothing gets a zero-length interval */
+        if (!tl_used) {
+            CRAW_IGNORE(hipEventRecord(r.pr->ev[r.slot][r.n][0], r.s));
+            CRAW_IGNORE(hipEventRecord(r.pr->ev[r.slot][r.n][1], r.s));
+        }
+        tl_start = tl_stop = NULL;
+        tl_used = false;
+    } else {
+        CRAW_IGNORE(hipEventRecord(r.pr->ev[r.slot][r.n][1], r.s));
+    }
     r.n++;
 }
 
