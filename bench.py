@@ -16,7 +16,10 @@ kernel dispatches (hipExtLaunchKernel) on the same stream: `roofline` is
 the dominant kernel's algorithmic bytes per launch over its average launch
 time from those events, `kernels` the same for every kernel, and
 `ms_per_step_profiled` that run's frame time (the dispatch timestamps add
-a few microseconds per kernel). `cpu_baseline` times the oracle
+a few microseconds per kernel). `roofline.traffic` is the dominant
+kernel's HBM bytes a launch (read + write) from the committed rocprofv3
+FETCH_SIZE / WRITE_SIZE summary in profiles/ (calibrated against a device
+copy of known size). `cpu_baseline` times the oracle
 (the CPU restatement of lib/ref) on the host: single-threaded on a bounded
 sample and on all cores on the full frame, both on the same seeded scene.
 
@@ -551,7 +554,9 @@ def main():
     achieved = algo / (stage_ms[dom] / 1e3) / 1e9
     roofline = {
         "kernel": dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-        "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+        "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": measured_traffic(dom),
+        "traffic_source": os.path.relpath(COUNTERS_MD, ROOT),
         "algo_bytes_per_launch": int(algo), "avg_launch_ms": round(stage_ms[dom], 4),
     }
     if dom == "denoise_nlm":
@@ -631,6 +636,29 @@ def main():
 
 def capi_slots():
     return 4
+
+
+# The kernel behind each stage of the plain 5.8K pipeline, as rocprofv3 names it.
+STAGE_KERNEL = {
+    "denoise_nlm": "void k_denoise_sym<1>(KFrame)",
+    "chroma_up_color": "k_chroma_up_finish(KFrame)",
+    "chroma_blur": "k_chroma_blur(KFrame)",
+}
+COUNTERS_MD = os.path.join(ROOT, "profiles", "counters_5760x3240_full_pipeline.md")
+
+
+def measured_traffic(stage):
+    """HBM bytes a launch of `stage`'s kernel moved, read + write, from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (calibrated against
+    a device copy of known size); None if it has no row for the kernel."""
+    name = STAGE_KERNEL.get(stage)
+    if not name or not os.path.exists(COUNTERS_MD):
+        return None
+    for line in open(COUNTERS_MD):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) >= 3 and cells[0] == "`%s`" % name:
+            return int((float(cells[1]) + float(cells[2])) * 1e6)
+    return None
 
 
 if __name__ == "__main__":

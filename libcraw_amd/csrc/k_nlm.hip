@@ -1,8 +1,8 @@
 /*
  * libcraw for MI355X: raw-domain denoise as lib/ref/ref_denoise.c:156-336
  * of cyanreg/libcraw defines it (its DESIGN.md sections 14.3 and 14.7).
- * The modes without a temporal history use k_nlm_sym.hip's kernel, which
- * evaluates each pairwise weight once; this one serves the others.
+ * The spatial modes use k_nlm_sym.hip's kernel, which evaluates each
+ * pairwise distance once; this one serves the temporal filter alone.
  *
  * One workgroup filters a tile of one CFA plane (grid z = plane). The
  * plane, variance stabilised, is staged in LDS over the tile plus the
@@ -277,7 +277,7 @@ hipError_t craw_launch_denoise(int mode, const KFrame &f, hipStream_t s)
 {
     dim3 grid((f.dn.w2 + DN_TW - 1) / DN_TW, (f.dn.h2 + DN_TH - 1) / DN_TH, 4);
 
-    if (craw_dn_geom(KFORM_EXACT, mode) == KDN_GEOM_SYM)
+    if (craw_dn_geom(KFORM_EXACT, mode) != KDN_GEOM_DIRECT)
         return craw_launch_denoise_sym(mode, f, s);
 
     switch (mode) {
@@ -298,7 +298,7 @@ hipError_t craw_launch_batch_denoise_exact(int mode, const KFrame *frames, const
 {
     if (!ntiles)
         return hipSuccess;
-    if (craw_dn_geom(KFORM_EXACT, mode) == KDN_GEOM_SYM)
+    if (craw_dn_geom(KFORM_EXACT, mode) != KDN_GEOM_DIRECT)
         return craw_launch_batch_denoise_sym(mode, frames, tiles, ntiles, grid, s);
     switch (mode) {
 #define X(M)                                                                               \

@@ -1,32 +1,46 @@
 /*
- * libcraw for MI355X: the exact spatial NLM with each weight evaluated once
- * for the pair of samples it joins.
+ * libcraw for MI355X: the exact spatial NLM with each patch distance
+ * evaluated once for the pair of samples it joins.
  *
  * lib/ref/ref_denoise.c:114-154 of cyanreg/libcraw weighs neighbour q of
  * sample p by exp(-max(d2(p, q) / 9 / h^2 - 2, 0)), d2 being the 3x3 patch
- * distance summed in raster order. The distance is symmetric bit for bit:
- * q's terms are p's negated differences, whose squares are equal, summed
- * in the same raster order; so w(p, q) = w(q, p) exactly. This kernel
- * evaluates the weight of every pair once, for the 24 offsets o after the
- * centre in raster order (W_o(v) = w(v, v + o)), and gives each sample its
- * 48 neighbour weights in the reference's raster order: offset -o of p is
- * W_o(p - o), offset o is W_o(p). The sums are then the reference's, with
- * half the distances and half the exps.
+ * distance summed in raster order; with a temporal history by
+ * exp(-max(d2 / 9 / h^2 - (v_p + v_q), 0) / v_p), v being the variance the
+ * temporal average leaves. The distance is symmetric bit for bit: q's
+ * terms are p's negated differences, whose squares are equal, summed in
+ * the same raster order; v_p + v_q = v_q + v_p as well. So the argument
+ * A(p, q) = max(d2 / 9 / h^2 - v, 0) is one value for both samples, and
+ * without history so is the weight. This kernel evaluates A once for every
+ * pair, for the 24 offsets o after the centre in raster order (p = v,
+ * q = v + o), and gives each sample its 48 neighbour weights in the
+ * reference's raster order: offset o of p is the pair's weight for p,
+ * offset -o of q the pair's weight for q. The sums are then the
+ * reference's, with half the distances, and without history half the
+ * exps.
  *
- * Layout: a workgroup's four waves stand side by side over a 224x57 band of
- * the plane, each lane one column, and walk down it a row at a time. A lane
- * keeps, per offset, the squared differences of three rows in registers
- * and forms the weights of one row; the patch sums across columns come from
- * the neighbouring lanes through wave-wide DPP shifts, and the weight of a
- * neighbouring column through ds_bpermute. Each weight row adds its terms
- * at once to the four output rows it feeds (its own and the three below),
- * which carry their sums as running partial sums: every output's terms
- * still arrive in raster order. Of a wave's 64 columns, 62 have weights
- * and the middle 56 are outputs. The stabilised plane is staged in LDS
- * over the band plus 7 columns and 4 rows of reach.
+ * Layout: a workgroup's four waves stand side by side over a band of
+ * 224 columns of the plane (57 rows, 30 with history), each lane one
+ * column, and walk down it a row at a time. A lane keeps, per offset, the
+ * squared differences of three rows in registers and forms the weights of
+ * one row; the patch sums across columns come from the neighbouring lanes
+ * through wave-wide DPP shifts, and the weight of a neighbouring column
+ * through ds_bpermute. Each weight row adds its terms at once to the four
+ * output rows it feeds (its own and the three below), which carry their
+ * sums as running partial sums: every output's terms still arrive in
+ * raster order. Of a wave's 64 columns, 62 have weights and the middle 56
+ * are outputs. The stabilised plane (and with history the variance) is
+ * staged in LDS over the band plus 7 columns and 4 rows of reach.
  *
- * Only the modes without a temporal history use it (their weights do not
- * involve the per-sample variance, so they are symmetric).
+ * With history, the band's temporal average is formed first
+ * (lib/ref/ref_denoise.c:82-105, 268-296): this frame and the previous
+ * state, stabilised, are staged one sample further, every position of
+ * the band and its reach gets its blend factor from the 3x3 comparison at
+ * its mirrored position, its average and the variance it leaves, and the
+ * band's own samples write the state for the next frame. The lane of p
+ * forms A(p, q) and p's weight; the lane of q receives A (not the weight)
+ * and forms its own. Each division by a variance is A times the lane's
+ * double reciprocal of it (one a row), rounded once to float: the
+ * correctly rounded float quotient the reference computes (xs_quot).
  */
 #include "k_common.h"
 #include "kernels.h"
@@ -35,17 +49,50 @@
 #define XS_NT    (64 * XS_NW)
 #define XS_OUTW  56                    /* output columns per wave */
 #define XS_TW    (XS_OUTW * XS_NW)     /* 224 */
-#define XS_TH    57
 #define XS_AX    7
 #define XS_AY    4
 #define XS_LW    (XS_TW + 2 * XS_AX)   /* 238 */
-#define XS_LH    (XS_TH + 2 * XS_AY)   /* 64 */
 #define XS_NO    24                    /* offsets after the centre */
 
 #define DPP_WAVE_SHR1 0x138
 #define DPP_WAVE_SHL1 0x130
 
-static_assert(XS_TW == KG_XS_TW && XS_TH == KG_XS_TH, "tile geometry");
+/* Band rows and the LDS it stages: without history the plane; with it the
+ * plane and the variance, which share the LDS with the staged frames the
+ * temporal average is formed from. */
+template <int MODE>
+struct XsG {
+    static constexpr bool HIST = MODE & KDN_HISTORY;
+    static constexpr int TH = HIST ? KG_XH_TH : KG_XS_TH;
+    static constexpr int LH = TH + 2 * XS_AY;
+};
+static_assert(XS_LW <= XS_NT, "a thread per staged column");
+
+template <int MODE, bool HIST = XsG<MODE>::HIST>
+union SmXs;
+
+template <int MODE>
+union SmXs<MODE, false> {
+    struct {
+        float plane[XsG<MODE>::LH][XS_LW];
+    } m;
+};
+
+template <int MODE>
+union SmXs<MODE, true> {
+    struct {
+        float plane[XsG<MODE>::LH][XS_LW];
+        float var[XsG<MODE>::LH][XS_LW];
+    } m;
+    struct {
+        float cur[XsG<MODE>::LH + 2][XS_LW + 2];
+        float prev[XsG<MODE>::LH + 2][XS_LW + 2];
+    } f;
+};
+
+static_assert(XS_TW == KG_XS_TW, "tile geometry");
+static_assert((KG_XS_TH + 3) % 3 == 0 && (KG_XH_TH + 3) % 3 == 0,
+              "the row loop is unrolled by three");
 
 /* Offset o (0..23) after the centre, in raster order: dy 0, dx 1..3; then
  * dy 1..3, dx -3..3. */
@@ -85,14 +132,20 @@ KDEV float xs_igat(const KDenoise &p, float t)
     return (t - p.gat_t0) * p.gat_sc;
 }
 
-struct SmXs {
-    float plane[XS_LH][XS_LW];
-};
-
 /* The weight the reference gives a pair at patch distance d2. */
 KDEV float xs_weight(float d2, float inv_h2)
 {
     return craw_dm_expf_negmax0(d2 * (1.0f / 9.0f) * inv_h2 - 2.0f);
+}
+
+/* A / v rounded to float, for A >= 0 finite and v > 0, given r = 1 / v
+ * correctly rounded to double: the product is within 2^-52 of the
+ * quotient, and a quotient of two floats lies at least 2^-49 (relative)
+ * from any midpoint of the float grid, so rounding the product gives the
+ * correctly rounded float quotient. */
+KDEV float xs_quot(float a, double r)
+{
+    return (float)((double)a * r);
 }
 
 /* A lane's running state down its column. The squared differences of
@@ -103,28 +156,37 @@ KDEV float xs_weight(float d2, float inv_h2)
  * in the reference's raster order. */
 struct XsLane {
     float a[4], ws[4];          /* acc and wsum of output rows r .. r + 3 */
+    double rv[4];               /* with history: 1 / v of rows r .. r + 3 */
 };
 
 template <int MODE>
 struct XsBand {
-    SmXs &sm;
+    static constexpr bool HIST = XsG<MODE>::HIST;
+    SmXs<MODE> &sm;
     const KFrame &f;
     int lane, lc, x, y0, px, py;
     bool out_lane;
     float black, range, inv_range, inv_h2;
 
     /* Weight row r (LDS row r): fills sc with the squared differences of
-     * row r + 1, forms the weights W_o(x, r) from rows r - 1 (sa), r (sb) and
-     * r + 1, adds every term they give to the four partial sums, and
-     * completes output row r. */
+     * row r + 1, forms the pair weights of row r from the squared
+     * differences of rows r - 1 (sa), r (sb) and r + 1, adds every term
+     * they give to the four partial sums, and completes output row r. */
     __attribute__((always_inline)) KDEV void step(float (&sa)[XS_NO], float (&sb)[XS_NO],
                                                   float (&sc)[XS_NO], int r, XsLane &L)
     {
-        float w[XS_NO];
+        auto &plane = sm.m.plane;
+        /* Without history, w[o]: the weight of the pair (x, x + o) in both
+         * sums. With it, w[o] is the pair's weight in this lane's sum and
+         * A[o] the pair's argument, from which the lane of x + o forms the
+         * weight in its own sum with its own variance. */
+        float w[XS_NO], A[XS_NO];
 
+        if constexpr (HIST)
+            L.rv[3] = 1.0 / (double)sm.m.var[r + 3][lc];
 #pragma unroll
         for (int o = 0; o < XS_NO; o++) {
-            const float e = sm.plane[r + 1][lc] - sm.plane[r + 1 + xs_dy(o)][lc + xs_dx(o)];
+            const float e = plane[r + 1][lc] - plane[r + 1 + xs_dy(o)][lc + xs_dx(o)];
             float d2;
 
             sc[o] = e * e;
@@ -138,12 +200,26 @@ struct XsBand {
             d2 = d2 + xs_shr(sc[o]);
             d2 = d2 + sc[o];
             d2 = d2 + xs_shl(sc[o]);
-            w[o] = xs_weight(d2, inv_h2);
+            if constexpr (HIST) {
+                /* lib/ref/ref_denoise.c:138-145 with v2 = v_p + v_q; the
+                 * argument clamped below at 0 (a NaN to 0, as the
+                 * reference's comparison does) and above at FLT_MAX,
+                 * where either quotient still gives the weight 0 */
+                const float vp = sm.m.var[r][lc];
+                const float vq = sm.m.var[r + xs_dy(o)][lc + xs_dx(o)];
+                const float arg = d2 * (1.0f / 9.0f) * inv_h2 - (vp + vq);
+
+                A[o] = __builtin_fminf(__builtin_fmaxf(arg, 0.0f), 3.40282347e38f);
+                w[o] = craw_dm_expf_negmax0(xs_quot(A[o], L.rv[0]));
+            } else {
+                w[o] = xs_weight(d2, inv_h2);
+            }
         }
 
         /* Row r is neighbour row ndy of output row r - ndy. Term (ndx, ndy)
-         * of an output at column x is weight W_o(x + ndx, r) with o =
-         * (-ndx, -ndy), fetched from lane + ndx. */
+         * of an output at column X is the weight of the pair (X + ndx, X)
+         * in X's sum, formed from what lane X + ndx holds for o = (-ndx,
+         * -ndy). */
         L.a[3] = 0.0f;
         L.ws[3] = 0.0f;
 #pragma unroll
@@ -153,25 +229,28 @@ struct XsBand {
                 if (k == 0 && ndx >= 0)
                     continue;
                 const int o = xs_index(-ndx, k);
-                const float wv = ndx == 0 ? w[o] :
+                const float h = HIST ? A[o] : w[o];
+                const float hv = ndx == 0 ? h :
                     __builtin_bit_cast(float, __builtin_amdgcn_ds_bpermute(
-                                                  (lane + ndx) << 2, __builtin_bit_cast(int, w[o])));
-                const float q = sm.plane[r][lc + ndx];
+                                                  (lane + ndx) << 2, __builtin_bit_cast(int, h)));
+                const float wv = HIST ? craw_dm_expf_negmax0(xs_quot(hv, L.rv[k])) : hv;
+                const float q = plane[r][lc + ndx];
 
                 L.ws[k] += wv;
                 L.a[k] += wv * q;
             }
         }
-        /* output row r: the centre (distance 0, weight exp(-0) = 1), then
-         * the offsets after it with this lane's own weights */
+        /* output row r: the centre (distance 0: the argument is at most 0,
+         * the weight exp(-0) = 1), then the offsets after it with this
+         * lane's own weights */
         {
-            const float wv = xs_weight(0.0f, inv_h2);
+            const float wv = HIST ? 1.0f : xs_weight(0.0f, inv_h2);
             L.ws[0] += wv;
-            L.a[0] += wv * sm.plane[r][lc];
+            L.a[0] += wv * plane[r][lc];
         }
 #pragma unroll
         for (int o = 0; o < XS_NO; o++) {
-            const float q = sm.plane[r + xs_dy(o)][lc + xs_dx(o)];
+            const float q = plane[r + xs_dy(o)][lc + xs_dx(o)];
 
             L.ws[0] += w[o];
             L.a[0] += w[o] * q;
@@ -205,15 +284,35 @@ struct XsBand {
         for (int k = 0; k < 3; k++) {
             L.a[k] = L.a[k + 1];
             L.ws[k] = L.ws[k + 1];
+            if constexpr (HIST)
+                L.rv[k] = L.rv[k + 1];
         }
     }
 };
 
-template <int MODE>
-KDEV void dev_denoise_sym(SmXs &sm, const KFrame &f, int x0, int y0, int pl)
+/* The variance's state encoding: its square root in eight bits
+ * (lib/ref/ref_denoise.c:100-105). */
+KDEV uint8_t xs_var_code(float v)
 {
+    v = v > 0.0f ? v : 0.0f;
+    v = v < 1.0f ? v : 1.0f;
+    return (uint8_t)rintf(sqrtf(v) * 255.0f);
+}
+
+KDEV float xs_var_decode(uint8_t c)
+{
+    float q = (float)c * (1.0f / 255.0f);
+    return q * q;
+}
+
+template <int MODE>
+KDEV void dev_denoise_sym(SmXs<MODE> &sm, const KFrame &f, int x0, int y0, int pl)
+{
+    using G = XsG<MODE>;
     constexpr bool TEMPORAL = MODE & KDN_TEMPORAL;
-    static_assert((MODE & KDN_SPATIAL) && !(MODE & KDN_HISTORY), "symmetric weights only");
+    constexpr bool HIST = G::HIST;
+    constexpr int TH = G::TH, LH = G::LH;
+    static_assert(MODE & KDN_SPATIAL, "the spatial modes");
     const KDenoise &p = f.dn;
     const KDenoiseState &st = f.st;
     const uint16_t *__restrict__ src = f.dn_src;
@@ -226,28 +325,101 @@ KDEV void dev_denoise_sym(SmXs &sm, const KFrame &f, int x0, int y0, int pl)
                                   p.inv_range[3]);
     const size_t psize = (size_t)w2 * h2;
     const bool interior = x0 >= XS_AX && y0 >= XS_AY && x0 + XS_TW + XS_AX <= w2 &&
-                          y0 + XS_TH + XS_AY <= h2;
+                          y0 + TH + XS_AY <= h2;
 
-    /* The stabilised plane over the band and its reach, through the plane
-     * mirror; a temporal pass without history also starts the state of
-     * the tile's own samples from this frame. */
-    for (int i = tid; i < XS_LW * XS_LH; i += XS_NT) {
-        const int lx = i % XS_LW, ly = i / XS_LW;
-        const int vx = x0 - XS_AX + lx, vy = y0 - XS_AY + ly;
-        const int mx = interior ? vx : kmirror(vx, w2), my = interior ? vy : kmirror(vy, h2);
-        const float raw = (float)src[(size_t)(2 * my + py) * p.src_pitch + 2 * mx + px] *
-                          (1.0f / 65535.0f);
-        const float t = xs_gat(p, (raw - black) * inv_range);
+#define RAW(X, Y) ((float)src[(size_t)(2 * (Y) + py) * p.src_pitch + 2 * (X) + px] *          \
+                   (1.0f / 65535.0f))
 
-        sm.plane[ly][lx] = t;
-        if (TEMPORAL && vx == mx && vy == my && vx >= x0 && vx < x0 + XS_TW && vy >= y0 &&
-            vy < y0 + XS_TH) {
-            const size_t o = pl * psize + (size_t)vy * w2 + vx;
-            st.next_level[o] = xs_igat(p, t);
-            st.next_var[o] = 255;
+    if constexpr (HIST) {
+        /* This frame and the previous state, stabilised, one sample beyond
+         * the band's reach; staged value i of a row is position
+         * x0 - 8 + i through the plane mirror. */
+        constexpr int SW = XS_LW + 2, SH = LH + 2;
+
+        for (int i = tid; i < SW * SH; i += XS_NT) {
+            const int lx = i % SW, ly = i / SW;
+            const int mx = kmirror(x0 - XS_AX - 1 + lx, w2), my = kmirror(y0 - XS_AY - 1 + ly, h2);
+
+            sm.f.cur[ly][lx] = xs_gat(p, (RAW(mx, my) - black) * inv_range);
+            sm.f.prev[ly][lx] = xs_gat(p, st.prev_level[pl * psize + (size_t)my * w2 + mx]);
+        }
+        __syncthreads();
+
+        /* The temporal average and the variance it leaves at every
+         * position of the band and its reach (lib/ref/ref_denoise.c:82-105,
+         * 268-296), each from the 3x3 comparison at its mirrored position;
+         * thread lx takes column lx, down the rows. A position more than 3
+         * columns right of the plane or 3 rows below it reaches no output:
+         * it is formed at the nearest that does, which keeps every staged
+         * read inside the staged window. */
+        float tv[LH], vv[LH];
+        const int lx = tid;
+        const int vx = x0 - XS_AX + lx;
+        const int mx = kmirror(vx < w2 + 3 ? vx : w2 + 3, w2);
+        /* (mx, my) in the staged window */
+        const int cx = mx - (x0 - XS_AX - 1);
+
+#pragma unroll
+        for (int ly = 0; ly < LH; ly++) {
+            const int vy = y0 - XS_AY + ly;
+            const int my = kmirror(vy < h2 + 3 ? vy : h2 + 3, h2);
+            const int cy = my - (y0 - XS_AY - 1);
+
+            tv[ly] = vv[ly] = 0.0f;
+            if (lx < XS_LW) {
+                float d2 = 0.0f, a, t, v;
+
+                for (int ky = -1; ky <= 1; ky++) {
+#pragma unroll
+                    for (int kx = -1; kx <= 1; kx++) {
+                        const float e = sm.f.prev[cy + ky][cx + kx] - sm.f.cur[cy + ky][cx + kx];
+                        d2 += e * e;
+                    }
+                }
+                a = p.alpha_max * craw_dm_expf_negmax0(d2 * (1.0f / 9.0f) * p.inv_h2 - 2.0f);
+                t = sm.f.cur[cy][cx] * (1.0f - a) + sm.f.prev[cy][cx] * a;
+                v = (1.0f - a) * (1.0f - a) +
+                    a * a * xs_var_decode(st.prev_var[pl * psize + (size_t)my * w2 + mx]);
+                v = v > 1.0e-4f ? v : 1.0e-4f;
+                tv[ly] = t;
+                vv[ly] = v;
+                if (vx == mx && vy == my && vx >= x0 && vx < x0 + XS_TW && vy >= y0 &&
+                    vy < y0 + TH) {
+                    const size_t o = pl * psize + (size_t)vy * w2 + vx;
+                    st.next_level[o] = xs_igat(p, t);
+                    st.next_var[o] = xs_var_code(v);
+                }
+            }
+        }
+        __syncthreads();
+        if (lx < XS_LW) {
+#pragma unroll
+            for (int ly = 0; ly < LH; ly++) {
+                sm.m.plane[ly][lx] = tv[ly];
+                sm.m.var[ly][lx] = vv[ly];
+            }
+        }
+    } else {
+        /* The stabilised plane over the band and its reach, through the
+         * plane mirror; a temporal pass without history also starts the
+         * state of the band's own samples from this frame. */
+        for (int i = tid; i < XS_LW * LH; i += XS_NT) {
+            const int lx = i % XS_LW, ly = i / XS_LW;
+            const int vx = x0 - XS_AX + lx, vy = y0 - XS_AY + ly;
+            const int mx = interior ? vx : kmirror(vx, w2), my = interior ? vy : kmirror(vy, h2);
+            const float t = xs_gat(p, (RAW(mx, my) - black) * inv_range);
+
+            sm.m.plane[ly][lx] = t;
+            if (TEMPORAL && vx == mx && vy == my && vx >= x0 && vx < x0 + XS_TW && vy >= y0 &&
+                vy < y0 + TH) {
+                const size_t o = pl * psize + (size_t)vy * w2 + vx;
+                st.next_level[o] = xs_igat(p, t);
+                st.next_var[o] = 255;
+            }
         }
     }
     __syncthreads();
+#undef RAW
 
     const int lane = tid & 63, wave = tid >> 6;
     /* LDS column of this lane; plane column x0 - 4 + 56 wave + lane */
@@ -263,20 +435,28 @@ KDEV void dev_denoise_sym(SmXs &sm, const KFrame &f, int x0, int y0, int pl)
      * (LDS row 1, plane row y0 - 3) */
 #pragma unroll
     for (int o = 0; o < XS_NO; o++) {
-        float e = sm.plane[0][lc] - sm.plane[xs_dy(o)][lc + xs_dx(o)];
+        float e = sm.m.plane[0][lc] - sm.m.plane[xs_dy(o)][lc + xs_dx(o)];
         s0[o] = e * e;
-        e = sm.plane[1][lc] - sm.plane[1 + xs_dy(o)][lc + xs_dx(o)];
+        e = sm.m.plane[1][lc] - sm.m.plane[1 + xs_dy(o)][lc + xs_dx(o)];
         s1[o] = e * e;
     }
 #pragma unroll
-    for (int k = 0; k < 4; k++)
+    for (int k = 0; k < 4; k++) {
         L.a[k] = L.ws[k] = 0.0f;
+        L.rv[k] = 0.0;
+    }
+    /* the reciprocal variances of rows 1 .. 3, before the first weight row
+     * forms that of row 4 */
+    if constexpr (HIST) {
+#pragma unroll
+        for (int k = 0; k < 3; k++)
+            L.rv[k] = 1.0 / (double)sm.m.var[1 + k][lc];
+    }
 
-    /* weight rows 1 .. XS_TH + 3: the rows of the outputs and the three
-     * above them */
-    static_assert((XS_TH + 3) % 3 == 0, "the row loop is unrolled by three");
+    /* weight rows 1 .. TH + 3: the rows of the outputs and the three above
+     * them */
 #pragma unroll 1
-    for (int r = 1; r <= XS_TH + 3; r += 3) {
+    for (int r = 1; r <= TH + 3; r += 3) {
         band.step(s0, s1, s2, r, L);
         band.step(s1, s2, s0, r + 1, L);
         band.step(s2, s0, s1, r + 2, L);
@@ -286,8 +466,8 @@ KDEV void dev_denoise_sym(SmXs &sm, const KFrame &f, int x0, int y0, int pl)
 template <int MODE>
 __global__ __launch_bounds__(XS_NT) void k_denoise_sym(KFrame f)
 {
-    __shared__ SmXs sm;
-    dev_denoise_sym<MODE>(sm, f, blockIdx.x * XS_TW, blockIdx.y * XS_TH, blockIdx.z);
+    __shared__ SmXs<MODE> sm;
+    dev_denoise_sym<MODE>(sm, f, blockIdx.x * XS_TW, blockIdx.y * XsG<MODE>::TH, blockIdx.z);
 }
 
 template <int MODE>
@@ -295,31 +475,35 @@ __global__ __launch_bounds__(XS_NT) void kb_denoise_sym(const KFrame *__restrict
                                                         const uint32_t *__restrict__ tiles,
                                                         uint32_t ntiles)
 {
-    __shared__ SmXs sm;
+    constexpr int TH = XsG<MODE>::TH;
+    __shared__ SmXs<MODE> sm;
 
     for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const uint32_t d = tiles[t];
         const KFrame &f = frames[KTILE_FRAME(d)];
-        const int ntx = (f.dn.w2 + XS_TW - 1) / XS_TW, nty = (f.dn.h2 + XS_TH - 1) / XS_TH;
+        const int ntx = (f.dn.w2 + XS_TW - 1) / XS_TW, nty = (f.dn.h2 + TH - 1) / TH;
         const int i = KTILE_IDX(d), pl = i / (ntx * nty), r = i % (ntx * nty);
 
         __syncthreads();
-        dev_denoise_sym<MODE>(sm, f, (r % ntx) * XS_TW, (r / ntx) * XS_TH, pl);
+        dev_denoise_sym<MODE>(sm, f, (r % ntx) * XS_TW, (r / ntx) * TH, pl);
     }
 }
 
+#define XS_MODES(X)                                                                        \
+    X(KDN_SPATIAL) X(KDN_SPATIAL | KDN_TEMPORAL) X(KDN_SPATIAL | KDN_TEMPORAL | KDN_HISTORY)
+
 hipError_t craw_launch_denoise_sym(int mode, const KFrame &f, hipStream_t s)
 {
-    dim3 grid((f.dn.w2 + XS_TW - 1) / XS_TW, (f.dn.h2 + XS_TH - 1) / XS_TH, 4);
-
     switch (mode) {
-    case KDN_SPATIAL:
-        KLAUNCH(k_denoise_sym<KDN_SPATIAL>, grid, dim3(XS_NT), 0, s, f);
-        break;
-    case KDN_SPATIAL | KDN_TEMPORAL:
-        KLAUNCH(k_denoise_sym<KDN_SPATIAL | KDN_TEMPORAL>, grid, dim3(XS_NT), 0, s,
-                           f);
-        break;
+#define X(M)                                                                               \
+    case M: {                                                                              \
+        const dim3 grid((f.dn.w2 + XS_TW - 1) / XS_TW, (f.dn.h2 + XsG<M>::TH - 1) / XsG<M>::TH, \
+                        4);                                                                \
+        KLAUNCH(k_denoise_sym<M>, grid, dim3(XS_NT), 0, s, f);                             \
+        break;                                                                             \
+    }
+    XS_MODES(X)
+#undef X
     default:
         return hipErrorInvalidValue;
     }
@@ -332,14 +516,12 @@ hipError_t craw_launch_batch_denoise_sym(int mode, const KFrame *frames, const u
     if (!ntiles)
         return hipSuccess;
     switch (mode) {
-    case KDN_SPATIAL:
-        KLAUNCH(kb_denoise_sym<KDN_SPATIAL>, dim3(grid), dim3(XS_NT), 0, s, frames,
-                           tiles, ntiles);
+#define X(M)                                                                               \
+    case M:                                                                                \
+        KLAUNCH(kb_denoise_sym<M>, dim3(grid), dim3(XS_NT), 0, s, frames, tiles, ntiles);  \
         break;
-    case KDN_SPATIAL | KDN_TEMPORAL:
-        KLAUNCH(kb_denoise_sym<KDN_SPATIAL | KDN_TEMPORAL>, dim3(grid), dim3(XS_NT),
-                           0, s, frames, tiles, ntiles);
-        break;
+    XS_MODES(X)
+#undef X
     default:
         return hipErrorInvalidValue;
     }

@@ -113,6 +113,7 @@ struct KFrame {
 #define KG_AWB_CHUNK 8192
 #define KG_XS_TW 224
 #define KG_XS_TH 57
+#define KG_XH_TH 30             /* the same kernel with a temporal history */
 #define KG_RS_OUT 116           /* streamed RCD: output columns per strip */
 #define KG_RS_OUT_F 112         /* the same without the chroma stage */
 #define KG_RS_R   64            /* and rows */
@@ -136,20 +137,24 @@ enum KStage {
 #define KFORM_FAST  1
 #define KFORM_FP16  2
 
-/* Which denoise kernel's tiles: the exact kernel of the temporal-history
- * modes (32x16), the fast forms' (28x64), the exact kernel with pairwise
- * weights of the modes without history (224x57). */
+/* Which denoise kernel's tiles: the exact tiled kernel of the temporal
+ * filter alone (32x16), the fast forms' (28x64), the exact kernel with
+ * pairwise distances for the spatial modes, without (224x57) and with
+ * (224x30) a temporal history. */
 enum KDnGeom {
     KDN_GEOM_DIRECT = 0,
     KDN_GEOM_FAST,
     KDN_GEOM_SYM,
+    KDN_GEOM_SYMH,
 };
 
 static inline int craw_dn_geom(int form, int mode)
 {
     if (form != KFORM_EXACT)
         return KDN_GEOM_FAST;
-    return ((mode & KDN_SPATIAL) && !(mode & KDN_HISTORY)) ? KDN_GEOM_SYM : KDN_GEOM_DIRECT;
+    if (!(mode & KDN_SPATIAL))
+        return KDN_GEOM_DIRECT;
+    return (mode & KDN_HISTORY) ? KDN_GEOM_SYMH : KDN_GEOM_SYM;
 }
 
 /* Tile geometry of `stage`: tw x th samples of a grid `div` times coarser
@@ -162,8 +167,23 @@ static inline void craw_stage_geom(int stage, int method, int dn_geom, int *tw, 
     *tw = *th = *div = *planes = 1;
     switch (stage) {
     case KSTAGE_DENOISE:
-        *tw = dn_geom == KDN_GEOM_FAST ? KG_FT_TW : (dn_geom == KDN_GEOM_SYM ? KG_XS_TW : KG_DN_TW);
-        *th = dn_geom == KDN_GEOM_FAST ? KG_FT_TH : (dn_geom == KDN_GEOM_SYM ? KG_XS_TH : KG_DN_TH);
+        switch (dn_geom) {
+        case KDN_GEOM_FAST:
+            *tw = KG_FT_TW;
+            *th = KG_FT_TH;
+            break;
+        case KDN_GEOM_SYM:
+            *tw = KG_XS_TW;
+            *th = KG_XS_TH;
+            break;
+        case KDN_GEOM_SYMH:
+            *tw = KG_XS_TW;
+            *th = KG_XH_TH;
+            break;
+        default:
+            *tw = KG_DN_TW;
+            *th = KG_DN_TH;
+        }
         *div = 2;
         *planes = 4;
         break;

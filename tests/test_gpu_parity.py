@@ -142,6 +142,29 @@ def test_temporal_sequence(gpu_available):
         state = ref["state"]
 
 
+@pytest.mark.parametrize("size", [(912, 272), (1000, 300), (458, 140), (896, 240), (40, 520)])
+def test_temporal_history_bands(gpu_available, size):
+    """The exact NLM with a temporal history over several bands: interior
+    bands away from every border, partial last bands across and down,
+    planes a whole number of bands, and a plane narrower than a band's
+    reach. Three frames of a moving clip (the first without history), each
+    equal to the oracle with the state threaded through."""
+    w, h = size
+    flags = (capi.CRAW_PIPELINE_FLAG_DENOISE | capi.CRAW_PIPELINE_FLAG_TEMPORAL_DENOISE)
+    run = Runner(w, h, debayer=capi.CRAW_DEBAYER_BILINEAR, flags=flags)
+    prof = scene.make_profile()
+    state = None
+    for i in range(3):
+        bayer = scene.clip_frame(w, h, i, square=min(64, w // 2, h // 2), speed=6)
+        fp = frame_params(prof, denoise_strength=0.8 + 0.2 * i, denoise_temporal_strength=0.7,
+                          noise_override=(4e-4, 1.6e-5))
+        assert run.run(bayer, fp) == 0
+        ref = oracle.f32d.process(bayer, run.pp, fp, prev=state, want_state=True)
+        assert ref["ret"] == 0
+        assert_bit_exact(run.output(), ref["out"], "history %dx%d frame %d" % (w, h, i))
+        state = ref["state"]
+
+
 def test_full_path(gpu_available):
     """Config 2's stages at a small size: temporal first frame + NLM, RCD,
     chroma 0.5, AWB, ACEScg with gamut compression."""
