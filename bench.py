@@ -82,8 +82,10 @@ NLM_OPS_PER_SAMPLE = 24 * (2 + 8 + 3 + 1 + 18) + 48 * 3 + 3
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    # the GPU's clocks ramp up over the first ~25 ms of sustained work: the
+    # default warm-up covers that, so the timed frames run at steady state
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--plain-only", action="store_true")
@@ -540,7 +542,7 @@ def main():
     # a few microseconds each, so the throughput above is measured
     # without them).
     pipe_prof = make(capi.CRAW_PIPELINE_FLAG_PROFILE)
-    gpu_ms_prof, _, _ = timed(pipe_prof, args.steps, 2)
+    gpu_ms_prof, _, _ = timed(pipe_prof, args.steps, min(args.warmup, 20))
     stages = {}
     for back in range(min(args.steps, capi_slots())):
         for name, count, ms in pipe_prof.stage_times(back):

@@ -2,8 +2,8 @@
  * libcraw for MI355X: the batched form and per-stream temporal state.
  *
  * This library's own addition, with no counterpart in cyanreg/libcraw,
- * whose recordings develop one frame each (DESIGN.md:991-993 of the
- * reference). A batch develops many frames of possibly different sizes in
+ * whose recordings develop one frame each (include/craw/pipeline.h:450-455
+ * of the reference). A batch develops many frames of possibly different sizes in
  * one recording: every stage is one persistent launch over a work-list of
  * tiles drawn from all the frames, so small frames fill the gaps large
  * ones leave and a stage's launch cost is paid once per batch. Each frame

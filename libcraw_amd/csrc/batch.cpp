@@ -296,7 +296,6 @@ struct BatchPlan {
 static int batch_plan(CRAWBatch *b, const CRAWBatchFrame *frames, uint32_t nb, BatchPlan &bp)
 {
     size_t off_px = 0, off_q = 0, off_awb = 0;
-    bool fast = b->form != KFORM_EXACT;
 
     if (!nb || nb > b->params.max_frames)
         return CRAW_ERROR(ENOSPC);

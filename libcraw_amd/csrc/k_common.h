@@ -48,7 +48,7 @@ KDEV float ksel4(int s, float a0, float a1, float a2, float a3)
 }
 
 /* Black/white normalisation, white balance and clip of one sample,
- * DESIGN 4.1 as lib/ref/ref_debayer.c:42-63 computes it: divide by the
+ * as lib/ref/ref_debayer.c:42-63 computes it: divide by the
  * range (white - black, formed on the host in float, exactly as the
  * reference forms it), multiply by the channel's gain, clip at 1.0;
  * below black is kept. */

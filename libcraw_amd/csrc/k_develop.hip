@@ -256,7 +256,7 @@ KDEV float rcd_ratio(float a, float b)
     return (a - b) / (a + b + RCD_EPS);
 }
 
-/* DESIGN 5.3 / lib/ref/ref_debayer.c:204-395. Stages and the apron each
+/* lib/ref/ref_debayer.c:204-395. Stages and the apron each
  * is needed over: V 5, low pass 4, gradients 3, wv 2, wp 1, green 2,
  * colour estimate 1, output 0. Every stage computes at the mirrored
  * position of each apron position, from the previous stage's values at
@@ -789,7 +789,7 @@ __global__ __launch_bounds__(RC_NT) void k_debayer_rcd(KFrame f)
 
     /* the ring around the streamed interior, blockIdx.x a ring tile */
     int tx, ty;
-    craw_rcd_ring_tile(craw_rcd_split(f.W, f.H), blockIdx.x, &tx, &ty);
+    craw_ring_tile(craw_rcd_split(f.W, f.H), blockIdx.x, &tx, &ty);
     dev_debayer_rcd_tile<INTER>(sm, f, tx * RC_TW, ty * RC_TH);
 }
 
@@ -841,7 +841,7 @@ __global__ __launch_bounds__(RC_NT) void kb_debayer_rcd(const KFrame *__restrict
 
         /* entries are ring tiles */
         (void)ntx;
-        craw_rcd_ring_tile(craw_rcd_split(f.W, f.H), i, &tx, &ty);
+        craw_ring_tile(craw_rcd_split(f.W, f.H), i, &tx, &ty);
         __syncthreads();
         dev_debayer_rcd_tile<INTER>(sm, f, tx * RC_TW, ty * RC_TH);
     }

@@ -1,20 +1,15 @@
 /*
- * libcraw for MI355X: raw-domain denoise as lib/ref/ref_denoise.c:156-336
- * of cyanreg/libcraw defines it (its DESIGN.md sections 14.3 and 14.7).
- * The spatial modes use k_nlm_sym.hip's kernel, which evaluates each
- * pairwise distance once; this one serves the temporal filter alone.
+ * libcraw for MI355X: the temporal filter of the raw-domain denoise alone
+ * (no spatial NLM), as lib/ref/ref_denoise.c:156-336 of cyanreg/libcraw
+ * defines it; the spatial modes are k_nlm_sym.hip's.
  *
- * One workgroup filters a tile of one CFA plane (grid z = plane). The
- * plane, variance stabilised, is staged in LDS over the tile plus the
- * 7x7 search and 3x3 patch reach (4 samples), loaded through the plane's
- * own mirror. With a temporal history the tile's current and previous
- * stabilised planes are staged one sample further (the 3x3 comparison),
- * averaged into the LDS plane, and the averaged state of the tile's own
- * samples is written for the next frame.
- *
- * The FP32 form evaluates every patch distance and weight exactly as the
- * CPU reference does (raster order, same operations, deterministic exp),
- * so its 16-bit output is the reference's bit for bit.
+ * One workgroup filters a 32x16 tile of one CFA plane (grid z = plane).
+ * Without history the tile's stabilised samples are the output and start
+ * the state. With a history the tile's current and previous stabilised
+ * planes are staged in LDS one sample beyond it (the 3x3 comparison at the
+ * mirrored position of each sample), averaged, and the average's state is
+ * written for the next frame. Every operation is the reference's, so the
+ * 16-bit output is bit for bit.
  */
 #include "k_common.h"
 #include "kernels.h"
@@ -64,7 +59,6 @@ struct SmDn {
     static constexpr int PW = DN_TW + 2 * DN_R, PH = DN_TH + 2 * DN_R;
     static constexpr int CW = PW + 2, CH = PH + 2;
     float splane[PH * PW];
-    float svar[HISTORY ? PH * PW : 1];
     float scur[HISTORY ? CH * CW : 1];
     float stp[HISTORY ? CH * CW : 1];
 };
@@ -72,12 +66,12 @@ struct SmDn {
 template <int MODE>
 KDEV void dev_denoise(SmDn<MODE> &sm, const KFrame &f, int x0, int y0, int pl)
 {
-    constexpr bool SPATIAL = MODE & KDN_SPATIAL;
     constexpr bool TEMPORAL = MODE & KDN_TEMPORAL;
+    static_assert(!(MODE & KDN_SPATIAL), "the spatial modes are k_nlm_sym.hip's");
     constexpr bool HISTORY = MODE & KDN_HISTORY;
     constexpr int PW = SmDn<MODE>::PW;
     constexpr int CW = SmDn<MODE>::CW;
-    float *splane = sm.splane, *svar = sm.svar, *scur = sm.scur, *stp = sm.stp;
+    float *splane = sm.splane, *scur = sm.scur, *stp = sm.stp;
     const KDenoise &p = f.dn;
     const KDenoiseState &st = f.st;
     const uint16_t *__restrict__ src = f.dn_src;
@@ -110,8 +104,8 @@ KDEV void dev_denoise(SmDn<MODE> &sm, const KFrame &f, int x0, int y0, int pl)
         }
         __syncthreads();
 
-        /* The average and the variance it leaves, DESIGN 14.7, at the
-         * mirrored position of every position the NLM reads. */
+        /* The average and the variance it leaves (lib/ref/ref_denoise.c:82-105)
+         * at the mirrored position of every position the filter reads. */
         {
             const int qw = tw + 2 * DN_R, qh = th + 2 * DN_R;
 
@@ -139,7 +133,6 @@ KDEV void dev_denoise(SmDn<MODE> &sm, const KFrame &f, int x0, int y0, int pl)
                 v = (1.0f - a) * (1.0f - a) + a * a * pv;
                 v = v > 1.0e-4f ? v : 1.0e-4f;
                 splane[ly * PW + lx] = t;
-                svar[ly * PW + lx] = v;
                 if (vx == mx && vy == my && vx >= x0 && vx < xe && vy >= y0 && vy < ye) {
                     size_t o = pl * psize + (size_t)vy * w2 + vx;
                     st.next_level[o] = kigat(p, t);
@@ -181,49 +174,8 @@ KDEV void dev_denoise(SmDn<MODE> &sm, const KFrame &f, int x0, int y0, int pl)
         if (x >= w2 || y >= h2)
             continue;
 
-        if (SPATIAL) {
-            /* lib/ref/ref_denoise.c:114-154: 7x7 search, 3x3 patch, all
-             * sums in raster order. */
-            float cp[9], acc = 0.0f, wsum = 0.0f, vc = 0.0f;
-
-#pragma unroll
-            for (int ky = -1; ky <= 1; ky++)
-#pragma unroll
-                for (int kx = -1; kx <= 1; kx++)
-                    cp[(ky + 1) * 3 + kx + 1] = splane[c + ky * PW + kx];
-            if (HISTORY)
-                vc = svar[c];
-
-            for (int dy = -3; dy <= 3; dy++) {
-#pragma unroll
-                for (int dx = -3; dx <= 3; dx++) {
-                    const int n = c + dy * PW + dx;
-                    float d2 = 0.0f, wgt, arg;
-
-#pragma unroll
-                    for (int ky = -1; ky <= 1; ky++) {
-#pragma unroll
-                        for (int kx = -1; kx <= 1; kx++) {
-                            float e = cp[(ky + 1) * 3 + kx + 1] - splane[n + ky * PW + kx];
-                            d2 += e * e;
-                        }
-                    }
-                    if (HISTORY) {
-                        float v2 = vc + svar[n];
-                        arg = d2 * (1.0f / 9.0f) * p.inv_h2 - v2;
-                        wgt = craw_dm_expf_nonpos(-(arg > 0.0f ? arg : 0.0f) / vc);
-                    } else {
-                        arg = d2 * (1.0f / 9.0f) * p.inv_h2 - 2.0f;
-                        wgt = craw_dm_expf_nonpos(-(arg > 0.0f ? arg : 0.0f));
-                    }
-                    wsum += wgt;
-                    acc += wgt * splane[n];
-                }
-            }
-            u = kigat(p, acc / wsum);
-        } else {
-            u = kigat(p, splane[c]);
-        }
+        /* the temporal average alone */
+        u = kigat(p, splane[c]);
 
         /* Highlight guard and the 16-bit re-encode
          * (lib/ref/ref_denoise.c:298-323). */
@@ -269,9 +221,8 @@ __global__ __launch_bounds__(DN_NT) void kb_denoise(const KFrame *__restrict__ f
 
 static_assert(DN_TW == KG_DN_TW && DN_TH == KG_DN_TH, "tile geometry");
 
-#define DN_MODES(X)                                                                        \
-    X(KDN_SPATIAL) X(KDN_TEMPORAL) X(KDN_TEMPORAL | KDN_HISTORY)                           \
-    X(KDN_SPATIAL | KDN_TEMPORAL) X(KDN_SPATIAL | KDN_TEMPORAL | KDN_HISTORY)
+/* the modes this kernel serves: the spatial ones go to k_nlm_sym.hip */
+#define DN_MODES(X) X(KDN_TEMPORAL) X(KDN_TEMPORAL | KDN_HISTORY)
 
 hipError_t craw_launch_denoise(int mode, const KFrame &f, hipStream_t s)
 {

@@ -1,8 +1,8 @@
 /*
  * libcraw for MI355X: the fast forms of the raw-domain denoise.
  *
- * Same filter as k_nlm.hip (DESIGN 14.3 / 14.7 of cyanreg/libcraw, the
- * definition of lib/ref/ref_denoise.c:114-154), evaluated for throughput
+ * Same filter as k_nlm_sym.hip (the definition of
+ * lib/ref/ref_denoise.c:114-154 of cyanreg/libcraw), evaluated for throughput
  * rather than bit-exactness: the patch distance of every offset is built
  * from each position's squared difference computed once, summed in three
  * across the lanes of a DPP row (row_shr:1 / row_shl:1) and in three down

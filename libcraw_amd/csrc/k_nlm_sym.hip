@@ -184,22 +184,49 @@ struct XsBand {
 
         if constexpr (HIST)
             L.rv[3] = 1.0 / (double)sm.m.var[r + 3][lc];
+        float dd[XS_NO];
+
+        /* The patch distances, three offsets' sums interleaved step by
+         * step: a DPP operation must not read a register the instruction
+         * just before it wrote, and the interleaving keeps two others
+         * between them instead of a wait. Each sum is still in raster
+         * order: row above, own row, row below; columns x - 1, x, x + 1. */
 #pragma unroll
         for (int o = 0; o < XS_NO; o++) {
             const float e = plane[r + 1][lc] - plane[r + 1 + xs_dy(o)][lc + xs_dx(o)];
-            float d2;
-
             sc[o] = e * e;
-            /* raster order: row above, own row, row below; columns x - 1,
-             * x, x + 1 */
-            d2 = xs_shr(sa[o]) + sa[o];
-            d2 = d2 + xs_shl(sa[o]);
-            d2 = d2 + xs_shr(sb[o]);
-            d2 = d2 + sb[o];
-            d2 = d2 + xs_shl(sb[o]);
-            d2 = d2 + xs_shr(sc[o]);
-            d2 = d2 + sc[o];
-            d2 = d2 + xs_shl(sc[o]);
+        }
+#pragma unroll
+        for (int g = 0; g < XS_NO; g += 3) {
+#pragma unroll
+            for (int j = 0; j < 3; j++)
+                dd[g + j] = xs_shr(sa[g + j]) + sa[g + j];
+#pragma unroll
+            for (int j = 0; j < 3; j++)
+                dd[g + j] = dd[g + j] + xs_shl(sa[g + j]);
+#pragma unroll
+            for (int j = 0; j < 3; j++)
+                dd[g + j] = dd[g + j] + xs_shr(sb[g + j]);
+#pragma unroll
+            for (int j = 0; j < 3; j++)
+                dd[g + j] = dd[g + j] + sb[g + j];
+#pragma unroll
+            for (int j = 0; j < 3; j++)
+                dd[g + j] = dd[g + j] + xs_shl(sb[g + j]);
+#pragma unroll
+            for (int j = 0; j < 3; j++)
+                dd[g + j] = dd[g + j] + xs_shr(sc[g + j]);
+#pragma unroll
+            for (int j = 0; j < 3; j++)
+                dd[g + j] = dd[g + j] + sc[g + j];
+#pragma unroll
+            for (int j = 0; j < 3; j++)
+                dd[g + j] = dd[g + j] + xs_shl(sc[g + j]);
+        }
+#pragma unroll
+        for (int o = 0; o < XS_NO; o++) {
+            const float d2 = dd[o];
+
             if constexpr (HIST) {
                 /* lib/ref/ref_denoise.c:138-145 with v2 = v_p + v_q; the
                  * argument clamped below at 0 (a NaN to 0, as the

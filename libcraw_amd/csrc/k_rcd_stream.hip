@@ -25,30 +25,13 @@
  * columns (29 4x4 blocks) into the chroma stage, 112 (14 8x8 blocks) to
  * the output, so no block straddles two strips.
  */
-#include "k_common.h"
-#include "kernels.h"
+#include "k_stream.h"
 
 #define RS_NW 4
 #define RS_NT (64 * RS_NW)
 #define RS_EPS 1e-5f
 
 static_assert(KG_RS_OUT == 116 && KG_RS_OUT_F == 112 && KG_RS_R % 8 == 0, "strip geometry");
-
-#define DPP_WAVE_SHR1 0x138
-#define DPP_WAVE_SHL1 0x130
-
-/* The value of the lane to the left / right (0 past the wave's ends). */
-KDEV float rs_shr(float v)
-{
-    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v),
-                                                                 DPP_WAVE_SHR1, 0xf, 0xf, true));
-}
-
-KDEV float rs_shl(float v)
-{
-    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v),
-                                                                 DPP_WAVE_SHL1, 0xf, 0xf, true));
-}
 
 /* lib/ref/ref_debayer.c:195-202 */
 KDEV float rs_ratio(float a, float b)
@@ -57,17 +40,6 @@ KDEV float rs_ratio(float a, float b)
     b = b > 0.0f ? b : 0.0f;
     return (a - b) / (a + b + RS_EPS);
 }
-
-/* A row of a stage: the lane's two columns. */
-struct RsRow {
-    float c[2];
-};
-
-/* Column C's left / right neighbour of row r, for the lane's column C. */
-template <int C> KDEV float rs_l1(const RsRow &r) { return C == 0 ? rs_shr(r.c[1]) : r.c[0]; }
-template <int C> KDEV float rs_r1(const RsRow &r) { return C == 0 ? r.c[1] : rs_shl(r.c[0]); }
-template <int C> KDEV float rs_l2(const RsRow &r) { return rs_shr(r.c[C]); }
-template <int C> KDEV float rs_r2(const RsRow &r) { return rs_shl(r.c[C]); }
 
 struct RsState {
     RsRow v[7];                 /* V rows vt - 6 .. vt */
@@ -160,54 +132,6 @@ KDEV float rs_colour(const RsState &s)
     return w * (g00 + 0.5f * (dsw + dne)) + (1.0f - w) * (g00 + 0.5f * (dnw + dse));
 }
 
-/* The output of row oy without the chroma stage: each 8x8 block's sums in
- * raster order (the block's first lane, column X0 % 8 == 0, adds its two
- * columns and then its three right neighbours'; lib/awb.c:249-256), its
- * sample on the block's last row, and the colour stage into the
- * destination (lib/ref/ref_color.c:10-76). */
-KDEV void rs_finish(RsState &s, const KFrame &f, int oy, bool out_cols, int X0,
-                    const float r[2], const float g[2], const float b[2])
-{
-    if (f.out.hist) {
-        const bool first_row = (oy & 7) == 0;
-        const float *chan[3] = { r, g, b };
-
-#pragma unroll
-        for (int c = 0; c < 3; c++) {
-            const float n10 = rs_shl(chan[c][0]), n11 = rs_shl(chan[c][1]);
-            const float n20 = rs_shl(n10), n21 = rs_shl(n11);
-            const float n30 = rs_shl(n20), n31 = rs_shl(n21);
-            float a = (first_row ? 0.0f : s.acc[c]) + chan[c][0];
-
-            a = a + chan[c][1];
-            a = a + n10;
-            a = a + n11;
-            a = a + n20;
-            a = a + n21;
-            a = a + n30;
-            a = a + n31;
-            s.acc[c] = a;
-        }
-        if ((oy & 7) == 7 && out_cols && (X0 & 7) == 0) {
-            CRAWAwbSample smp;
-            const bool valid = craw_awb_sample(s.acc, 64.0f, f.out.awb_wb0, f.out.awb_wb2, &smp);
-
-            f.out.samples[(size_t)(oy / 8) * ((f.W + 7) / 8) + X0 / 8] =
-                valid ? make_uint2(smp.bin | (smp.weight << 14), smp.bright)
-                      : make_uint2(KAWB_NONE, 0u);
-        }
-    }
-    if (out_cols) {
-#pragma unroll 1
-        for (int c = 0; c < 2; c++) {
-            float o[3];
-
-            kcolor(f.col, r[c], g[c], b[c], o);
-            kstore_rgba(f.out, X0 + c, oy, o);
-        }
-    }
-}
-
 /* One step: P is the parity of the step (the newest V row is odd for
  * P = 0, the strip starting on an even row). */
 template <bool INTER, int P>
@@ -266,8 +190,6 @@ KDEV void rs_step(RsState &s, const KFrame &f, int vt, int ld0, bool out_row, bo
     /* output, row vt - 5: green column 1 - P, red/blue column P */
     if (out_row) {
         constexpr int CG = 1 - P, CN = P;
-        const KInter &it = f.inter;
-        const size_t plane = (size_t)f.W * f.H;
         const int oy = vt - 5;
         float r[2], g[2], b[2];
 
@@ -293,50 +215,10 @@ KDEV void rs_step(RsState &s, const KFrame &f, int vt, int ld0, bool out_row, bo
             b[CG] = P == 0 ? cv : ch;
         }
 
-        if (!INTER) {
-            rs_finish(s, f, oy, out_cols, X0, r, g, b);
-            return;
-        }
-
-        if (out_cols) {
-            const size_t o = (size_t)oy * f.W + X0;
-            *(float2 *)&it.rgb[o] = make_float2(r[0], r[1]);
-            *(float2 *)&it.rgb[plane + o] = make_float2(g[0], g[1]);
-            *(float2 *)&it.rgb[2 * plane + o] = make_float2(b[0], b[1]);
-        }
-
-        /* 4x4 block sums in raster order: the block's first lane (column
-         * X0 % 4 == 0) adds its two columns and then its right neighbour's
-         * (lib/ref/ref_chroma.c:59-82) */
-        {
-            const float d0[2] = { r[0] - g[0], r[1] - g[1] };
-            const float d1[2] = { b[0] - g[0], b[1] - g[1] };
-            const float n00 = rs_shl(d0[0]), n01 = rs_shl(d0[1]);
-            const float n10 = rs_shl(d1[0]), n11 = rs_shl(d1[1]);
-            const float n20 = rs_shl(g[0]), n21 = rs_shl(g[1]);
-            const bool first_row = (oy & 3) == 0;
-
-            s.acc[0] = (first_row ? 0.0f : s.acc[0]) + d0[0];
-            s.acc[0] = s.acc[0] + d0[1];
-            s.acc[0] = s.acc[0] + n00;
-            s.acc[0] = s.acc[0] + n01;
-            s.acc[1] = (first_row ? 0.0f : s.acc[1]) + d1[0];
-            s.acc[1] = s.acc[1] + d1[1];
-            s.acc[1] = s.acc[1] + n10;
-            s.acc[1] = s.acc[1] + n11;
-            s.acc[2] = (first_row ? 0.0f : s.acc[2]) + g[0];
-            s.acc[2] = s.acc[2] + g[1];
-            s.acc[2] = s.acc[2] + n20;
-            s.acc[2] = s.acc[2] + n21;
-            if ((oy & 3) == 3 && out_cols && (X0 & 3) == 0) {
-                const size_t q = (size_t)(oy / 4) * it.wq + X0 / 4;
-                const size_t qplane = (size_t)it.wq * it.hq;
-
-                it.down[q] = s.acc[0] / 16.0f;
-                it.down[qplane + q] = s.acc[1] / 16.0f;
-                it.down[2 * qplane + q] = s.acc[2] / 16.0f;
-            }
-        }
+        if (INTER)
+            rs_store_inter(s.acc, f, oy, out_cols, X0, r, g, b);
+        else
+            rs_finish(s.acc, f, oy, out_cols, X0, r, g, b);
     }
 }
 

@@ -118,6 +118,8 @@ struct KFrame {
 #define KG_RS_OUT_F 112         /* the same without the chroma stage */
 #define KG_RS_R   64            /* and rows */
 #define KG_RS_NW  4             /* strips per workgroup */
+#define KG_SS_OUT 120           /* streamed bilinear/MHC: output columns per strip */
+#define KG_SS_R   64            /* and rows */
 
 enum KStage {
     KSTAGE_DENOISE = 0,
@@ -208,11 +210,12 @@ static inline void craw_stage_geom(int stage, int method, int dn_geom, int *tw, 
     }
 }
 
-/* The split of a frame for RCD: whole 32x16 tiles at
- * least 5 pixels (the filter's reach) from every border form the interior,
+/* The split of a frame for a streamed debayer: whole tw x th tiles whose
+ * filter reach (`reach` pixels) lies inside the frame form the interior,
  * developed by the streamed kernel; the ring of tiles around it by the
  * tiled one. Tile units [itx0, itx1) x [ity0, ity1), empty for a frame too
- * small to have one. */
+ * small to have one. RCD splits 32x16 tiles with a reach of 5, bilinear
+ * and MHC 64x16 tiles with a reach of 2. */
 struct KRcdSplit {
     int ntx, nty;
     int itx0, itx1, ity0, ity1;
@@ -222,23 +225,39 @@ struct KRcdSplit {
 #ifdef __HIPCC__
 __host__ __device__
 #endif
-static inline KRcdSplit craw_rcd_split(int W, int H)
+static inline KRcdSplit craw_tile_split(int W, int H, int tw, int th, int reach)
 {
     KRcdSplit s;
 
-    s.ntx = (W + KG_RCD_TW - 1) / KG_RCD_TW;
-    s.nty = (H + KG_RCD_TH - 1) / KG_RCD_TH;
+    s.ntx = (W + tw - 1) / tw;
+    s.nty = (H + th - 1) / th;
     s.itx0 = 1;
-    s.itx1 = W - KG_RCD_TW * (s.ntx - 1) >= 5 ? s.ntx - 1 : s.ntx - 2;
+    s.itx1 = W - tw * (s.ntx - 1) >= reach ? s.ntx - 1 : s.ntx - 2;
     s.ity0 = 1;
-    s.ity1 = H - KG_RCD_TH * (s.nty - 1) >= 5 ? s.nty - 1 : s.nty - 2;
+    s.ity1 = H - th * (s.nty - 1) >= reach ? s.nty - 1 : s.nty - 2;
     if (s.itx1 <= s.itx0 || s.ity1 <= s.ity0)
         s.itx0 = s.itx1 = s.ity0 = s.ity1 = 0;
-    s.x_lo = KG_RCD_TW * s.itx0;
-    s.x_hi = KG_RCD_TW * s.itx1;
-    s.y_lo = KG_RCD_TH * s.ity0;
-    s.y_hi = KG_RCD_TH * s.ity1;
+    s.x_lo = tw * s.itx0;
+    s.x_hi = tw * s.itx1;
+    s.y_lo = th * s.ity0;
+    s.y_hi = th * s.ity1;
     return s;
+}
+
+#ifdef __HIPCC__
+__host__ __device__
+#endif
+static inline KRcdSplit craw_rcd_split(int W, int H)
+{
+    return craw_tile_split(W, H, KG_RCD_TW, KG_RCD_TH, 5);
+}
+
+#ifdef __HIPCC__
+__host__ __device__
+#endif
+static inline KRcdSplit craw_simple_split(int W, int H)
+{
+    return craw_tile_split(W, H, KG_SIMPLE_TW, KG_SIMPLE_TH, 2);
 }
 
 /* Ring tile i: the rows above the interior, the columns either side of it,
@@ -246,7 +265,7 @@ static inline KRcdSplit craw_rcd_split(int W, int H)
 #ifdef __HIPCC__
 __host__ __device__
 #endif
-static inline void craw_rcd_ring_tile(const KRcdSplit &s, int i, int *tx, int *ty)
+static inline void craw_ring_tile(const KRcdSplit &s, int i, int *tx, int *ty)
 {
     const int top = s.ity0 * s.ntx, per = s.ntx - (s.itx1 - s.itx0);
     const int mid = (s.ity1 - s.ity0) * per;

@@ -97,7 +97,7 @@ static inline real o_fetch(const real *p, uint32_t w, uint32_t h, int x, int y)
     return p[(size_t)o_mirror(y, (int)h) * w + o_mirror(x, (int)w)];
 }
 
-/* ref_debayer.c:42-63, DESIGN.md 4.1: black, range, gain, clip at 1.0;
+/* lib/ref/ref_debayer.c:42-63: black, range, gain, clip at 1.0;
  * below black is kept. */
 void PFX(prep)(const uint16_t *bayer, uint32_t width, uint32_t height, size_t stride,
                const CRAWProfile *p, const float wb[3], real *v)
@@ -118,7 +118,7 @@ void PFX(prep)(const uint16_t *bayer, uint32_t width, uint32_t height, size_t st
 
 #define V(dx, dy) o_fetch(v, width, height, x + (dx), y + (dy))
 
-/* ref_debayer.c:65-96, DESIGN.md 5.1. */
+/* lib/ref/ref_debayer.c:65-96. */
 static void o_bilinear(const real *v, uint32_t width, uint32_t height, real *rgb)
 {
     OMP_FOR
@@ -142,7 +142,7 @@ static void o_bilinear(const real *v, uint32_t width, uint32_t height, real *rgb
     }
 }
 
-/* ref_debayer.c:101-145, DESIGN.md 5.2: the four MHC kernels over 8. */
+/* lib/ref/ref_debayer.c:101-145: the four MHC kernels over 8. */
 static inline real o_mhc_green(const real *v, uint32_t width, uint32_t height, int x, int y)
 {
     return (K(4.0f, 4.0) * V(0, 0)
@@ -233,7 +233,7 @@ static inline real o_rcd_ratio(real a, real b)
     return (a - b) / (a + b + O_RCD_EPS);
 }
 
-/* ref_debayer.c:204-395, DESIGN.md 5.3: RCD in full-frame sweeps. */
+/* lib/ref/ref_debayer.c:204-395: RCD in full-frame sweeps. */
 static int o_rcd(const real *v, uint32_t width, uint32_t height, real *rgb)
 {
     size_t n = (size_t)width * height;
@@ -443,7 +443,7 @@ int PFX(debayer_wb)(const uint16_t *bayer, uint32_t width, uint32_t height, size
 #define O_DN_GUARD_HI K(1.00f, 1.00)
 #define O_DN_VAR_MIN  K(1.0e-4f, 1.0e-4)
 
-/* ref_denoise.c:26-58, DESIGN.md 14.3: the generalised Anscombe transform
+/* lib/ref/ref_denoise.c:26-58: the generalised Anscombe transform
  * with its tangent below black, and the algebraic inverse. */
 struct o_gat {
     real a, c, sc, fwd, inv, off, t0;
@@ -716,7 +716,7 @@ static inline int o_cd_mirror(int v, int n)
     return v;
 }
 
-/* ref_chroma.c:33-158, DESIGN.md 14.4. */
+/* lib/ref/ref_chroma.c:33-158. */
 int PFX(chroma_denoise)(real *rgb, uint32_t width, uint32_t height, float strength)
 {
     real *down = NULL, *blur = NULL, rs = (real)strength;
@@ -960,7 +960,7 @@ int PFX(color)(real *rgb, uint32_t width, uint32_t height, const CRAWFrameParams
 
 /* ------------------------------------------------------------------ AWB */
 
-/* awb.c:213-296, DESIGN.md 4.4: one sample per 8x8 block (partial blocks
+/* lib/awb.c:213-296: one sample per 8x8 block (partial blocks
  * at the edges average what they have), block means summed row by row,
  * clipped and near-black blocks skipped, ln(R/G) and ln(B/G) of the raw
  * response binned over [-2.5, 1] into 128 bins, weights in 16.16. */

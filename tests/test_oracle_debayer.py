@@ -189,7 +189,8 @@ def _conv_mirror(v, k):
 
 
 def test_bilinear_and_mhc_against_brute_force_convolutions():
-    """Independent restatement of DESIGN 5.1/5.2 as kernels applied by site,
+    """Independent restatement of the reference's bilinear and MHC filters
+    (lib/ref/ref_debayer.c:65-176) as kernels applied by site,
     float64, on a random prepped mosaic: bilinear and MHC of every oracle
     mode within float rounding (1e-6)."""
     rng = np.random.default_rng(4)

@@ -6,7 +6,8 @@ against what the mathematics and the reference's tests fix:
   - flat-field noise falls to at most 60% and a checkerboard keeps at least
     70% of its contrast (tests/test_pipeline.c:699-730 of the reference);
   - on a static scene the temporal state's variance converges to
-    (1 - a) / (1 + a) (DESIGN.md 14.7) and the noise falls every frame;
+    (1 - a) / (1 + a), the fixed point of the state recursion of
+    lib/ref/ref_denoise.c:82-105, and the noise falls every frame;
   - the temporal pass without history is the spatial filter exactly;
   - chroma denoise leaves G bit for bit, keeps a flat colour and reduces
     chroma blotches (tests/test_pipeline.c:4256-4312);
@@ -76,7 +77,8 @@ def test_flat_noise_and_checkerboard(mode):
 
 def test_static_scene_temporal_convergence():
     """Variance of the state on a static noise-only scene settles at
-    (1 - a)/(1 + a) = 0.111 at alpha_max 0.8 where the
+    (1 - a)/(1 + a) = 0.111 at alpha_max 0.8 (the fixed point of the
+    recursion v = (1 - a)^2 + a^2 v of lib/ref/ref_denoise.c:82-105) where the
     full weight is taken, and the
     output noise falls over the first frames."""
     h, w = 64, 64
