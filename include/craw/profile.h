@@ -114,6 +114,22 @@ CRAW_API int craw_profile_eval(const CRAWProfile *p, enum CRAWWbMode mode,
                                const CRAWIlluminant *illum,
                                CRAWColorParams *out);
 
+/* The point of the white balance locus a neutral surface's raw ratios
+ * (R/G, B/G before any balance) lie from along the green direction, and
+ * the signed distance as a tint (profile.c:367-389 of the reference): the
+ * inverse of craw_profile_illuminant_to_neutral() wherever the table
+ * reaches; a point past an end of the locus goes to that end. Needs at
+ * least two white balance knots (CRAW_ERROR(ENOTSUP)). */
+CRAW_API int craw_profile_neutral_to_illuminant(const CRAWProfile *p,
+                                                float raw_rg, float raw_bg,
+                                                CRAWIlluminant *out);
+
+/* The raw ratios a neutral surface gives under `illum`
+ * (profile.c:391-415 of the reference). */
+CRAW_API int craw_profile_illuminant_to_neutral(const CRAWProfile *p,
+                                                const CRAWIlluminant *illum,
+                                                float *raw_rg, float *raw_bg);
+
 /* The profile's noise table at `iso` (as_shot_ei when not positive);
  * CRAW_ERROR(ENOTSUP) without a table. */
 CRAW_API int craw_profile_noise_eval(const CRAWProfile *p, float iso,

@@ -148,6 +148,33 @@ class CRAWAwbHistogram(ctypes.Structure):
                 ("weight_bright", ctypes.POINTER(c_u32))]
 
 
+class CRAWAwbParams(ctypes.Structure):
+    _fields_ = [("band", c_float), ("sigma", c_float), ("min_zone_fraction", c_float),
+                ("ambience", c_float), ("ambience_knee_cct", c_float), ("cct_min", c_float),
+                ("cct_max", c_float), ("prefer_bright", ctypes.c_bool),
+                ("bright_max_spread_mired", c_float)]
+
+
+class CRAWAwbCandidate(ctypes.Structure):
+    _fields_ = [("valid", ctypes.c_bool), ("illuminant", CRAWIlluminant),
+                ("in_zone_fraction", c_float), ("spread_mired", c_float)]
+
+
+CRAW_AWB_SOURCE_NONE, CRAW_AWB_SOURCE_ZONE, CRAW_AWB_SOURCE_BRIGHT = range(3)
+
+
+class CRAWAwbEstimate(ctypes.Structure):
+    _fields_ = [("valid", ctypes.c_bool), ("illuminant", CRAWIlluminant),
+                ("confidence", c_float), ("in_zone_fraction", c_float),
+                ("spread_mired", c_float), ("source", c_i32), ("zone", CRAWAwbCandidate),
+                ("bright", CRAWAwbCandidate)]
+
+
+class CRAWAwbSmootherParams(ctypes.Structure):
+    _fields_ = [("time_constant", c_float), ("hysteresis_mired", c_float),
+                ("max_slew_mired_per_s", c_float), ("min_confidence", c_float)]
+
+
 class CRAWStageTime(ctypes.Structure):
     _fields_ = [("name", ctypes.c_char_p), ("count", c_u32), ("ms", ctypes.c_double)]
 
@@ -213,6 +240,20 @@ SIGNATURES = [
     ("craw_bradford_adaptation", c_i32, [P(c_float), P(c_float), P(c_float)]),
     ("craw_awb_histogram_from_rgb", c_i32, [P(c_float), c_u32, c_u32, ctypes.c_size_t, c_i32,
                                              P(c_float), P(c_u32), P(c_u32), P(c_u32)]),
+    ("craw_profile_neutral_to_illuminant", c_i32, [P(CRAWProfile), c_float, c_float,
+                                                    P(CRAWIlluminant)]),
+    ("craw_profile_illuminant_to_neutral", c_i32, [P(CRAWProfile), P(CRAWIlluminant),
+                                                    P(c_float), P(c_float)]),
+    ("craw_awb_params_init", None, [P(CRAWAwbParams)]),
+    ("craw_awb_estimate", c_i32, [P(CRAWProfile), P(CRAWAwbHistogram), P(CRAWAwbParams),
+                                   P(CRAWAwbEstimate)]),
+    ("craw_awb_smoother_params_init", None, [P(CRAWAwbSmootherParams)]),
+    ("craw_awb_smoother_create", c_i32, [P(CRAWAwbSmootherParams), P(vp)]),
+    ("craw_awb_smoother_free", None, [P(vp)]),
+    ("craw_awb_smoother_reset", None, [vp]),
+    ("craw_awb_smoother_set", None, [vp, ctypes.c_double, P(CRAWIlluminant)]),
+    ("craw_awb_smoother_push", c_i32, [vp, ctypes.c_double, P(CRAWAwbEstimate), ctypes.c_bool,
+                                        P(CRAWIlluminant)]),
     ("craw_batch_params_init", None, [P(CRAWBatchParams)]),
     ("craw_batch_create", c_i32, [vp, P(CRAWBatchParams), P(vp)]),
     ("craw_batch_free", None, [P(vp)]),

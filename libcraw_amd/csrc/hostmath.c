@@ -2,7 +2,7 @@
  * libcraw for MI355X: host-side math of the path, the per-frame constants
  * the kernels are handed.
  *
- *   craw_profile_*            lib/profile.c:26-252 of cyanreg/libcraw
+ *   craw_profile_*            lib/profile.c:26-252, 255-415 of cyanreg/libcraw
  *   craw_noise_model_eval     lib/noise.c:37-110
  *   craw_colorspace_*, craw_planckian_xy, craw_bradford_adaptation
  *                             lib/colorspace.c:25-319
@@ -124,6 +124,26 @@ static double clampd(double v, double lo, double hi)
     return v < hi ? v : hi;
 }
 
+/* The white balance gains at `cct` (two knots or more: linear in Kelvin
+ * inside the interval, the position clamped to it). */
+static void wb_gains_at(const CRAWProfile *p, double cct, double wb[3])
+{
+    double f;
+    uint32_t i;
+
+    if (p->nb_wb_knots == 1) {
+        for (int c = 0; c < 3; c++)
+            wb[c] = p->wb_knots[0].wb[c];
+        return;
+    }
+    i = knot_find(&p->wb_knots[0].cct, sizeof(CRAWWbKnot) / sizeof(float), p->nb_wb_knots, cct,
+                  &f);
+    for (int c = 0; c < 3; c++) {
+        double a = p->wb_knots[i].wb[c], b = p->wb_knots[i + 1].wb[c];
+        wb[c] = a + (b - a) * f;
+    }
+}
+
 int craw_profile_eval(const CRAWProfile *p, enum CRAWWbMode mode,
                       const CRAWIlluminant *illum, CRAWColorParams *out)
 {
@@ -146,17 +166,7 @@ int craw_profile_eval(const CRAWProfile *p, enum CRAWWbMode mode,
 
         cct = clampd((double)illum->cct, (double)p->wb_knots[0].cct,
                      (double)p->wb_knots[p->nb_wb_knots - 1].cct);
-        if (p->nb_wb_knots == 1) {
-            for (int c = 0; c < 3; c++)
-                wb[c] = p->wb_knots[0].wb[c];
-        } else {
-            i = knot_find(&p->wb_knots[0].cct, sizeof(CRAWWbKnot) / sizeof(float),
-                          p->nb_wb_knots, cct, &f);
-            for (int c = 0; c < 3; c++) {
-                double a = p->wb_knots[i].wb[c], b = p->wb_knots[i + 1].wb[c];
-                wb[c] = a + (b - a) * f;
-            }
-        }
+        wb_gains_at(p, cct, wb);
         tint_gain = exp((double)illum->tint * INV_SQRT2);
         wb[0] *= tint_gain;
         wb[2] *= tint_gain;
@@ -188,6 +198,140 @@ int craw_profile_eval(const CRAWProfile *p, enum CRAWWbMode mode,
     out->gain = p->gain;
     if (p->as_shot_ei > 0.0f && p->exposure_index > 0.0f)
         out->gain *= p->exposure_index / p->as_shot_ei;
+    return 0;
+}
+
+/* ------------------------------------------------ the white balance locus */
+
+/* Resampling step of the locus walk, in mired (equal steps in mired: the
+ * sensor's ratios and perceived balance are both near linear in it). */
+#define LOCUS_STEP_MIRED 5.0
+
+/* How far the neutral (lx, ly) = (ln R/G, ln B/G) lies across the green
+ * direction from the locus point at `cct`: the residual r = neutral minus
+ * locus (the locus point of gains wb is (-ln wb_R, -ln wb_B)), dotted with
+ * (1, -1). Zero when r is along (1, 1), the direction a tint moves a
+ * neutral. */
+static double locus_across(const CRAWProfile *p, double lx, double ly, double cct, double r[2])
+{
+    double wb[3];
+
+    wb_gains_at(p, cct, wb);
+    r[0] = lx + log(wb[0]);
+    r[1] = ly + log(wb[2]);
+    return r[0] - r[1];
+}
+
+/* The locus point (lx, ly) lies from along the green direction
+ * (lib/profile.c:285-365): the locus is sampled from the warm end to the
+ * cool one at every knot and at most LOCUS_STEP_MIRED apart between them;
+ * the first pair of samples the across-distance changes sign between is
+ * bisected in mired (60 halvings), and a point beyond both ends goes to the
+ * end nearer in that distance. The residual of the point found is left in
+ * r. */
+static void locus_green(const CRAWProfile *p, double lx, double ly, double *cct_out, double r[2])
+{
+    const uint32_t n = p->nb_wb_knots;
+    double prev_cct = 0.0, prev_g = 0.0, next_cct = 0.0, ma, mb;
+    int have_prev = 0, bracket = 0;
+
+    for (uint32_t k = 0; k < n && !bracket; k++) {
+        const double m0 = 1e6 / p->wb_knots[k].cct;
+        const double span = k + 1 < n ? m0 - 1e6 / p->wb_knots[k + 1].cct : 0.0;
+        int steps = k + 1 < n ? (int)ceil(span / LOCUS_STEP_MIRED) : 1;
+
+        steps = steps > 0 ? steps : 1;
+        for (int j = 0; j < steps; j++) {
+            const double cct = 1e6 / (m0 - span * ((double)j / steps));
+            const double g = locus_across(p, lx, ly, cct, r);
+
+            if (g == 0.0) {
+                *cct_out = cct;
+                return;
+            }
+            if (have_prev && (g > 0.0) != (prev_g > 0.0)) {
+                next_cct = cct;
+                bracket = 1;
+                break;
+            }
+            prev_cct = cct;
+            prev_g = g;
+            have_prev = 1;
+        }
+    }
+
+    if (!bracket) {
+        const double warm = p->wb_knots[0].cct, cool = p->wb_knots[n - 1].cct;
+        double rw[2], rc[2];
+        const int to_warm = fabs(locus_across(p, lx, ly, warm, rw)) <=
+                            fabs(locus_across(p, lx, ly, cool, rc));
+
+        *cct_out = to_warm ? warm : cool;
+        r[0] = to_warm ? rw[0] : rc[0];
+        r[1] = to_warm ? rw[1] : rc[1];
+        return;
+    }
+
+    ma = 1e6 / prev_cct;
+    mb = 1e6 / next_cct;
+    for (int k = 0; k < 60; k++) {
+        const double m = 0.5 * (ma + mb);
+        const double g = locus_across(p, lx, ly, 1e6 / m, r);
+
+        if (g == 0.0) {
+            ma = mb = m;
+            break;
+        }
+        if ((g > 0.0) == (prev_g > 0.0)) {
+            ma = m;
+            prev_g = g;
+        } else {
+            mb = m;
+        }
+    }
+    *cct_out = 1e6 / (0.5 * (ma + mb));
+    locus_across(p, lx, ly, *cct_out, r);
+}
+
+int craw_profile_neutral_to_illuminant(const CRAWProfile *p, float raw_rg, float raw_bg,
+                                       CRAWIlluminant *out)
+{
+    double cct, r[2];
+
+    if (!out || !p)
+        return CRAW_ERROR(EINVAL);
+    if (!(raw_rg > 0.0f) || !(raw_bg > 0.0f) || !isfinite(raw_rg) || !isfinite(raw_bg))
+        return CRAW_ERROR(EINVAL);
+    if (p->nb_wb_knots < 2)
+        return CRAW_ERROR(ENOTSUP);
+
+    locus_green(p, log((double)raw_rg), log((double)raw_bg), &cct, r);
+    out->cct = (float)cct;
+    /* the residual along (1, 1) / sqrt(2), as a tint (its axis is
+     * (-1, -1) / sqrt(2)) */
+    out->tint = (float)(-(r[0] + r[1]) * INV_SQRT2);
+    return 0;
+}
+
+int craw_profile_illuminant_to_neutral(const CRAWProfile *p, const CRAWIlluminant *illum,
+                                       float *raw_rg, float *raw_bg)
+{
+    double wb[3], cct, g;
+
+    if (!p || !illum || !raw_rg || !raw_bg)
+        return CRAW_ERROR(EINVAL);
+    if (!p->nb_wb_knots)
+        return CRAW_ERROR(ENOTSUP);
+    if (!isfinite(illum->cct) || !isfinite(illum->tint))
+        return CRAW_ERROR(EINVAL);
+
+    cct = clampd((double)illum->cct, (double)p->wb_knots[0].cct,
+                 (double)p->wb_knots[p->nb_wb_knots - 1].cct);
+    wb_gains_at(p, cct, wb);
+    /* what the gains of craw_profile_eval() for this illuminant undo */
+    g = exp(-(double)illum->tint * INV_SQRT2);
+    *raw_rg = (float)(g / wb[0]);
+    *raw_bg = (float)(g / wb[2]);
     return 0;
 }
 

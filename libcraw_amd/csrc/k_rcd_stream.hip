@@ -52,14 +52,6 @@ struct RsState {
     float acc[3];               /* 4x4 block sums */
 };
 
-template <int N>
-KDEV void rs_shift(RsRow (&a)[N])
-{
-#pragma unroll
-    for (int k = 0; k + 1 < N; k++)
-        a[k] = a[k + 1];
-}
-
 /* Gradients of column C at the centre row v[4] (rows v[2] .. v[6]). */
 template <int C>
 KDEV void rs_gradients(RsState &s)

@@ -36,6 +36,15 @@ struct RsRow {
     float c[2];
 };
 
+/* Shifts a window of rows by one: row k + 1 becomes row k. */
+template <int N>
+KDEV void rs_shift(RsRow (&a)[N])
+{
+#pragma unroll
+    for (int k = 0; k + 1 < N; k++)
+        a[k] = a[k + 1];
+}
+
 /* Column C's left / right neighbour of row r, for the lane's column C. */
 template <int C> KDEV float rs_l1(const RsRow &r) { return C == 0 ? rs_shr(r.c[1]) : r.c[0]; }
 template <int C> KDEV float rs_r1(const RsRow &r) { return C == 0 ? r.c[1] : rs_shl(r.c[0]); }

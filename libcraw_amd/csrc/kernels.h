@@ -314,6 +314,10 @@ static inline int craw_stage_tiles(int stage, int W, int H, int method, int dn_g
 }
 
 hipError_t craw_launch_zero_u32(uint32_t *p, size_t n, hipStream_t s);
+/* The streamed bilinear/MHC over the frame's interior (the plain form's
+ * debayer launches it after the ring of tiles). */
+hipError_t craw_launch_simple_stream(bool to_inter, const KFrame &f, hipStream_t s);
+int craw_simple_stream_groups(int W, int H);
 /* n words (a multiple of 4, 16-byte aligned) of hist to the device-visible
  * host buffer host, zeroing hist. */
 hipError_t craw_launch_hist_export(uint32_t *hist, uint32_t *host, size_t n, hipStream_t s);

@@ -123,6 +123,29 @@ def test_rcd_interior_and_ring(gpu_available, size, chroma):
            "rcd %dx%d chroma %d" % (w, h, chroma), hist=True)
 
 
+@pytest.mark.parametrize("method", ["bilinear", "mhc"])
+@pytest.mark.parametrize("chroma", [True, False])
+@pytest.mark.parametrize("size", [(250, 98), (520, 336), (610, 298), (96, 40), (194, 50),
+                                  (192, 48), (1000, 200)])
+def test_simple_interior_and_ring(gpu_available, method, size, chroma):
+    """Bilinear and MHC develop the frame's interior with the streamed
+    kernel and the ring of 64x16 tiles around it with the tiled one, into
+    the chroma stage or to the output (AWB samples and colour): several
+    strips across and down with partial last strips, an interior ending
+    exactly at the filter's reach, frames a whole number of tiles, and
+    frames too small to have an interior."""
+    w, h = size
+    flags = capi.CRAW_PIPELINE_FLAG_AWB_HISTOGRAM
+    if chroma:
+        flags |= capi.CRAW_PIPELINE_FLAG_CHROMA_DENOISE
+    debayer = capi.CRAW_DEBAYER_BILINEAR if method == "bilinear" else capi.CRAW_DEBAYER_MHC
+    run = Runner(w, h, debayer=debayer, out_space=capi.CRAW_CS_BT2020_LINEAR, flags=flags)
+    prof = scene.make_profile()
+    bayer = scene.make_frame(w, h, seed=51 + w + h)
+    _check(run, bayer, frame_params(prof, denoise_chroma_strength=0.6),
+           "%s %dx%d chroma %d" % (method, w, h, chroma), hist=True)
+
+
 def test_temporal_sequence(gpu_available):
     """Four frames of a moving clip: every frame's output equals the
     oracle run with the state threaded through it."""
