@@ -175,6 +175,17 @@ class CRAWAwbSmootherParams(ctypes.Structure):
                 ("max_slew_mired_per_s", c_float), ("min_confidence", c_float)]
 
 
+class CRAWNoiseEstimateParams(ctypes.Structure):
+    _fields_ = [("nb_bins", c_u32), ("min_bin_samples", c_u32), ("max_level", c_float),
+                ("padding", ctypes.c_uint8 * (64 - 12))]
+
+
+class CRAWNoiseEstimate(ctypes.Structure):
+    _fields_ = [("model", CRAWNoiseModel), ("nb_bins", c_u32), ("nb_inliers", c_u32),
+                ("fit_rms", c_float), ("read_noise_lsb", c_float),
+                ("padding", ctypes.c_uint8 * (64 - 24))]
+
+
 class CRAWStageTime(ctypes.Structure):
     _fields_ = [("name", ctypes.c_char_p), ("count", c_u32), ("ms", ctypes.c_double)]
 
@@ -244,6 +255,10 @@ SIGNATURES = [
                                                     P(CRAWIlluminant)]),
     ("craw_profile_illuminant_to_neutral", c_i32, [P(CRAWProfile), P(CRAWIlluminant),
                                                     P(c_float), P(c_float)]),
+    ("craw_noise_estimate_params_init", None, [P(CRAWNoiseEstimateParams)]),
+    ("craw_noise_estimate", c_i32, [P(ctypes.c_uint16), c_u32, c_u32, ctypes.c_size_t,
+                                     P(CRAWProfile), P(CRAWNoiseEstimateParams),
+                                     P(CRAWNoiseEstimate)]),
     ("craw_awb_params_init", None, [P(CRAWAwbParams)]),
     ("craw_awb_estimate", c_i32, [P(CRAWProfile), P(CRAWAwbHistogram), P(CRAWAwbParams),
                                    P(CRAWAwbEstimate)]),

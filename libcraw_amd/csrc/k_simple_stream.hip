@@ -19,6 +19,7 @@
 
 #define SS_NW 4
 #define SS_NT (64 * SS_NW)
+#define SS_PF 4                 /* rows loaded ahead */
 
 static_assert(KG_SS_OUT % 8 == 0 && KG_SS_R % 8 == 0 && KG_SS_OUT / 2 + 2 <= 64,
               "strip geometry");
@@ -90,19 +91,16 @@ KDEV void ss_pixel(const SsState &s, int site, float &r, float &g, float &b)
 #undef V
 }
 
-/* One step: the newest row vt enters; row vt - 2 is developed. P is its
- * parity (the strip starts on an even row). */
+/* One step: the newest row vt (its two raw samples loaded ahead) enters;
+ * row vt - 2 is developed. P is the parity of both (the strip starts on an
+ * even row). */
 template <int METHOD, bool INTER, int P>
-KDEV void ss_step(SsState &s, const KFrame &f, int vt, int ld0, bool out_row, bool out_cols,
-                  int X0)
+KDEV void ss_step(SsState &s, const KFrame &f, int vt, const uint32_t raw[2], bool out_row,
+                  bool out_cols, int X0)
 {
     rs_shift(s.v);
-    {
-        const uint16_t *row = f.src + (size_t)vt * f.src_pitch;
-        const int sb = (vt & 1) * 2;
-        s.v[4].c[0] = kprep(row[ld0], sb, f.prep);
-        s.v[4].c[1] = kprep(row[ld0 + 1], sb + 1, f.prep);
-    }
+    s.v[4].c[0] = kprep(raw[0], 2 * P, f.prep);
+    s.v[4].c[1] = kprep(raw[1], 2 * P + 1, f.prep);
     if (out_row) {
         const int oy = vt - 2;
         float r[2], g[2], b[2];
@@ -144,12 +142,33 @@ KDEV void dev_simple_stream(const KFrame &f, int strip)
             s.v[k].c[0] = s.v[k].c[1] = 0.0f;
         s.acc[0] = s.acc[1] = s.acc[2] = 0.0f;
 
-        /* rows ys - 2 .. ye + 1 enter; ys and nsteps are even, so the
-         * developed row's parity alternates 0, 1 */
+        /* rows ys - 2 .. ye + 1 enter, SS_PF at a time: their loads are
+         * issued together, then the steps run on them (a step's load would
+         * otherwise wait out its own latency). ys is even, so step t's
+         * rows have the parity of t; steps past the strip's end (the last
+         * group's padding) load a row inside the frame and develop
+         * nothing. */
 #pragma unroll 1
-        for (int t = 0; t < nsteps; t += 2) {
-            ss_step<METHOD, INTER, 0>(s, f, ys - 2 + t, ld0, t >= 4, out_cols, X0);
-            ss_step<METHOD, INTER, 1>(s, f, ys - 1 + t, ld0, t + 1 >= 4, out_cols, X0);
+        for (int t0 = 0; t0 < nsteps; t0 += SS_PF) {
+            uint32_t raw[SS_PF][2];
+
+#pragma unroll
+            for (int k = 0; k < SS_PF; k++) {
+                const int vt = min(ys - 2 + t0 + k, f.H - 1);
+                const uint16_t *row = f.src + (size_t)vt * f.src_pitch;
+
+                raw[k][0] = row[ld0];
+                raw[k][1] = row[ld0 + 1];
+            }
+#pragma unroll
+            for (int k = 0; k < SS_PF; k += 2) {
+                const int t = t0 + k;
+
+                ss_step<METHOD, INTER, 0>(s, f, ys - 2 + t, raw[k], t >= 4 && t < nsteps,
+                                          out_cols, X0);
+                ss_step<METHOD, INTER, 1>(s, f, ys - 1 + t, raw[k + 1],
+                                          t + 1 >= 4 && t + 1 < nsteps, out_cols, X0);
+            }
         }
     }
 }

@@ -119,7 +119,7 @@ struct KFrame {
 #define KG_RS_R   64            /* and rows */
 #define KG_RS_NW  4             /* strips per workgroup */
 #define KG_SS_OUT 120           /* streamed bilinear/MHC: output columns per strip */
-#define KG_SS_R   64            /* and rows */
+#define KG_SS_R   16            /* and rows */
 
 enum KStage {
     KSTAGE_DENOISE = 0,
