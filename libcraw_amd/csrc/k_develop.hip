@@ -779,11 +779,7 @@ template <int METHOD, bool INTER>
 __global__ __launch_bounds__(BL_NT) void k_debayer_simple(KFrame f)
 {
     __shared__ SmSimple sm;
-    int tx, ty;
-
-    /* the ring around the streamed interior, blockIdx.x a ring tile */
-    craw_ring_tile(craw_simple_split(f.W, f.H), blockIdx.x, &tx, &ty);
-    dev_debayer_simple_tile<METHOD, INTER>(sm, f, tx * BL_TW, ty * BL_TH);
+    dev_debayer_simple_tile<METHOD, INTER>(sm, f, blockIdx.x * BL_TW, blockIdx.y * BL_TH);
 }
 
 template <bool INTER>
@@ -941,11 +937,7 @@ hipError_t craw_launch_debayer(bool to_inter, const KFrame &f, hipStream_t s)
             return e;
         return craw_launch_rcd_stream(to_inter, f, s);
     } else {
-        /* the ring of tiles, then the streamed interior */
-        const KRcdSplit sp = craw_simple_split(f.W, f.H);
-        const dim3 grid(sp.ntx * sp.nty - (sp.itx1 - sp.itx0) * (sp.ity1 - sp.ity0));
-        hipError_t e;
-
+        dim3 grid((f.W + BL_TW - 1) / BL_TW, (f.H + BL_TH - 1) / BL_TH);
         if (f.method == 0 && to_inter)
             KLAUNCH((k_debayer_simple<0, true>), grid, dim3(BL_NT), 0, s, f);
         else if (f.method == 0)
@@ -954,10 +946,8 @@ hipError_t craw_launch_debayer(bool to_inter, const KFrame &f, hipStream_t s)
             KLAUNCH((k_debayer_simple<1, true>), grid, dim3(BL_NT), 0, s, f);
         else
             KLAUNCH((k_debayer_simple<1, false>), grid, dim3(BL_NT), 0, s, f);
-        if ((e = hipGetLastError()) != hipSuccess)
-            return e;
-        return craw_launch_simple_stream(to_inter, f, s);
     }
+    return hipGetLastError();
 }
 
 hipError_t craw_launch_chroma_blur(const KFrame &f, hipStream_t s)

@@ -29,7 +29,6 @@
 
 #define RS_NW 4
 #define RS_NT (64 * RS_NW)
-#define RS_PF 8                 /* rows loaded ahead */
 #define RS_EPS 1e-5f
 
 static_assert(KG_RS_OUT == 116 && KG_RS_OUT_F == 112 && KG_RS_R % 8 == 0, "strip geometry");
@@ -128,15 +127,16 @@ KDEV float rs_colour(const RsState &s)
 /* One step: P is the parity of the step (the newest V row is odd for
  * P = 0, the strip starting on an even row). */
 template <bool INTER, int P>
-KDEV void rs_step(RsState &s, const KFrame &f, int vt, const uint32_t raw[2], bool out_row,
-                  bool out_cols, int X0)
+KDEV void rs_step(RsState &s, const KFrame &f, int vt, int ld0, bool out_row, bool out_cols,
+                  int X0)
 {
-    /* V row vt (its raw samples loaded ahead) */
+    /* V row vt */
     rs_shift(s.v);
     {
+        const uint16_t *row = f.src + (size_t)vt * f.src_pitch;
         const int sb = (vt & 1) * 2;
-        s.v[6].c[0] = kprep(raw[0], sb, f.prep);
-        s.v[6].c[1] = kprep(raw[1], sb + 1, f.prep);
+        s.v[6].c[0] = kprep(row[ld0], sb, f.prep);
+        s.v[6].c[1] = kprep(row[ld0 + 1], sb + 1, f.prep);
     }
 
     /* low pass, row vt - 1 */
@@ -256,31 +256,11 @@ KDEV void dev_rcd_stream(const KFrame &f, int strip)
         s.wp[0].c[0] = s.wp[0].c[1] = s.wp[1].c[0] = s.wp[1].c[1] = 0.0f;
         s.acc[0] = s.acc[1] = s.acc[2] = 0.0f;
 
-        /* ys is even and nsteps is even: steps alternate parity 0, 1.
-         * The rows enter RS_PF at a time, their loads issued together
-         * ahead of the steps; the last group's steps past nsteps are
-         * skipped (their loads stay inside the frame). */
+        /* ys is even and nsteps is even: steps alternate parity 0, 1 */
 #pragma unroll 1
-        for (int t0 = 0; t0 < nsteps; t0 += RS_PF) {
-            uint32_t raw[RS_PF][2];
-
-#pragma unroll
-            for (int k = 0; k < RS_PF; k++) {
-                const int vt = min(ys - 5 + t0 + k, f.H - 1);
-                const uint16_t *row = f.src + (size_t)vt * f.src_pitch;
-
-                raw[k][0] = row[ld0];
-                raw[k][1] = row[ld0 + 1];
-            }
-#pragma unroll
-            for (int k = 0; k < RS_PF; k += 2) {
-                const int t = t0 + k;
-
-                if (t >= nsteps)
-                    break;
-                rs_step<INTER, 0>(s, f, ys - 5 + t, raw[k], t >= 10, out_cols, X0);
-                rs_step<INTER, 1>(s, f, ys - 4 + t, raw[k + 1], t + 1 >= 10, out_cols, X0);
-            }
+        for (int t = 0; t < nsteps; t += 2) {
+            rs_step<INTER, 0>(s, f, ys - 5 + t, ld0, t >= 10, out_cols, X0);
+            rs_step<INTER, 1>(s, f, ys - 4 + t, ld0, t + 1 >= 10, out_cols, X0);
         }
     }
 }

@@ -118,8 +118,6 @@ struct KFrame {
 #define KG_RS_OUT_F 112         /* the same without the chroma stage */
 #define KG_RS_R   64            /* and rows */
 #define KG_RS_NW  4             /* strips per workgroup */
-#define KG_SS_OUT 120           /* streamed bilinear/MHC: output columns per strip */
-#define KG_SS_R   16            /* and rows */
 
 enum KStage {
     KSTAGE_DENOISE = 0,
@@ -214,8 +212,7 @@ static inline void craw_stage_geom(int stage, int method, int dn_geom, int *tw, 
  * filter reach (`reach` pixels) lies inside the frame form the interior,
  * developed by the streamed kernel; the ring of tiles around it by the
  * tiled one. Tile units [itx0, itx1) x [ity0, ity1), empty for a frame too
- * small to have one. RCD splits 32x16 tiles with a reach of 5, bilinear
- * and MHC 64x16 tiles with a reach of 2. */
+ * small to have one. RCD splits 32x16 tiles with a reach of 5. */
 struct KRcdSplit {
     int ntx, nty;
     int itx0, itx1, ity0, ity1;
@@ -250,14 +247,6 @@ __host__ __device__
 static inline KRcdSplit craw_rcd_split(int W, int H)
 {
     return craw_tile_split(W, H, KG_RCD_TW, KG_RCD_TH, 5);
-}
-
-#ifdef __HIPCC__
-__host__ __device__
-#endif
-static inline KRcdSplit craw_simple_split(int W, int H)
-{
-    return craw_tile_split(W, H, KG_SIMPLE_TW, KG_SIMPLE_TH, 2);
 }
 
 /* Ring tile i: the rows above the interior, the columns either side of it,
@@ -314,10 +303,6 @@ static inline int craw_stage_tiles(int stage, int W, int H, int method, int dn_g
 }
 
 hipError_t craw_launch_zero_u32(uint32_t *p, size_t n, hipStream_t s);
-/* The streamed bilinear/MHC over the frame's interior (the plain form's
- * debayer launches it after the ring of tiles). */
-hipError_t craw_launch_simple_stream(bool to_inter, const KFrame &f, hipStream_t s);
-int craw_simple_stream_groups(int W, int H);
 /* n words (a multiple of 4, 16-byte aligned) of hist to the device-visible
  * host buffer host, zeroing hist. */
 hipError_t craw_launch_hist_export(uint32_t *hist, uint32_t *host, size_t n, hipStream_t s);

@@ -127,13 +127,11 @@ def test_rcd_interior_and_ring(gpu_available, size, chroma):
 @pytest.mark.parametrize("chroma", [True, False])
 @pytest.mark.parametrize("size", [(250, 98), (520, 336), (610, 298), (96, 40), (194, 50),
                                   (192, 48), (1000, 200)])
-def test_simple_interior_and_ring(gpu_available, method, size, chroma):
-    """Bilinear and MHC develop the frame's interior with the streamed
-    kernel and the ring of 64x16 tiles around it with the tiled one, into
-    the chroma stage or to the output (AWB samples and colour): several
-    strips across and down with partial last strips, an interior ending
-    exactly at the filter's reach, frames a whole number of tiles, and
-    frames too small to have an interior."""
+def test_simple_sizes(gpu_available, method, size, chroma):
+    """Bilinear and MHC over 64x16 tiles, into the chroma stage or to the
+    output (AWB samples and colour): partial last tiles across and down,
+    a last tile narrower than the filter's reach, frames a whole number of
+    tiles, and frames of a single row of tiles."""
     w, h = size
     flags = capi.CRAW_PIPELINE_FLAG_AWB_HISTOGRAM
     if chroma:
