@@ -61,6 +61,24 @@ typedef struct CRAWImage {
     uint32_t         width, height;
 } CRAWImage;
 
+/* The noise fit of craw_noise_estimate() with the frame's measurement on
+ * the device: `src` is the R16_UNORM mosaic in device memory. This
+ * library's addition (the reference measures on the host) for callers
+ * whose frames are already on the GPU. The measurement is recorded on
+ * `stream` and waited for (the host fit needs it); scratch of 8 bytes a
+ * sample is allocated and freed by the call. The bins' counts and medians
+ * are the host measurement's; their sums of means are exact here (the
+ * host sums them in order in double), so the model agrees with
+ * craw_noise_estimate() to a few ulp. */
+struct CRAWProfile;
+struct CRAWNoiseEstimateParams;
+struct CRAWNoiseEstimate;
+CRAW_API int craw_noise_estimate_image(CRAWContext *ctx, CRAWStream stream,
+                                       const CRAWImage *src,
+                                       const struct CRAWProfile *profile,
+                                       const struct CRAWNoiseEstimateParams *params,
+                                       struct CRAWNoiseEstimate *out);
+
 CRAW_END_DECLS
 
 #endif /* CRAW_HIP_H */

@@ -259,6 +259,8 @@ SIGNATURES = [
     ("craw_noise_estimate", c_i32, [P(ctypes.c_uint16), c_u32, c_u32, ctypes.c_size_t,
                                      P(CRAWProfile), P(CRAWNoiseEstimateParams),
                                      P(CRAWNoiseEstimate)]),
+    ("craw_noise_estimate_image", c_i32, [vp, vp, P(CRAWImage), P(CRAWProfile),
+                                           P(CRAWNoiseEstimateParams), P(CRAWNoiseEstimate)]),
     ("craw_awb_params_init", None, [P(CRAWAwbParams)]),
     ("craw_awb_estimate", c_i32, [P(CRAWProfile), P(CRAWAwbHistogram), P(CRAWAwbParams),
                                    P(CRAWAwbEstimate)]),

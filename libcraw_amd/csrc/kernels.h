@@ -302,6 +302,31 @@ static inline int craw_stage_tiles(int stage, int W, int H, int method, int dn_g
     return planes * *ntx * ((h + th - 1) / th);
 }
 
+/* The noise fit's device measurement (k_noise.hip): the frame's R16 samples,
+ * per site black and range in double (the range formed in float), and the
+ * scratch it fills: a key per plane sample, and per bin the count, the
+ * fixed-point sum of means in KNZ_LIMBS 32-bit limbs (units of 2^-149,
+ * wide enough for any finite float and 2^32 samples), a KNZ_D-digit histogram row and the selection state (prefix,
+ * rank; after the last level the median's bits). */
+#define KNZ_MAX_BINS 512
+#define KNZ_LIMBS 9
+#define KNZ_D 2048
+struct KNoise {
+    const uint16_t *src;
+    size_t pitch;                /* uint16 elements */
+    int w2, h2;
+    double black[4], range[4];
+    uint32_t nb_bins;
+    float max_level;
+    uint64_t *keys;              /* 4 planes of w2 x h2 */
+    uint32_t *count;
+    unsigned long long *limbs;
+    uint32_t *hist;
+    uint32_t *sel;
+};
+
+hipError_t craw_launch_noise_measure(const KNoise &n, hipStream_t s);
+
 hipError_t craw_launch_zero_u32(uint32_t *p, size_t n, hipStream_t s);
 /* n words (a multiple of 4, 16-byte aligned) of hist to the device-visible
  * host buffer host, zeroing hist. */
