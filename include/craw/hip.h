@@ -65,8 +65,10 @@ typedef struct CRAWImage {
  * the device: `src` is the R16_UNORM mosaic in device memory. This
  * library's addition (the reference measures on the host) for callers
  * whose frames are already on the GPU. The measurement is recorded on
- * `stream` and waited for (the host fit needs it); scratch of 8 bytes a
- * sample is allocated and freed by the call. The bins' counts and medians
+ * `stream` and waited for (the host fit needs it). Its scratch, 8 bytes a
+ * sample plus 8 KiB a bin, belongs to the context: allocated by the first
+ * call that needs more, kept until craw_context_free(); calls on one
+ * context take turns on it (a lock). The bins' counts and medians
  * are the host measurement's; their sums of means are exact here (the
  * host sums them in order in double), so the model agrees with
  * craw_noise_estimate() to a few ulp. */

@@ -19,6 +19,8 @@ ROOT = os.path.dirname(HERE)
 
 
 def build(jobs=8):
-    """Compile libcraw.so for gfx950 (always recompiles what changed)."""
-    subprocess.check_call(["make", "-C", os.path.join(HERE, "csrc"), "-j%d" % jobs])
+    """Compile libcraw.so for gfx950 and the bench driver
+    (bench/craw_bench) linked against it; recompiles what changed."""
+    subprocess.check_call(["make", "-C", os.path.join(HERE, "csrc"), "-j%d" % jobs, "all",
+                           "bench"])
     return os.path.join(HERE, "libcraw.so")

@@ -71,6 +71,7 @@ CRAW_API int craw_context_create(CRAWContext **out, const CRAWHipInitParams *par
         return CRAW_ERROR(ENOTSUP);
     }
 
+    pthread_mutex_init(&ctx->noise_lock, NULL);
     craw_log(ctx, CRAW_LOG_VERBOSE, "Context on device %d (%s, %d CUs)\n", ctx->device,
              ctx->arch, ctx->cu_count);
     *out = ctx;
@@ -81,6 +82,11 @@ CRAW_API void craw_context_free(CRAWContext **ctx)
 {
     if (!ctx || !*ctx)
         return;
+    if ((*ctx)->noise_scratch) {
+        CRAW_IGNORE(hipSetDevice((*ctx)->device));
+        CRAW_IGNORE(hipFree((*ctx)->noise_scratch));
+    }
+    pthread_mutex_destroy(&(*ctx)->noise_lock);
     free(*ctx);
     *ctx = NULL;
 }

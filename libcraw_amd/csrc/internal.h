@@ -5,6 +5,8 @@
 #ifndef CRAW_INTERNAL_H
 #define CRAW_INTERNAL_H
 
+#include <pthread.h>
+
 #include <hip/hip_runtime.h>
 
 #include <craw/pipeline.h>
@@ -17,6 +19,13 @@ struct CRAWContext {
     CRAWContextOptions opts;
     char arch[64];
     int cu_count;
+
+    /* craw_noise_estimate_image()'s device scratch, grown to the largest
+     * frame measured and kept until the context is freed; the lock makes
+     * the measurements on one context take turns. */
+    pthread_mutex_t noise_lock;
+    void *noise_scratch;
+    size_t noise_scratch_size;
 };
 
 #define CRAW_MAX_STAGES 12

@@ -2,19 +2,26 @@
  * libcraw for MI355X: the noise fit's measurement of a frame in device
  * memory (lib/noise.c:394-455 of cyanreg/libcraw measures it on the host).
  *
- * k_noise_measure: one workgroup per 64x16 samples of one CFA plane. The
+ * k_noise_measure: one workgroup per 64x60 samples of one CFA plane. The
  * plane is normalised in double and stored as float in LDS with the 9x9
- * mean's reach, and every interior sample gets the reference's high-pass
- * residual and 9x9 mean (summed in raster order) and so its bin. It writes
- * the sample's key, (bin << 32) | bits(|r|) (all ones for a sample the
- * measurement skips), and adds to its bin's count, to the bin's exact sum
- * of means and to the first-level histogram of |r|, each privatised in LDS
- * and flushed with global atomics:
+ * mean's reach; each lane walks one column down 15 rows, holding the 9x9
+ * window in registers (one new row of nine a sample), and gives every
+ * interior sample the reference's high-pass residual and 9x9 mean (summed
+ * in raster order) and so its bin. It writes the sample's key,
+ * (bin << 32) | bits(|r|) (all ones for a sample the measurement skips),
+ * and adds to its bin's count, to the bin's exact sum of means and to the
+ * first-level histogram of |r|:
  *   - a mean is 24 mantissa bits times 2^(e - 150); added at bit e - 1 of
  *     a fixed point in units of 2^-149 split in 32-bit limbs, each
- *     accumulated in 64 bits, the sum is exact in any order;
+ *     accumulated in 64 bits, the sum is exact in any order. A lane sums
+ *     its run of samples of one bin (and two neighbouring limbs) in
+ *     registers and adds the run to the workgroup's tables in LDS when the
+ *     bin changes; the tables go to memory with global atomics at the end;
  *   - the histogram is keyed by bits 30..20 of |r| (sign 0), in a 4096-entry
- *     LDS hash of (bin, digit) with a global fallback.
+ *     LDS hash whose entries hold (bin, digit) << 12 | count (a tile's 3840
+ *     samples cannot overflow the count; digits of finite |r| stay below
+ *     2040, so the all-ones word is free to mark an empty entry), with a
+ *     global fallback after 32 probes.
  * The median of each bin (rank n / 2) is then found by radix selection:
  * k_noise_select picks each bin's digit holding the rank (and clears the
  * row for the next level), k_noise_level histograms the next digit (bits
@@ -24,57 +31,84 @@
 #include "kernels.h"
 
 #define NZ_TW 64
-#define NZ_TH 16
+#define NZ_WR 15
+#define NZ_TH (4 * NZ_WR)
 #define NZ_R  4
 #define NZ_NT 256
 #define NZ_LW (NZ_TW + 2 * NZ_R)
 #define NZ_LH (NZ_TH + 2 * NZ_R)
 #define NZ_HASH 4096
 #define NZ_PROBES 32
-#define NZ_NONE 0xffffffffu
+#define NZ_EMPTY 0xffffffffu
 
 struct SmNoise {
     float plane[NZ_LH][NZ_LW];
-    uint32_t count[KNZ_MAX_BINS];
-    unsigned long long limb[KNZ_MAX_BINS][KNZ_LIMBS];
-    uint32_t hkey[NZ_HASH];
-    uint32_t hcnt[NZ_HASH];
+    uint32_t hash[NZ_HASH];
 };
 
-KDEV void nz_hist_add(SmNoise &sm, const KNoise &n, uint32_t key)
+/* The per-bin tables after SmNoise: limbs (8-byte aligned) then counts. */
+static size_t nz_dyn_bytes(uint32_t nb_bins)
+{
+    return (size_t)nb_bins * (KNZ_LIMBS * sizeof(unsigned long long) + sizeof(uint32_t));
+}
+
+KDEV void nz_hist_add(uint32_t *hash, uint32_t *ghist, uint32_t key)
 {
     uint32_t h = (key * 0x9e3779b1u) >> 20;
 
+#pragma unroll 1
     for (int probe = 0; probe < NZ_PROBES; probe++) {
-        const uint32_t k = atomicCAS(&sm.hkey[h], NZ_NONE, key);
+        uint32_t v = __hip_atomic_load(&hash[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 
-        if (k == NZ_NONE || k == key) {
-            atomicAdd(&sm.hcnt[h], 1u);
+        if (v == NZ_EMPTY) {
+            v = atomicCAS(&hash[h], NZ_EMPTY, (key << 12) | 1u);
+            if (v == NZ_EMPTY)
+                return;
+        }
+        if ((v >> 12) == key) {
+            atomicAdd(&hash[h], 1u);
             return;
         }
         h = (h + 1) & (NZ_HASH - 1);
     }
-    atomicAdd(&n.hist[(key >> 11) * KNZ_D + (key & 2047u)], 1u);
+    atomicAdd(&ghist[(key >> 11) * KNZ_D + (key & 2047u)], 1u);
+}
+
+/* A run's count and limb sums l, l + 1, l + 2 into the workgroup's tables.
+ * A sum is non-zero only for a limb some sample of the run reached, and
+ * every sample's limbs lie below KNZ_LIMBS. */
+KDEV void nz_run_flush(uint32_t *count, unsigned long long *limb, int b, uint32_t cnt, int l,
+                       unsigned long long a0, unsigned long long a1, unsigned long long a2)
+{
+    unsigned long long *lb = limb + (size_t)b * KNZ_LIMBS + l;
+
+    atomicAdd(&count[b], cnt);
+    if (a0)
+        atomicAdd(lb, a0);
+    if (a1)
+        atomicAdd(lb + 1, a1);
+    if (a2)
+        atomicAdd(lb + 2, a2);
 }
 
 __global__ __launch_bounds__(NZ_NT) void k_noise_measure(KNoise n)
 {
     __shared__ SmNoise sm;
-    const int tid = threadIdx.x;
+    extern __shared__ unsigned long long nz_dyn[];
+    unsigned long long *limb = nz_dyn;
+    uint32_t *count = (uint32_t *)(nz_dyn + (size_t)n.nb_bins * KNZ_LIMBS);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int pl = blockIdx.z, px = pl & 1, py = pl >> 1;
     const int i0 = blockIdx.x * NZ_TW, k0 = blockIdx.y * NZ_TH;
     const double black = n.black[pl], range = n.range[pl];
     uint64_t *keys = n.keys + (size_t)pl * n.w2 * n.h2;
 
-    for (int b = tid; b < (int)n.nb_bins; b += NZ_NT) {
-        sm.count[b] = 0u;
-        for (int l = 0; l < KNZ_LIMBS; l++)
-            sm.limb[b][l] = 0ull;
-    }
-    for (int i = tid; i < NZ_HASH; i += NZ_NT) {
-        sm.hkey[i] = NZ_NONE;
-        sm.hcnt[i] = 0u;
-    }
+    for (int b = tid; b < (int)(n.nb_bins * KNZ_LIMBS); b += NZ_NT)
+        limb[b] = 0ull;
+    for (int b = tid; b < (int)n.nb_bins; b += NZ_NT)
+        count[b] = 0u;
+    for (int i = tid; i < NZ_HASH; i += NZ_NT)
+        sm.hash[i] = NZ_EMPTY;
     /* the plane over the tile and its reach; positions past the plane
      * (which no measured sample reaches) repeat its edge */
     for (int i = tid; i < NZ_LW * NZ_LH; i += NZ_NT) {
@@ -87,64 +121,101 @@ __global__ __launch_bounds__(NZ_NT) void k_noise_measure(KNoise n)
     }
     __syncthreads();
 
-    for (int q = tid; q < NZ_TW * NZ_TH; q += NZ_NT) {
-        const int lx = q % NZ_TW, ly = q / NZ_TW;
-        const int i = i0 + lx, k = k0 + ly;
-        uint64_t key = ~0ull;
+    {
+        const int i = i0 + lane, kb = k0 + wave * NZ_WR, cyb = wave * NZ_WR;
+        const bool col_in = i < n.w2, col_mid = i >= NZ_R && i < n.w2 - NZ_R;
+        int run_bin = -1, run_l = 0;
+        uint32_t run_n = 0u;
+        unsigned long long a0 = 0ull, a1 = 0ull, a2 = 0ull;
+        float win[2 * NZ_R + 1][2 * NZ_R + 1];
 
-        if (i >= n.w2 || k >= n.h2)
-            continue;
-        if (i >= NZ_R && i < n.w2 - NZ_R && k >= NZ_R && k < n.h2 - NZ_R) {
-            const float(&p)[NZ_LH][NZ_LW] = sm.plane;
-            const int cx = lx + NZ_R, cy = ly + NZ_R;
-            float r, mean = 0.0f;
-
-            /* lib/noise.c:415-426: [1 -2 1; -2 4 -2; 1 -2 1] / 6, and the
-             * 9x9 mean in raster order */
-            r = (p[cy - 1][cx - 1] + p[cy - 1][cx + 1] + p[cy + 1][cx - 1] + p[cy + 1][cx + 1] -
-                 2.0f * (p[cy - 1][cx] + p[cy + 1][cx] + p[cy][cx - 1] + p[cy][cx + 1]) +
-                 4.0f * p[cy][cx]) * (1.0f / 6.0f);
-            for (int dy = -NZ_R; dy <= NZ_R; dy++)
+        /* win[j][d]: plane row (sample row - 4 + j), column (lane - 4 + d) */
 #pragma unroll
-                for (int dx = -NZ_R; dx <= NZ_R; dx++)
-                    mean += p[cy + dy][cx + dx];
-            mean *= 1.0f / ((2 * NZ_R + 1) * (2 * NZ_R + 1));
+        for (int j = 0; j < 2 * NZ_R; j++)
+#pragma unroll
+            for (int d = 0; d <= 2 * NZ_R; d++)
+                win[j][d] = sm.plane[cyb + j][lane + d];
 
-            if (mean > 0.0f && !(mean > n.max_level)) {
-                int bin = (int)(sqrtf(mean) * (float)n.nb_bins);
-                const uint32_t rb = __float_as_uint(fabsf(r));
-                const uint32_t mb = __float_as_uint(mean);
-                const uint32_t e = mb >> 23;
-                const uint64_t m24 = (mb & 0x7fffffu) | (e ? 0x800000u : 0u);
-                const int bit = (e ? (int)e : 1) - 1;
-                const uint64_t v = m24 << (bit & 31);
+#pragma unroll
+        for (int s = 0; s < NZ_WR; s++) {
+            const int k = kb + s;
+            uint64_t key = ~0ull;
 
-                bin = bin < 0 ? 0 : (bin >= (int)n.nb_bins ? (int)n.nb_bins - 1 : bin);
-                key = ((uint64_t)bin << 32) | rb;
-                atomicAdd(&sm.count[bin], 1u);
-                atomicAdd(&sm.limb[bin][bit >> 5], v & 0xffffffffull);
-                if (v >> 32)
-                    atomicAdd(&sm.limb[bin][(bit >> 5) + 1], v >> 32);
-                nz_hist_add(sm, n, ((uint32_t)bin << 11) | (rb >> 20));
+#pragma unroll
+            for (int d = 0; d <= 2 * NZ_R; d++)
+                win[2 * NZ_R][d] = sm.plane[cyb + s + 2 * NZ_R][lane + d];
+
+            if (col_mid && k >= NZ_R && k < n.h2 - NZ_R) {
+                float r, mean = 0.0f;
+
+                /* lib/noise.c:415-426: [1 -2 1; -2 4 -2; 1 -2 1] / 6, and the
+                 * 9x9 mean in raster order */
+                r = (win[3][3] + win[3][5] + win[5][3] + win[5][5] -
+                     2.0f * (win[3][4] + win[5][4] + win[4][3] + win[4][5]) + 4.0f * win[4][4]) *
+                    (1.0f / 6.0f);
+#pragma unroll
+                for (int dy = 0; dy <= 2 * NZ_R; dy++)
+#pragma unroll
+                    for (int dx = 0; dx <= 2 * NZ_R; dx++)
+                        mean += win[dy][dx];
+                mean *= 1.0f / ((2 * NZ_R + 1) * (2 * NZ_R + 1));
+
+                if (mean > 0.0f && !(mean > n.max_level)) {
+                    int bin = (int)(sqrtf(mean) * (float)n.nb_bins);
+                    const uint32_t rb = __float_as_uint(fabsf(r));
+                    const uint32_t mb = __float_as_uint(mean);
+                    const uint32_t e = mb >> 23;
+                    const uint64_t m24 = (mb & 0x7fffffu) | (e ? 0x800000u : 0u);
+                    const int bit = (e ? (int)e : 1) - 1, l = bit >> 5;
+                    const uint64_t v = m24 << (bit & 31);
+                    const unsigned long long lo = v & 0xffffffffull, hi = v >> 32;
+                    bool up;
+
+                    bin = bin < 0 ? 0 : (bin >= (int)n.nb_bins ? (int)n.nb_bins - 1 : bin);
+                    key = ((uint64_t)bin << 32) | rb;
+                    if (bin != run_bin || (l != run_l && l != run_l + 1)) {
+                        if (run_n)
+                            nz_run_flush(count, limb, run_bin, run_n, run_l, a0, a1, a2);
+                        run_bin = bin;
+                        run_l = l;
+                        run_n = 0u;
+                        a0 = a1 = a2 = 0ull;
+                    }
+                    up = l != run_l;
+                    run_n++;
+                    a0 += up ? 0ull : lo;
+                    a1 += up ? lo : hi;
+                    a2 += up ? hi : 0ull;
+                    nz_hist_add(sm.hash, n.hist, ((uint32_t)bin << 11) | (rb >> 20));
+                }
             }
+            if (col_in && k < n.h2)
+                keys[(size_t)k * n.w2 + i] = key;
+
+#pragma unroll
+            for (int j = 0; j < 2 * NZ_R; j++)
+#pragma unroll
+                for (int d = 0; d <= 2 * NZ_R; d++)
+                    win[j][d] = win[j + 1][d];
         }
-        keys[(size_t)k * n.w2 + i] = key;
+        if (run_n)
+            nz_run_flush(count, limb, run_bin, run_n, run_l, a0, a1, a2);
     }
     __syncthreads();
 
     for (int b = tid; b < (int)n.nb_bins; b += NZ_NT) {
-        if (!sm.count[b])
+        if (!count[b])
             continue;
-        atomicAdd(&n.count[b], sm.count[b]);
+        atomicAdd(&n.count[b], count[b]);
         for (int l = 0; l < KNZ_LIMBS; l++)
-            if (sm.limb[b][l])
-                atomicAdd(&n.limbs[(size_t)b * KNZ_LIMBS + l], sm.limb[b][l]);
+            if (limb[(size_t)b * KNZ_LIMBS + l])
+                atomicAdd(&n.limbs[(size_t)b * KNZ_LIMBS + l], limb[(size_t)b * KNZ_LIMBS + l]);
     }
     for (int i = tid; i < NZ_HASH; i += NZ_NT) {
-        const uint32_t key = sm.hkey[i];
+        const uint32_t v = sm.hash[i];
 
-        if (key != NZ_NONE)
-            atomicAdd(&n.hist[(key >> 11) * KNZ_D + (key & 2047u)], sm.hcnt[i]);
+        if (v != NZ_EMPTY)
+            atomicAdd(&n.hist[(v >> 23) * KNZ_D + ((v >> 12) & 2047u)], v & 4095u);
     }
 }
 
@@ -225,7 +296,7 @@ hipError_t craw_launch_noise_measure(const KNoise &n, hipStream_t s)
     hipError_t e;
 
     KLAUNCH(k_noise_measure, dim3((n.w2 + NZ_TW - 1) / NZ_TW, (n.h2 + NZ_TH - 1) / NZ_TH, 4),
-            dim3(NZ_NT), 0, s, n);
+            dim3(NZ_NT), nz_dyn_bytes(n.nb_bins), s, n);
     if ((e = hipGetLastError()) != hipSuccess)
         return e;
     for (int level = 1; level <= 3; level++) {

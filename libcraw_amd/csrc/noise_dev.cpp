@@ -94,8 +94,20 @@ extern "C" CRAW_API int craw_noise_estimate_image(CRAWContext *ctx, CRAWStream s
     off_hist = off_limbs + (size_t)nb_bins * KNZ_LIMBS * 8;
     off_sel = off_hist + (size_t)nb_bins * KNZ_D * 4;
     total = off_sel + (size_t)nb_bins * 8;
-    if (hipMalloc((void **)&scratch, total) != hipSuccess)
-        return CRAW_ERROR(ENOMEM);
+    pthread_mutex_lock(&ctx->noise_lock);
+    if (ctx->noise_scratch_size < total) {
+        /* the previous measurement's work must be done before its memory
+         * goes; the stream it ran on was synchronised before it returned */
+        CRAW_IGNORE(hipFree(ctx->noise_scratch));
+        ctx->noise_scratch = NULL;
+        ctx->noise_scratch_size = 0;
+        if (hipMalloc(&ctx->noise_scratch, total) != hipSuccess) {
+            pthread_mutex_unlock(&ctx->noise_lock);
+            return CRAW_ERROR(ENOMEM);
+        }
+        ctx->noise_scratch_size = total;
+    }
+    scratch = (char *)ctx->noise_scratch;
     n.keys = (uint64_t *)scratch;
     n.count = (uint32_t *)(scratch + off_count);
     n.limbs = (unsigned long long *)(scratch + off_limbs);
@@ -134,7 +146,7 @@ extern "C" CRAW_API int craw_noise_estimate_image(CRAWContext *ctx, CRAWStream s
     ret = craw_noise_fit(bins, nb_bins, min_samples, profile, out);
 
 done:
-    CRAW_IGNORE(hipFree(scratch));
+    pthread_mutex_unlock(&ctx->noise_lock);
     free(h_count);
     free(h_sel);
     free(h_limbs);

@@ -20,6 +20,8 @@
  * ln(1 + samples), damped and clamped to half a log unit a step.
  */
 #include <math.h>
+#include <pthread.h>
+#include <stdbool.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -70,12 +72,6 @@ int craw_noise_check(uint32_t width, uint32_t height, size_t stride, const CRAWP
     return 0;
 }
 
-static int cmp_double(const void *a, const void *b)
-{
-    const double x = *(const double *)a, y = *(const double *)b;
-    return x < y ? -1 : (x > y ? 1 : 0);
-}
-
 /* The element of rank k (0-based, ascending) of v[0..n), reordering v:
  * Hoare's selection with a median-of-three pivot. */
 static double select_rank(double *v, size_t n, size_t k)
@@ -113,42 +109,105 @@ static double select_rank(double *v, size_t n, size_t k)
     return v[k];
 }
 
-/* Least trimmed squares seed: false when no pair gives a > 0, b >= 0. */
+/* Least trimmed squares seed over every pair of bins (i < j): the line
+ * through the two, kept when a > 0 and b >= 0, costed by the sum of the
+ * smallest two thirds of the squared log residuals in ascending order; the
+ * first pair of least cost in (i, j) order wins. The pairs' costs are
+ * independent, so rows of i are costed on several threads into a table and
+ * the table is then scanned in order: the same pair and the same bits as a
+ * single loop. */
+struct LtsJob {
+    const double *m, *v, *lv;
+    int nb, keep, first, step;
+    double *cost;               /* nb x nb, NAN for a pair not kept */
+};
+
+static void lts_rows(const struct LtsJob *jb)
+{
+    const int nb = jb->nb;
+    double *sq = malloc((size_t)nb * sizeof(*sq));
+
+    for (int i = jb->first; i < nb; i += jb->step) {
+        for (int j = i + 1; j < nb; j++) {
+            double a, b, cost = 0.0;
+
+            jb->cost[(size_t)i * nb + j] = NAN;
+            if (!sq || jb->m[j] - jb->m[i] < 1e-5)
+                continue;
+            a = (jb->v[j] - jb->v[i]) / (jb->m[j] - jb->m[i]);
+            b = jb->v[i] - a * jb->m[i];
+            if (!(a > 0.0) || b < 0.0)
+                continue;
+            /* an insertion sort: any sort gives the same ascending array */
+            for (int k = 0; k < nb; k++) {
+                const double d = jb->lv[k] - log(a * jb->m[k] + b), x = d * d;
+                int q = k;
+
+                for (; q > 0 && sq[q - 1] > x; q--)
+                    sq[q] = sq[q - 1];
+                sq[q] = x;
+            }
+            for (int k = 0; k < jb->keep; k++)
+                cost += sq[k];
+            jb->cost[(size_t)i * nb + j] = cost;
+        }
+    }
+    free(sq);
+}
+
+static void *lts_thread(void *arg)
+{
+    lts_rows((const struct LtsJob *)arg);
+    return NULL;
+}
+
+#define LTS_THREADS 8
+
 static int lts_seed(const double *m, const double *v, int nb, double *a_out, double *b_out,
                     double *scale)
 {
     const int keep = (nb * 2) / 3 + 1;
+    const int nt = nb >= 48 ? LTS_THREADS : 1;
     double best = INFINITY, ba = 0.0, bb = 0.0;
-    double *sq = malloc((size_t)nb * sizeof(*sq));
+    double *lv = malloc((size_t)nb * sizeof(*lv));
+    double *cost = malloc((size_t)nb * nb * sizeof(*cost));
+    struct LtsJob jobs[LTS_THREADS];
+    pthread_t th[LTS_THREADS];
+    bool started[LTS_THREADS] = { false };
 
-    if (!sq)
+    if (!lv || !cost) {
+        free(lv);
+        free(cost);
         return 0;
+    }
+    /* ln v once (the same value every pair would compute) */
+    for (int k = 0; k < nb; k++)
+        lv[k] = log(v[k]);
+    for (int t = 0; t < nt; t++) {
+        jobs[t] = (struct LtsJob) { m, v, lv, nb, keep, t, nt, cost };
+        if (t > 0)
+            started[t] = pthread_create(&th[t], NULL, lts_thread, &jobs[t]) == 0;
+    }
+    lts_rows(&jobs[0]);
+    for (int t = 1; t < nt; t++) {
+        if (started[t])
+            pthread_join(th[t], NULL);
+        else
+            lts_rows(&jobs[t]);
+    }
     for (int i = 0; i < nb; i++) {
         for (int j = i + 1; j < nb; j++) {
-            double a, b, cost = 0.0;
+            const double c = cost[(size_t)i * nb + j];
 
-            if (m[j] - m[i] < 1e-5)
-                continue;
-            a = (v[j] - v[i]) / (m[j] - m[i]);
-            b = v[i] - a * m[i];
-            if (!(a > 0.0) || b < 0.0)
-                continue;
-            for (int k = 0; k < nb; k++) {
-                const double d = log(v[k]) - log(a * m[k] + b);
-                sq[k] = d * d;
-            }
-            /* the trimmed sum runs over the smallest in ascending order */
-            qsort(sq, (size_t)nb, sizeof(*sq), cmp_double);
-            for (int k = 0; k < keep; k++)
-                cost += sq[k];
-            if (cost < best) {
-                best = cost;
-                ba = a;
-                bb = b;
+            if (c < best) {
+                best = c;
+                ba = (v[j] - v[i]) / (m[j] - m[i]);
+                bb = v[i] - ba * m[i];
             }
         }
     }
-    free(sq);
+    free(lv);
+    free(cost);
     if (!(ba > 0.0))
         return 0;
     *a_out = ba;

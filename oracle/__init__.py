@@ -117,6 +117,13 @@ class Mode:
                                         ctypes.c_float(strength))
         return ret, rgb
 
+    def false_color(self, rgb, passes):
+        rgb = np.ascontiguousarray(rgb, self.dtype).copy()
+        h, w, _ = rgb.shape
+        ret = self._f("false_color")(_ptr(rgb), ctypes.c_uint32(w), ctypes.c_uint32(h),
+                                     ctypes.c_uint32(passes))
+        return ret, rgb
+
     def color(self, rgb, fp, out_space):
         rgb = np.ascontiguousarray(rgb, self.dtype).copy()
         h, w, _ = rgb.shape

@@ -34,6 +34,10 @@
  *   temporal: static-scene variance tends to (1 - a) / (1 + a), state
  *       rules, first frame equals the spatial filter;
  *   chroma: G untouched, flat colour preserved, blotch reduction;
+ *   false colour: each sweep against scipy's median filter and a double
+ *       box mean (mirror mode), G and alpha untouched, and the reference's
+ *       spike known answers (tests/test_pipeline.c:3994-4050: a site spike's
+ *       colour at most 65% after one pass through bilinear, 10% after five);
  *   AWB histogram: oracle/_ref (the reference's own lib/awb.c, built from
  *       its sources) on random and structured inputs, plus test_awb.c's
  *       flat-neutral, skip and partial-block cases;
@@ -88,6 +92,7 @@ void oracle_gamut_params_d(double thr[3], double scl[3], double *pwr, double *in
                             float alpha_max, const real *prev, real *next,           \
                             uint16_t *out, size_t out_stride, real *out_level);      \
     int P##chroma_denoise(real *rgb, uint32_t w, uint32_t h, float strength);        \
+    int P##false_color(real *rgb, uint32_t w, uint32_t h, uint32_t passes);          \
     int P##color(real *rgb, uint32_t w, uint32_t h, const CRAWFrameParams *fp,       \
                  int out_space);                                                     \
     void P##gamut_compress(real *rgb, size_t n);                                     \

@@ -835,6 +835,94 @@ fail:
     return ret;
 }
 
+/* -------------------------------------------------------- false colour */
+
+/* lib/ref/ref_false_color.c:27-41: the median of nine as a selection, by
+ * an insertion sort (so the result is one of the nine, bit for bit). */
+static real o_fc_median9(real v[9])
+{
+    for (int i = 1; i < 9; i++) {
+        const real x = v[i];
+        int j = i;
+
+        for (; j > 0 && v[j - 1] > x; j--)
+            v[j] = v[j - 1];
+        v[j] = x;
+    }
+    return v[4];
+}
+
+/* lib/ref/ref_false_color.c:46-54: the nine summed in stencil order (top
+ * row left to right first), then scaled by the nearest 1/9. */
+static real o_fc_mean9(const real v[9])
+{
+    real s = v[0];
+
+    for (int i = 1; i < 9; i++)
+        s += v[i];
+    return s * K(1.0f / 9.0f, 1.0 / 9.0);
+}
+
+/* lib/ref/ref_false_color.c:56-134: `passes` passes, each a 3x3 median
+ * sweep then a 3x3 box sweep over the whole frame, on the two colour
+ * differences R - G and B - G (borders mirrored, one reflection); G and
+ * alpha stay as they are and R, B become G plus the filtered differences.
+ * Held here as two planes; every sweep reads the whole previous one. */
+int PFX(false_color)(real *rgb, uint32_t width, uint32_t height, uint32_t passes)
+{
+    const int w = (int)width, h = (int)height;
+    const size_t n = (size_t)width * height;
+    real *a = NULL, *b = NULL;
+
+    if (!rgb || width < 2 || height < 2)
+        return -EINVAL;
+    if (passes < 1 || passes > CRAW_FALSE_COLOR_MAX_PASSES)
+        return -EINVAL;
+    a = malloc(2 * n * sizeof(*a));
+    b = malloc(2 * n * sizeof(*b));
+    if (!a || !b) {
+        free(a);
+        free(b);
+        return -ENOMEM;
+    }
+    for (size_t i = 0; i < n; i++) {
+        a[i] = rgb[4 * i + 0] - rgb[4 * i + 1];
+        a[n + i] = rgb[4 * i + 2] - rgb[4 * i + 1];
+    }
+    for (uint32_t sweep = 0; sweep < 2 * passes; sweep++) {
+        const int blur = sweep & 1;
+        real *t;
+
+        OMP_FOR
+        for (int y = 0; y < h; y++) {
+            for (int x = 0; x < w; x++) {
+                for (int c = 0; c < 2; c++) {
+                    const real *src = a + c * n;
+                    real v[9];
+
+                    for (int j = 0; j < 3; j++)
+                        for (int i = 0; i < 3; i++)
+                            v[3 * j + i] = src[(size_t)o_mirror(y - 1 + j, h) * w +
+                                               o_mirror(x - 1 + i, w)];
+                    b[c * n + (size_t)y * w + x] = blur ? o_fc_mean9(v) : o_fc_median9(v);
+                }
+            }
+        }
+        t = a;
+        a = b;
+        b = t;
+    }
+    for (size_t i = 0; i < n; i++) {
+        const real g = rgb[4 * i + 1];
+
+        rgb[4 * i + 0] = g + a[i];
+        rgb[4 * i + 2] = g + a[n + i];
+    }
+    free(a);
+    free(b);
+    return 0;
+}
+
 /* -------------------------------------------------------------- colour */
 
 /* gamut.c:66-92: ACES 1.3 reference gamut compression of one AP1 triple. */
@@ -1092,6 +1180,12 @@ int PFX(process)(const uint16_t *bayer, uint32_t width, uint32_t height, size_t 
     ret = PFX(debayer_wb)(bayer, width, height, stride, fp->profile, par.wb, pp->debayer, out);
     if (ret < 0)
         goto done;
+    /* ref_color.c:178-185: the false colour on the intermediate first */
+    if ((pp->flags & CRAW_PIPELINE_FLAG_FALSE_COLOR) && fp->false_color_passes > 0) {
+        ret = PFX(false_color)(out, width, height, fp->false_color_passes);
+        if (ret < 0)
+            goto done;
+    }
     if ((pp->flags & CRAW_PIPELINE_FLAG_CHROMA_DENOISE) && fp->denoise_chroma_strength > 0.0f) {
         ret = PFX(chroma_denoise)(out, width, height, fp->denoise_chroma_strength);
         if (ret < 0)

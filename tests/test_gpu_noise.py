@@ -3,8 +3,9 @@
 (craw_noise_estimate, itself bit-exact against the reference's noise.c):
 the same bins, counts and medians (the selection is exact), sums of means
 exact on the device, so the model agrees to a few ulp. Frames of several
-scenes, noise levels, sizes (whole and partial 64x16 tiles, the smallest
-frame) and options; error paths."""
+scenes, noise levels, sizes (whole and partial 64x60 tiles, the smallest
+frame), the most bins, a flat region (a tile's samples all on one key) and
+options, on one context whose scratch the calls share; error paths."""
 import ctypes
 
 import numpy as np
@@ -59,7 +60,7 @@ def ulps(a, b):
     return abs(int(ia) - int(ib))
 
 
-@pytest.mark.parametrize("case", range(5))
+@pytest.mark.parametrize("case", range(7))
 def test_device_measurement_matches_host(gpu_available, case):
     w, h, a, b, kw, pad = [
         (1024, 768, 2e-4, 4e-6, {}, 0),
@@ -67,9 +68,13 @@ def test_device_measurement_matches_host(gpu_available, case):
         (2048, 1080, 4e-4, 1.6e-5, {"nb_bins": 256}, 0),
         (640, 480, 1e-4, 2e-6, {"nb_bins": 64, "min_bin_samples": 500, "max_level": 0.6}, 8),
         (96, 64, 4e-4, 1.6e-5, {"min_bin_samples": 50}, 0),
+        (1536, 1024, 4e-4, 1.6e-5, {"nb_bins": 512, "min_bin_samples": 200}, 0),
+        (1280, 720, 2e-4, 4e-6, {"min_bin_samples": 2000}, 0),
     ][case]
     prof = scene.make_profile()
     mosaic = scene.make_frame(w, h, seed=70 + case, a=a, b=b)
+    if case == 6:
+        mosaic[:, :512] = 1500     # flat: every residual 0, one key a tile
     pr = params(**kw)
     rh, eh = host_estimate(mosaic, prof, pr)
     rd, ed = device_estimate(mosaic, prof, pr, pitch_pad=pad)
