@@ -15,16 +15,19 @@
  *     -> [non-local means on the four CFA planes, variance stabilised]
  *     -> black/white normalisation, white balance, clip at 1.0
  *     -> bilinear, MHC or RCD debayer to camera RGB
+ *     -> [false colour suppression: 1 to 5 passes of a 3x3 median and a
+ *         3x3 box on R - G and B - G]
  *     -> [chroma denoise of the quarter resolution chroma band]
  *     -> [auto white balance histogram]
  *     -> camera matrix, gain, exposure, XYZ to the output space
  *     -> [ACES 1.3 gamut compression, ACEScg output only]
  *   dst (RGBA16F or RGBA32F)
  *
- * The reference's lens, false colour, highlight, defringe and sharpening
- * stages are not part of this library; their pipeline flags make
- * craw_pipeline_create() fail with CRAW_ERROR(ENOTSUP) and their frame
- * fields are ignored.
+ * The reference's lens, highlight, defringe and sharpening stages are not
+ * part of this library; their pipeline flags make craw_pipeline_create()
+ * fail with CRAW_ERROR(ENOTSUP) and their frame fields are ignored. The
+ * false colour stage is in the plain form only (craw_batch_create()
+ * refuses its flag with CRAW_ERROR(ENOTSUP)).
  */
 #ifndef CRAW_PIPELINE_H
 #define CRAW_PIPELINE_H
@@ -49,7 +52,7 @@ enum CRAWDebayer {
 #define CRAW_PIPELINE_FLAG_AWB_HISTOGRAM            (1 << 0)
 #define CRAW_PIPELINE_FLAG_DENOISE                  (1 << 1)
 #define CRAW_PIPELINE_FLAG_LENS                     (1 << 2)  /* ENOTSUP here */
-#define CRAW_PIPELINE_FLAG_FALSE_COLOR              (1 << 3)  /* ENOTSUP here */
+#define CRAW_PIPELINE_FLAG_FALSE_COLOR              (1 << 3)
 #define CRAW_PIPELINE_FLAG_CHROMA_DENOISE           (1 << 4)
 #define CRAW_PIPELINE_FLAG_HIGHLIGHT_RECONSTRUCTION (1 << 5)  /* ENOTSUP here */
 #define CRAW_PIPELINE_FLAG_DEFRINGE                 (1 << 6)  /* ENOTSUP here */
@@ -115,7 +118,9 @@ typedef struct CRAWFrameParams {
     /* Used when a > 0; else the profile's table at its as_shot_ei. */
     CRAWNoiseModel noise_override;
 
-    uint32_t false_color_passes;    /* ignored */
+    /* False colour passes, 0 (off) to CRAW_FALSE_COLOR_MAX_PASSES; more is
+     * CRAW_ERROR(EINVAL) with CRAW_PIPELINE_FLAG_FALSE_COLOR. */
+    uint32_t false_color_passes;
 
     /* Chroma denoise, 0 (off) to 1. */
     float denoise_chroma_strength;

@@ -7,8 +7,8 @@
  *   craw_bench [--raw FILE] [--width W] [--height H] [--frames N]
  *              [--warmup K] [--debayer rcd|mhc|bilinear]
  *              [--space xyz|709|2020|p3|ap0|ap1] [--denoise S]
- *              [--chroma S] [--temporal A] [--awb] [--fp16] [--half]
- *              [--cold] [--estimate-noise] [--out FILE]
+ *              [--chroma S] [--temporal A] [--false-color N] [--awb]
+ *              [--fp16] [--half] [--cold] [--estimate-noise] [--out FILE]
  *
  * The frame is a raw little-endian uint16 RGGB dump of W x H samples
  * (12-bit data, black 512, white 4095 of 65535, the format of the
@@ -115,7 +115,7 @@ static void usage(void)
 {
     fprintf(stderr, "usage: craw_bench [--raw FILE] [--width W] [--height H] [--frames N] "
                     "[--warmup K] [--debayer rcd|mhc|bilinear] [--space xyz|709|2020|p3|ap0|ap1] "
-                    "[--denoise S] [--chroma S] [--temporal A] [--awb] [--fp16] [--half] "
+                    "[--denoise S] [--chroma S] [--temporal A] [--false-color N] [--awb] [--fp16] [--half] "
                     "[--cold] [--estimate-noise] [--out FILE]\n");
 }
 
@@ -132,7 +132,7 @@ int main(int argc, char **argv)
 {
     const char *raw_path = NULL, *out_path = NULL;
     int w = 5760, h = 3240, frames = 100, warmup = 50, cold = 0, awb = 0, fp16 = 0, half = 0;
-    int estimate_noise = 0;
+    int estimate_noise = 0, false_color = 0;
     float denoise = 1.0f, chroma = 0.5f, temporal = 0.0f;
     enum CRAWDebayer debayer = CRAW_DEBAYER_RCD;
     enum CRAWColorSpace space = CRAW_CS_ACES_AP1_LINEAR;
@@ -165,6 +165,7 @@ int main(int argc, char **argv)
         else if (!strcmp(a, "--denoise") && v) denoise = (float)atof(argv[++i]);
         else if (!strcmp(a, "--chroma") && v) chroma = (float)atof(argv[++i]);
         else if (!strcmp(a, "--temporal") && v) temporal = (float)atof(argv[++i]);
+        else if (!strcmp(a, "--false-color") && v) false_color = atoi(argv[++i]);
         else if (!strcmp(a, "--awb")) awb = 1;
         else if (!strcmp(a, "--fp16")) fp16 = 1;
         else if (!strcmp(a, "--half")) half = 1;
@@ -248,6 +249,7 @@ int main(int argc, char **argv)
     fp.denoise_strength = denoise;
     fp.denoise_chroma_strength = chroma;
     fp.denoise_temporal_strength = temporal;
+    fp.false_color_passes = (uint32_t)false_color;
     fp.noise_override.a = 2e-4f;
     fp.noise_override.b = 4e-6f;
     if (estimate_noise && (denoise > 0.0f || temporal > 0.0f)) {
@@ -283,6 +285,7 @@ int main(int argc, char **argv)
                (denoise > 0.0f ? CRAW_PIPELINE_FLAG_DENOISE : 0) |
                (chroma > 0.0f ? CRAW_PIPELINE_FLAG_CHROMA_DENOISE : 0) |
                (temporal > 0.0f ? CRAW_PIPELINE_FLAG_TEMPORAL_DENOISE : 0) |
+               (false_color > 0 ? CRAW_PIPELINE_FLAG_FALSE_COLOR : 0) |
                (fp16 ? CRAW_PIPELINE_FLAG_NLM_FP16 : 0);
     if ((ret = craw_pipeline_create(ctx, &pp, &pipe)) < 0) {
         fprintf(stderr, "craw_pipeline_create: %d\n", ret);

@@ -48,13 +48,14 @@ struct CRAWStageConfig {
     enum CRAWDebayer debayer;
     enum CRAWColorSpace out_space;
     enum CRAWFormat out_format;
-    bool awb, has_denoise, has_temporal, has_chroma, fp16, fast;
+    bool awb, has_denoise, has_temporal, has_chroma, has_fc, fp16, fast;
 };
 
 /* The device buffers one frame's recording uses. */
 struct CRAWFrameBuffers {
     uint16_t *clean;            /* W x H, when denoising */
-    float *inter_rgb;           /* 3 x W x H, with chroma denoise */
+    float *inter_rgb;           /* 3 x W x H, with chroma denoise or false colour */
+    float *inter_rgb2;          /* the same, false colour's output for the chroma stage */
     float *down, *blur;         /* 3 and 2 quarter planes */
     uint32_t *hist;             /* CRAW_HIST_WORDS, with the histogram */
     uint2 *awb_samples;         /* one per 8x8 block, with the histogram */
@@ -67,6 +68,8 @@ struct CRAWFramePlan {
     KFrame f;                   /* what the kernels read */
     int dn_mode;                /* KDN_* bits, 0: no denoise */
     bool chroma;
+    int fc_passes;              /* false colour passes, 0: none */
+    KInter fc_to;               /* where the false colour stage writes for the chroma stage */
     int t_read, t_write;        /* temporal state sets, -1: none */
     bool temporal, replay;
     CRAWTemporalState *ts;

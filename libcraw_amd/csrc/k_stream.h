@@ -103,7 +103,8 @@ KDEV void rs_finish(float (&acc)[3], const KFrame &f, int oy, bool out_cols, int
  * the 4x4 block sums of R - G, B - G, G in raster order (the block's first
  * lane, column X0 % 4 == 0, adds its two columns and then its right
  * neighbour's; lib/ref/ref_chroma.c:59-82), stored as means on the
- * block's last row. */
+ * block's last row unless f.inter.down is NULL (the false colour stage,
+ * not the chroma stage, reads the planes next). */
 KDEV void rs_store_inter(float (&acc)[3], const KFrame &f, int oy, bool out_cols, int X0,
                          const float r[2], const float g[2], const float b[2])
 {
@@ -136,7 +137,7 @@ KDEV void rs_store_inter(float (&acc)[3], const KFrame &f, int oy, bool out_cols
         acc[2] = acc[2] + g[1];
         acc[2] = acc[2] + n20;
         acc[2] = acc[2] + n21;
-        if ((oy & 3) == 3 && out_cols && (X0 & 3) == 0) {
+        if ((oy & 3) == 3 && out_cols && (X0 & 3) == 0 && it.down) {
             const size_t q = (size_t)(oy / 4) * it.wq + X0 / 4;
             const size_t qplane = (size_t)it.wq * it.hq;
 

@@ -343,6 +343,12 @@ hipError_t craw_launch_batch_rcd_stream(bool to_inter, const KFrame *frames,
                                         const uint32_t *tiles, uint32_t ntiles, int grid,
                                         hipStream_t s);
 
+/* False colour suppression, `passes` 1..5, on f.inter.rgb: into the chroma
+ * stage's inputs `to` (planes and quarter means) when chroma is set, else
+ * through the AWB block samples and the colour stage into f.out. */
+hipError_t craw_launch_false_color(int passes, bool chroma, const KFrame &f, const KInter &to,
+                                   hipStream_t s);
+
 /* The AWB histogram from the finish stage's block samples. */
 hipError_t craw_launch_awb_hist(const KFrame &f, hipStream_t s);
 

@@ -54,6 +54,7 @@ CRAW_API void craw_pipeline_free(CRAWPipeline **pipe)
     CRAW_IGNORE(hipSetDevice(p->ctx->device));
     CRAW_IGNORE(hipFree(p->buf.clean));
     CRAW_IGNORE(hipFree(p->buf.inter_rgb));
+    CRAW_IGNORE(hipFree(p->buf.inter_rgb2));
     CRAW_IGNORE(hipFree(p->buf.down));
     CRAW_IGNORE(hipFree(p->buf.blur));
     CRAW_IGNORE(hipFree(p->buf.hist));
@@ -142,11 +143,16 @@ CRAW_API int craw_pipeline_create(CRAWContext *ctx, const CRAWPipelineParams *pa
             ret = CRAW_ERROR(EINVAL);
             goto fail;
         }
-        if ((ret = dmalloc((void **)&p->buf.inter_rgb, 3 * W * H * sizeof(float))) < 0 ||
-            (ret = dmalloc((void **)&p->buf.down, 3 * wq * hq * sizeof(float))) < 0 ||
+        if ((ret = dmalloc((void **)&p->buf.down, 3 * wq * hq * sizeof(float))) < 0 ||
             (ret = dmalloc((void **)&p->buf.blur, 2 * wq * hq * sizeof(float))) < 0)
             goto fail;
+        if (p->cfg.has_fc &&
+            (ret = dmalloc((void **)&p->buf.inter_rgb2, 3 * W * H * sizeof(float))) < 0)
+            goto fail;
     }
+    if ((p->cfg.has_chroma || p->cfg.has_fc) &&
+        (ret = dmalloc((void **)&p->buf.inter_rgb, 3 * W * H * sizeof(float))) < 0)
+        goto fail;
     if (p->cfg.awb) {
         p->nb_slots = params->nb_awb_slots ? params->nb_awb_slots : 1;
         if ((ret = dmalloc((void **)&p->buf.hist, CRAW_HIST_WORDS * sizeof(uint32_t))) < 0 ||
@@ -224,16 +230,26 @@ static int pipeline_enqueue(CRAWPipeline *p, const CRAWFramePlan *plan, hipStrea
         static const char *names[3] = { "debayer_bilinear", "debayer_mhc", "rcd" };
 
         craw_profiler_begin(pr, names[f.method]);
-        e = craw_launch_debayer(plan->chroma, f, stream);
+        e = craw_launch_debayer(plan->chroma || plan->fc_passes, f, stream);
+        craw_profiler_end(pr);
+    }
+    if (e == hipSuccess && plan->fc_passes) {
+        craw_profiler_begin(pr, plan->chroma ? "false_color" : "false_color_color");
+        e = craw_launch_false_color(plan->fc_passes, plan->chroma, f, plan->fc_to, stream);
         craw_profiler_end(pr);
     }
     if (e == hipSuccess && plan->chroma) {
+        KFrame fc = f;
+
+        /* after the false colour stage the chroma stage reads what it wrote */
+        if (plan->fc_passes)
+            fc.inter = plan->fc_to;
         craw_profiler_begin(pr, "chroma_blur");
-        e = craw_launch_chroma_blur(f, stream);
+        e = craw_launch_chroma_blur(fc, stream);
         craw_profiler_end(pr);
         if (e == hipSuccess) {
             craw_profiler_begin(pr, "chroma_up_color");
-            e = craw_launch_chroma_up_finish(f, stream);
+            e = craw_launch_chroma_up_finish(fc, stream);
             craw_profiler_end(pr);
         }
     }

@@ -24,15 +24,14 @@
 #define CD_MIN_SIZE 20
 
 static const uint32_t unsupported_flags =
-    CRAW_PIPELINE_FLAG_LENS | CRAW_PIPELINE_FLAG_FALSE_COLOR |
-    CRAW_PIPELINE_FLAG_HIGHLIGHT_RECONSTRUCTION | CRAW_PIPELINE_FLAG_DEFRINGE |
+    CRAW_PIPELINE_FLAG_LENS | CRAW_PIPELINE_FLAG_HIGHLIGHT_RECONSTRUCTION | CRAW_PIPELINE_FLAG_DEFRINGE |
     CRAW_PIPELINE_FLAG_SHARPEN;
 
 static const uint32_t known_flags =
     CRAW_PIPELINE_FLAG_AWB_HISTOGRAM | CRAW_PIPELINE_FLAG_DENOISE |
     CRAW_PIPELINE_FLAG_CHROMA_DENOISE | CRAW_PIPELINE_FLAG_TEMPORAL_DENOISE |
     CRAW_PIPELINE_FLAG_PROFILE | CRAW_PIPELINE_FLAG_NLM_FP16 | CRAW_PIPELINE_FLAG_NLM_FAST |
-    unsupported_flags;
+    CRAW_PIPELINE_FLAG_FALSE_COLOR | unsupported_flags;
 
 int craw_stage_config(CRAWContext *ctx, uint32_t flags, enum CRAWDebayer debayer,
                       enum CRAWColorSpace out_space, enum CRAWFormat out_format,
@@ -59,6 +58,7 @@ int craw_stage_config(CRAWContext *ctx, uint32_t flags, enum CRAWDebayer debayer
     cfg->has_denoise = !!(flags & CRAW_PIPELINE_FLAG_DENOISE);
     cfg->has_temporal = !!(flags & CRAW_PIPELINE_FLAG_TEMPORAL_DENOISE);
     cfg->has_chroma = !!(flags & CRAW_PIPELINE_FLAG_CHROMA_DENOISE);
+    cfg->has_fc = !!(flags & CRAW_PIPELINE_FLAG_FALSE_COLOR);
     cfg->fp16 = !!(flags & CRAW_PIPELINE_FLAG_NLM_FP16);
     cfg->fast = !!(flags & CRAW_PIPELINE_FLAG_NLM_FAST);
     return 0;
@@ -255,6 +255,26 @@ int craw_frame_plan(CRAWContext *ctx, const CRAWStageConfig &cfg, const CRAWFram
     plan->f.inter.blur = buf.blur;
     plan->f.inter.wq = (int)((w + 3) / 4);
     plan->f.inter.hq = (int)((h + 3) / 4);
+
+    /* lib/pipeline.c:1820-1836: zero passes leave the frame alone, with or
+     * without the stage; more than the stage has are refused */
+    if (cfg.has_fc && fp->false_color_passes > 0) {
+        if (fp->false_color_passes > CRAW_FALSE_COLOR_MAX_PASSES) {
+            craw_log(ctx, CRAW_LOG_ERROR, "false_color_passes = %u; at most %u passes are "
+                     "supported\n", fp->false_color_passes,
+                     (unsigned)CRAW_FALSE_COLOR_MAX_PASSES);
+            return CRAW_ERROR(EINVAL);
+        }
+        if (!buf.inter_rgb || (plan->chroma && !buf.inter_rgb2))
+            return CRAW_ERROR(EINVAL);
+        plan->fc_passes = (int)fp->false_color_passes;
+        /* the debayer leaves the planes alone (no quarter means); the
+         * false colour stage writes the chroma stage's inputs, which the
+         * chroma stage then reads */
+        plan->fc_to = plan->f.inter;
+        plan->fc_to.rgb = buf.inter_rgb2;
+        plan->f.inter.down = NULL;
+    }
 
     /* Colour: the frame's matrix and scale, then XYZ to the output space,
      * as lib/ref/ref_color.c:10-76 applies them; gamut compression for

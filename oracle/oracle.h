@@ -93,6 +93,8 @@ void oracle_gamut_params_d(double thr[3], double scl[3], double *pwr, double *in
                             uint16_t *out, size_t out_stride, real *out_level);      \
     int P##chroma_denoise(real *rgb, uint32_t w, uint32_t h, float strength);        \
     int P##false_color(real *rgb, uint32_t w, uint32_t h, uint32_t passes);          \
+    int P##highlights(const uint16_t *bayer, uint32_t w, uint32_t h, size_t stride,  \
+                      const CRAWProfile *p, const float wb[3], real *out);           \
     int P##color(real *rgb, uint32_t w, uint32_t h, const CRAWFrameParams *fp,       \
                  int out_space);                                                     \
     void P##gamut_compress(real *rgb, size_t n);                                     \

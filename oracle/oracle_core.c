@@ -835,6 +835,303 @@ fail:
     return ret;
 }
 
+/* ------------------------------------------ highlight reconstruction */
+
+#define O_HL_U_LO    K(0.92f, 0.92)
+#define O_HL_U_HI    K(0.98f, 0.98)
+#define O_HL_EPS     K(1e-4f, 1e-4)
+#define O_HL_LOG_MAX K(2.0f, 2.0)
+#define O_HL_W_SAT   K(1.0f, 1.0)
+#define O_HL_DILATE  3
+#define O_HL_LEVELS  16
+
+/* The rebuilt mosaic's R16F store, rounded to nearest, ties to even, kept
+ * as a float (lib/ref/ref.h:20-56): a subnormal half is a multiple of
+ * 2^-24; a normal one keeps 10 mantissa bits. The double mode does not
+ * round. */
+static real o_hl_store(real v)
+{
+#if ORACLE_MODE == 3
+    return v;
+#else
+    uint32_t b, a, sign;
+    float f = v;
+
+    memcpy(&b, &f, sizeof(b));
+    sign = b & 0x80000000u;
+    a = b & 0x7fffffffu;
+    if (a == 0u || a >= 0x7f800000u)
+        return f;
+    if (a < 0x38800000u) {
+        const float q = nearbyintf(fabsf(f) * 16777216.0f) * (1.0f / 16777216.0f);
+        return sign ? -q : q;
+    }
+    a = (a + 0x0fffu + ((a >> 13) & 1u)) & ~0x1fffu;
+    b = sign | a;
+    memcpy(&f, &b, sizeof(f));
+    return f;
+#endif
+}
+
+static inline real o_hl_smoothstep(real u)
+{
+    real t = (u - O_HL_U_LO) / (O_HL_U_HI - O_HL_U_LO);
+
+    t = OCLAMP(t, K(0.0f, 0.0), K(1.0f, 1.0));
+    return t * t * (K(3.0f, 3.0) - K(2.0f, 2.0) * t);
+}
+
+/* One level of the colour field: (sum of w c_r, sum of w c_b, sum of w)
+ * per texel, then (c_r, c_b, w) once pushed. */
+struct o_hl_level {
+    int w, h;
+    real *t;
+};
+
+/* A texel's colour, its coordinates clamped to the level; the top level
+ * still holds sums and is divided on the way. */
+static void o_hl_texel(const struct o_hl_level *l, int x, int y, int top, real c[2])
+{
+    const real *t;
+
+    x = x < 0 ? 0 : (x >= l->w ? l->w - 1 : x);
+    y = y < 0 ? 0 : (y >= l->h ? l->h - 1 : y);
+    t = l->t + 3 * ((size_t)y * l->w + x);
+    c[0] = top ? (t[2] > 0 ? t[0] / t[2] : 0) : t[0];
+    c[1] = top ? (t[2] > 0 ? t[1] / t[2] : 0) : t[1];
+}
+
+/* Bilinear in texel coordinates, x first then y, each as a (1 - f) + b f. */
+static void o_hl_bilinear(const struct o_hl_level *l, real px, real py, int top, real out[2])
+{
+    const real x0 = FLOORF(px), y0 = FLOORF(py), fx = px - x0, fy = py - y0;
+    const int ix = (int)x0, iy = (int)y0;
+    real c00[2], c10[2], c01[2], c11[2];
+
+    o_hl_texel(l, ix, iy, top, c00);
+    o_hl_texel(l, ix + 1, iy, top, c10);
+    o_hl_texel(l, ix, iy + 1, top, c01);
+    o_hl_texel(l, ix + 1, iy + 1, top, c11);
+    for (int i = 0; i < 2; i++) {
+        const real a = c00[i] * (1 - fx) + c10[i] * fx;
+        const real b = c01[i] * (1 - fx) + c11[i] * fx;
+
+        out[i] = a * (1 - fy) + b * fy;
+    }
+}
+
+/* lib/ref/ref_highlight.c:91-335: the clip ramp a = smoothstep(0.92, 0.98,
+ * u) per site; the colour field's level 0 (a texel per 4x4 block: the
+ * block's quads, top row first, each weighted (1 - a over the 7x7 quads
+ * about it) min(u, 1)^2 with the clamped logs of R/G and B/G); the pull to
+ * 1x1 (2x2 sums); the push back down (each level its own colour where it
+ * has weight, blended towards the bilinear of the pushed level above by
+ * min(w, 1)); and the fill: every ramped site raised (never lowered)
+ * towards the level its unclipped neighbours and the field's colour
+ * predict, blended by a. `out` is the balanced mosaic, R16F-rounded. */
+int PFX(highlights)(const uint16_t *bayer, uint32_t width, uint32_t height, size_t stride,
+                    const CRAWProfile *p, const float wb[3], real *out)
+{
+    const int w = (int)width, h = (int)height, qw = w / 2, qh = h / 2;
+    struct o_hl_level lv[O_HL_LEVELS];
+    int nl = 0, ret = 0;
+    real *u = NULL, *a = NULL, *qa = NULL, rwb[3], inv_range[4];
+
+    memset(lv, 0, sizeof(lv));
+    if (!bayer || !p || !wb || !out)
+        return -EINVAL;
+    if (p->cfa != CRAW_CFA_RGGB)
+        return -ENOTSUP;
+    if (width < 8 || height < 8 || (width & 1) || (height & 1) || stride < width)
+        return -EINVAL;
+    for (int s = 0; s < 4; s++) {
+        if (!(p->white_level[s] > p->black_level[s]))
+            return -EINVAL;
+        inv_range[s] = K(1.0f, 1.0) / ((real)p->white_level[s] - (real)p->black_level[s]);
+    }
+    for (int c = 0; c < 3; c++)
+        rwb[c] = wb[c];
+
+    u = malloc((size_t)w * h * sizeof(*u));
+    a = malloc((size_t)w * h * sizeof(*a));
+    qa = malloc((size_t)qw * qh * sizeof(*qa));
+    if (!u || !a || !qa) {
+        ret = -ENOMEM;
+        goto done;
+    }
+    for (int y = 0; y < h; y++) {
+        for (int x = 0; x < w; x++) {
+            const int s = (y & 1) * 2 + (x & 1);
+            const real raw = (real)bayer[(size_t)y * stride + x] * K(1.0f / 65535.0f, 1.0 / 65535.0);
+            const size_t o = (size_t)y * w + x;
+
+            u[o] = (raw - (real)p->black_level[s]) * inv_range[s];
+            a[o] = o_hl_smoothstep(u[o]);
+        }
+    }
+    for (int qy = 0; qy < qh; qy++) {
+        for (int qx = 0; qx < qw; qx++) {
+            real m = 0;
+
+            for (int s = 0; s < 4; s++)
+                m = OMAX(m, a[(size_t)(2 * qy + (s >> 1)) * w + 2 * qx + (s & 1)]);
+            qa[(size_t)qy * qw + qx] = m;
+        }
+    }
+
+    /* the pyramid: level 0 a texel per 4x4 block, halved (rounding up) to 1x1 */
+    for (int lw = (w + 3) / 4, lh = (h + 3) / 4;; lw = (lw + 1) / 2, lh = (lh + 1) / 2) {
+        if (nl == O_HL_LEVELS) {
+            ret = -EINVAL;
+            goto done;
+        }
+        lv[nl].w = lw;
+        lv[nl].h = lh;
+        lv[nl].t = calloc((size_t)lw * lh * 3, sizeof(real));
+        if (!lv[nl].t) {
+            ret = -ENOMEM;
+            goto done;
+        }
+        nl++;
+        if (lw == 1 && lh == 1)
+            break;
+    }
+
+    /* level 0 */
+    for (int by = 0; by < lv[0].h; by++) {
+        for (int bx = 0; bx < lv[0].w; bx++) {
+            real *t = lv[0].t + 3 * ((size_t)by * lv[0].w + bx);
+
+            for (int j = 0; j < 2; j++) {
+                for (int i = 0; i < 2; i++) {
+                    const int qx = 2 * bx + i, qy = 2 * by + j;
+                    real amax = 0, umax = 0, rgb[3] = { 0, 0, 0 }, m, wq, g, r, b;
+
+                    if (qx >= qw || qy >= qh)
+                        continue;
+                    for (int ny = qy - O_HL_DILATE; ny <= qy + O_HL_DILATE; ny++)
+                        for (int nx = qx - O_HL_DILATE; nx <= qx + O_HL_DILATE; nx++)
+                            if (nx >= 0 && ny >= 0 && nx < qw && ny < qh)
+                                amax = OMAX(amax, qa[(size_t)ny * qw + nx]);
+                    for (int s = 0; s < 4; s++) {
+                        const real v = u[(size_t)(2 * qy + (s >> 1)) * w + 2 * qx + (s & 1)];
+
+                        rgb[o_site_chan[s]] += v * rwb[o_site_chan[s]];
+                        umax = OMAX(umax, v);
+                    }
+                    rgb[1] *= K(0.5f, 0.5);
+                    m = OMIN(umax, K(1.0f, 1.0));
+                    wq = (1 - amax) * m * m;
+                    g = OMAX(rgb[1], O_HL_EPS);
+                    r = OCLAMP(LOGF(OMAX(rgb[0], O_HL_EPS) / g), -O_HL_LOG_MAX, O_HL_LOG_MAX);
+                    b = OCLAMP(LOGF(OMAX(rgb[2], O_HL_EPS) / g), -O_HL_LOG_MAX, O_HL_LOG_MAX);
+                    t[0] += wq * r;
+                    t[1] += wq * b;
+                    t[2] += wq;
+                }
+            }
+        }
+    }
+
+    /* pull */
+    for (int l = 0; l + 1 < nl; l++) {
+        const struct o_hl_level *f = &lv[l];
+        struct o_hl_level *c = &lv[l + 1];
+
+        for (int y = 0; y < c->h; y++)
+            for (int x = 0; x < c->w; x++)
+                for (int dy = 0; dy < 2; dy++)
+                    for (int dx = 0; dx < 2; dx++) {
+                        const int fx = 2 * x + dx, fy = 2 * y + dy;
+
+                        if (fx < f->w && fy < f->h)
+                            for (int k = 0; k < 3; k++)
+                                c->t[3 * ((size_t)y * c->w + x) + k] +=
+                                    f->t[3 * ((size_t)fy * f->w + fx) + k];
+                    }
+    }
+
+    /* push */
+    for (int l = nl - 2; l >= 0; l--) {
+        struct o_hl_level *f = &lv[l];
+        const int top = l + 2 == nl;
+
+        for (int y = 0; y < f->h; y++) {
+            for (int x = 0; x < f->w; x++) {
+                real *t = f->t + 3 * ((size_t)y * f->w + x), up[2];
+
+                o_hl_bilinear(&lv[l + 1], ((real)x + K(0.5f, 0.5)) * K(0.5f, 0.5) - K(0.5f, 0.5),
+                              ((real)y + K(0.5f, 0.5)) * K(0.5f, 0.5) - K(0.5f, 0.5), top, up);
+                if (t[2] > 0) {
+                    const real sw = OMIN(t[2] / O_HL_W_SAT, K(1.0f, 1.0));
+
+                    t[0] = up[0] + sw * (t[0] / t[2] - up[0]);
+                    t[1] = up[1] + sw * (t[1] / t[2] - up[1]);
+                } else {
+                    t[0] = up[0];
+                    t[1] = up[1];
+                }
+            }
+        }
+    }
+
+    /* fill */
+    for (int y = 0; y < h; y++) {
+        for (int x = 0; x < w; x++) {
+            const size_t o = (size_t)y * w + x;
+            const int c = o_site_chan[(y & 1) * 2 + (x & 1)];
+            const real xc = u[o] * rwb[c], av = a[o];
+            real res = xc;
+
+            if (av > 0) {
+                real sum[3] = { 0, 0, 0 }, n[3] = { 0, 0, 0 }, am[3] = { 0, 0, 0 };
+                real lf[2], lrho[3], a1, a2, l1, l2, lc, A, lfull, lpart, L;
+                int k1, k2;
+
+                o_hl_bilinear(&lv[0], (real)(x >> 1) * K(0.5f, 0.5) - K(0.25f, 0.25),
+                              (real)(y >> 1) * K(0.5f, 0.5) - K(0.25f, 0.25), 0, lf);
+                lrho[0] = lf[0];
+                lrho[1] = 0;
+                lrho[2] = lf[1];
+                for (int dy = -1; dy <= 1; dy++) {
+                    for (int dx = -1; dx <= 1; dx++) {
+                        const int nx = o_mirror(x + dx, w), ny = o_mirror(y + dy, h);
+                        const int k = o_site_chan[(ny & 1) * 2 + (nx & 1)];
+                        const size_t no = (size_t)ny * w + nx;
+
+                        if (k == c)
+                            continue;
+                        sum[k] += u[no] * rwb[k];
+                        n[k] += 1;
+                        am[k] = OMAX(am[k], a[no]);
+                    }
+                }
+                k1 = c == 0 ? 1 : 0;
+                k2 = c == 2 ? 1 : 2;
+                a1 = 1 - am[k1];
+                a2 = 1 - am[k2];
+                l1 = LOGF(OMAX(sum[k1] / n[k1], O_HL_EPS)) - lrho[k1];
+                l2 = LOGF(OMAX(sum[k2] / n[k2], O_HL_EPS)) - lrho[k2];
+                lc = LOGF(OMAX(xc, O_HL_EPS)) - lrho[c];
+                A = a1 + a2;
+                lfull = EXPF(OMAX(lc, OMAX(l1, l2)));
+                lpart = A > 0 ? EXPF((a1 * l1 + a2 * l2) / A) : lfull;
+                L = lfull + OMIN(A, K(1.0f, 1.0)) * (lpart - lfull);
+                res = xc + av * OMAX(L * EXPF(lrho[c]) - xc, K(0.0f, 0.0));
+            }
+            out[o] = o_hl_store(res);
+        }
+    }
+
+done:
+    for (int i = 0; i < nl; i++)
+        free(lv[i].t);
+    free(u);
+    free(a);
+    free(qa);
+    return ret;
+}
+
 /* -------------------------------------------------------- false colour */
 
 /* lib/ref/ref_false_color.c:27-41: the median of nine as a selection, by
@@ -1177,7 +1474,24 @@ int PFX(process)(const uint16_t *bayer, uint32_t width, uint32_t height, size_t 
     ret = oracle_profile_eval(fp->profile, fp->wb_mode, &fp->illuminant, &par);
     if (ret < 0)
         goto done;
-    ret = PFX(debayer_wb)(bayer, width, height, stride, fp->profile, par.wb, pp->debayer, out);
+    /* ref_color.c:148-171: the rebuilt mosaic replaces the prep and its
+     * clip */
+    if ((pp->flags & CRAW_PIPELINE_FLAG_HIGHLIGHT_RECONSTRUCTION) &&
+        (fp->flags & CRAW_FRAME_FLAG_HIGHLIGHT_RECONSTRUCTION)) {
+        real *v = malloc((size_t)width * height * sizeof(*v));
+
+        if (!v) {
+            ret = -ENOMEM;
+            goto done;
+        }
+        ret = PFX(highlights)(bayer, width, height, stride, fp->profile, par.wb, v);
+        if (ret >= 0)
+            ret = PFX(debayer_mosaic)(v, width, height, pp->debayer, out);
+        free(v);
+    } else {
+        ret = PFX(debayer_wb)(bayer, width, height, stride, fp->profile, par.wb, pp->debayer,
+                              out);
+    }
     if (ret < 0)
         goto done;
     /* ref_color.c:178-185: the false colour on the intermediate first */
