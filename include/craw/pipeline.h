@@ -13,7 +13,10 @@
  *   src (R16_UNORM Bayer mosaic, RGGB)
  *     -> [temporal denoise against the previous frame's state]
  *     -> [non-local means on the four CFA planes, variance stabilised]
- *     -> black/white normalisation, white balance, clip at 1.0
+ *     -> black/white normalisation, white balance, clip at 1.0, or
+ *        [highlight reconstruction: clipped sites rebuilt from their
+ *         neighbours and a multi-scale colour field, balanced, unclipped,
+ *         stored as R16F]
  *     -> bilinear, MHC or RCD debayer to camera RGB
  *     -> [false colour suppression: 1 to 5 passes of a 3x3 median and a
  *         3x3 box on R - G and B - G]
@@ -23,11 +26,11 @@
  *     -> [ACES 1.3 gamut compression, ACEScg output only]
  *   dst (RGBA16F or RGBA32F)
  *
- * The reference's lens, highlight, defringe and sharpening stages are not
- * part of this library; their pipeline flags make craw_pipeline_create()
- * fail with CRAW_ERROR(ENOTSUP) and their frame fields are ignored. The
- * false colour stage is in the plain form only (craw_batch_create()
- * refuses its flag with CRAW_ERROR(ENOTSUP)).
+ * The reference's lens, defringe and sharpening stages are not part of
+ * this library; their pipeline flags make craw_pipeline_create() fail with
+ * CRAW_ERROR(ENOTSUP) and their frame fields are ignored. The highlight and
+ * false colour stages are in the plain form only (craw_batch_create()
+ * refuses their flags with CRAW_ERROR(ENOTSUP)).
  */
 #ifndef CRAW_PIPELINE_H
 #define CRAW_PIPELINE_H
@@ -54,7 +57,7 @@ enum CRAWDebayer {
 #define CRAW_PIPELINE_FLAG_LENS                     (1 << 2)  /* ENOTSUP here */
 #define CRAW_PIPELINE_FLAG_FALSE_COLOR              (1 << 3)
 #define CRAW_PIPELINE_FLAG_CHROMA_DENOISE           (1 << 4)
-#define CRAW_PIPELINE_FLAG_HIGHLIGHT_RECONSTRUCTION (1 << 5)  /* ENOTSUP here */
+#define CRAW_PIPELINE_FLAG_HIGHLIGHT_RECONSTRUCTION (1 << 5)
 #define CRAW_PIPELINE_FLAG_DEFRINGE                 (1 << 6)  /* ENOTSUP here */
 #define CRAW_PIPELINE_FLAG_TEMPORAL_DENOISE         (1 << 7)
 #define CRAW_PIPELINE_FLAG_SHARPEN                  (1 << 8)  /* ENOTSUP here */
@@ -95,7 +98,7 @@ typedef struct CRAWPipelineParams {
 #define CRAW_FRAME_FLAG_CLAMP_NEGATIVE           (1 << 0)
 #define CRAW_FRAME_FLAG_LENS_KEEP_ALL            (1 << 1)  /* ignored */
 #define CRAW_FRAME_FLAG_NO_GAMUT_COMPRESSION     (1 << 2)
-#define CRAW_FRAME_FLAG_HIGHLIGHT_RECONSTRUCTION (1 << 3)  /* ignored */
+#define CRAW_FRAME_FLAG_HIGHLIGHT_RECONSTRUCTION (1 << 3)  /* with the pipeline flag */
 #define CRAW_FRAME_FLAG_TEMPORAL_RESET           (1 << 4)
 
 typedef struct CRAWFrameParams {

@@ -42,6 +42,7 @@ CRAW_PIPELINE_FLAG_NLM_FAST = 1 << 17
 
 CRAW_FRAME_FLAG_CLAMP_NEGATIVE = 1 << 0
 CRAW_FRAME_FLAG_NO_GAMUT_COMPRESSION = 1 << 2
+CRAW_FRAME_FLAG_HIGHLIGHT_RECONSTRUCTION = 1 << 3
 CRAW_FRAME_FLAG_TEMPORAL_RESET = 1 << 4
 
 CRAW_AWB_HIST_BINS = 128

@@ -175,9 +175,9 @@ static int batch_sizes(CRAWContext *ctx, const CRAWBatchParams *params, CRAWStag
                             params->out_format, cfg);
     if (ret < 0)
         return ret;
-    if (cfg->has_fc) {
-        craw_log(ctx, CRAW_LOG_ERROR, "The batched form has no false colour stage; record such "
-                 "frames with craw_pipeline_record()\n");
+    if (cfg->has_fc || cfg->has_hl) {
+        craw_log(ctx, CRAW_LOG_ERROR, "The batched form has no false colour or highlight "
+                 "stage; record such frames with craw_pipeline_record()\n");
         return CRAW_ERROR(ENOTSUP);
     }
     fast = cfg->fp16 || cfg->fast;

@@ -28,7 +28,7 @@ struct CRAWContext {
     size_t noise_scratch_size;
 };
 
-#define CRAW_MAX_STAGES 12
+#define CRAW_MAX_STAGES 16
 #define CRAW_HIST_WORDS (3u * CRAW_AWB_HIST_BINS * CRAW_AWB_HIST_BINS)
 
 /* Two sets of (level, variance) for one clip stream and which set holds
@@ -48,7 +48,7 @@ struct CRAWStageConfig {
     enum CRAWDebayer debayer;
     enum CRAWColorSpace out_space;
     enum CRAWFormat out_format;
-    bool awb, has_denoise, has_temporal, has_chroma, has_fc, fp16, fast;
+    bool awb, has_denoise, has_temporal, has_chroma, has_fc, has_hl, fp16, fast;
 };
 
 /* The device buffers one frame's recording uses. */
@@ -56,6 +56,8 @@ struct CRAWFrameBuffers {
     uint16_t *clean;            /* W x H, when denoising */
     float *inter_rgb;           /* 3 x W x H, with chroma denoise or false colour */
     float *inter_rgb2;          /* the same, false colour's output for the chroma stage */
+    uint16_t *hl_mosaic;        /* W x H R16F, with highlight reconstruction */
+    float4 *hl_levels;          /* the colour field's levels, one after another */
     float *down, *blur;         /* 3 and 2 quarter planes */
     uint32_t *hist;             /* CRAW_HIST_WORDS, with the histogram */
     uint2 *awb_samples;         /* one per 8x8 block, with the histogram */
@@ -69,6 +71,8 @@ struct CRAWFramePlan {
     int dn_mode;                /* KDN_* bits, 0: no denoise */
     bool chroma;
     int fc_passes;              /* false colour passes, 0: none */
+    bool hl;                    /* highlight reconstruction before the debayer */
+    KHighlight hl_k;
     KInter fc_to;               /* where the false colour stage writes for the chroma stage */
     int t_read, t_write;        /* temporal state sets, -1: none */
     bool temporal, replay;
@@ -117,6 +121,12 @@ int craw_stage_config(CRAWContext *ctx, uint32_t flags, enum CRAWDebayer debayer
 
 /* Plans one frame of W x H (validation, constants, state selection); 0 or
  * a negative errno, nothing changed on failure. */
+/* The highlight field's pyramid for a W x H frame: level sizes and
+ * offsets (in texels) into one array; the level count, or
+ * CRAW_ERROR(EINVAL) past KHL_LEVELS. */
+int craw_hl_levels(uint32_t w, uint32_t h, int lw[KHL_LEVELS], int lh[KHL_LEVELS],
+                   size_t off[KHL_LEVELS], size_t *total);
+
 int craw_frame_plan(CRAWContext *ctx, const CRAWStageConfig &cfg, const CRAWFrameBuffers &buf,
                     uint32_t W, uint32_t H, const CRAWImage *src, const CRAWImage *dst,
                     const CRAWFrameParams *fp, CRAWFramePlan *plan);

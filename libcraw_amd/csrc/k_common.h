@@ -51,15 +51,20 @@ KDEV float ksel4(int s, float a0, float a1, float a2, float a3)
  * as lib/ref/ref_debayer.c:42-63 computes it: divide by the
  * range (white - black, formed on the host in float, exactly as the
  * reference forms it), multiply by the channel's gain, clip at 1.0;
- * below black is kept. */
+ * below black is kept. With `half` the mosaic is the highlight stage's
+ * rebuilt one (R16F bits, already balanced and unclipped) and the prep is
+ * the identity (lib/pipeline.c:2052-2062). */
 struct KPrep {
     float black[4];
     float range[4];
     float wb[4];      /* per quad site: R, G, G, B gains */
+    int half;
 };
 
 KDEV float kprep(uint32_t raw, int s, const KPrep &p)
 {
+    if (p.half)
+        return __half2float(__ushort_as_half((unsigned short)raw));
     float r = (float)raw * (1.0f / 65535.0f);
     float t = (r - ksel4(s, p.black[0], p.black[1], p.black[2], p.black[3])) /
               ksel4(s, p.range[0], p.range[1], p.range[2], p.range[3]);

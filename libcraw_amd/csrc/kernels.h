@@ -343,6 +343,31 @@ hipError_t craw_launch_batch_rcd_stream(bool to_inter, const KFrame *frames,
                                         const uint32_t *tiles, uint32_t ntiles, int grid,
                                         hipStream_t s);
 
+/* The highlight reconstruction's inputs and outputs: the mosaic it
+ * rebuilds (the caller's or the denoised one), the R16F mosaic it writes,
+ * the colour field's levels (level 0 a texel per 4x4 block, halved to
+ * 1x1), and the per-site constants, formed on the host in float as
+ * lib/ref/ref_highlight.c:120-130 forms them. */
+#define KHL_LEVELS 16
+
+struct KHighlight {
+    const uint16_t *src;
+    int src_pitch;               /* uint16 elements */
+    uint16_t *mosaic;            /* R16F bits, W x H */
+    float4 *level[KHL_LEVELS];   /* (sum w c_r, sum w c_b, sum w) / pushed (c_r, c_b, w) */
+    int lw[KHL_LEVELS], lh[KHL_LEVELS];
+    int nl;
+    int W, H;
+    float black[4], inv_range[4];
+    float wb[3];
+};
+
+/* Level 0 and its pull to levels 1-4; the rest of the pull and every push;
+ * the fill into h.mosaic. */
+hipError_t craw_launch_hl_field(const KHighlight &h, hipStream_t s);
+hipError_t craw_launch_hl_push(const KHighlight &h, hipStream_t s);
+hipError_t craw_launch_hl_fill(const KHighlight &h, hipStream_t s);
+
 /* False colour suppression, `passes` 1..5, on f.inter.rgb: into the chroma
  * stage's inputs `to` (planes and quarter means) when chroma is set, else
  * through the AWB block samples and the colour stage into f.out. */

@@ -55,6 +55,8 @@ CRAW_API void craw_pipeline_free(CRAWPipeline **pipe)
     CRAW_IGNORE(hipFree(p->buf.clean));
     CRAW_IGNORE(hipFree(p->buf.inter_rgb));
     CRAW_IGNORE(hipFree(p->buf.inter_rgb2));
+    CRAW_IGNORE(hipFree(p->buf.hl_mosaic));
+    CRAW_IGNORE(hipFree(p->buf.hl_levels));
     CRAW_IGNORE(hipFree(p->buf.down));
     CRAW_IGNORE(hipFree(p->buf.blur));
     CRAW_IGNORE(hipFree(p->buf.hist));
@@ -153,6 +155,19 @@ CRAW_API int craw_pipeline_create(CRAWContext *ctx, const CRAWPipelineParams *pa
     if ((p->cfg.has_chroma || p->cfg.has_fc) &&
         (ret = dmalloc((void **)&p->buf.inter_rgb, 3 * W * H * sizeof(float))) < 0)
         goto fail;
+    if (p->cfg.has_hl) {
+        int lw[KHL_LEVELS], lh[KHL_LEVELS];
+        size_t off[KHL_LEVELS], texels;
+
+        if (craw_hl_levels(params->width, params->height, lw, lh, off, &texels) < 0) {
+            craw_log(ctx, CRAW_LOG_ERROR, "Frame too large for the highlight field's pyramid\n");
+            ret = CRAW_ERROR(EINVAL);
+            goto fail;
+        }
+        if ((ret = dmalloc((void **)&p->buf.hl_mosaic, W * H * sizeof(uint16_t))) < 0 ||
+            (ret = dmalloc((void **)&p->buf.hl_levels, texels * sizeof(float4))) < 0)
+            goto fail;
+    }
     if (p->cfg.awb) {
         p->nb_slots = params->nb_awb_slots ? params->nb_awb_slots : 1;
         if ((ret = dmalloc((void **)&p->buf.hist, CRAW_HIST_WORDS * sizeof(uint32_t))) < 0 ||
@@ -225,6 +240,21 @@ static int pipeline_enqueue(CRAWPipeline *p, const CRAWFramePlan *plan, hipStrea
             e = craw_launch_denoise(plan->dn_mode, f, stream);
         }
         craw_profiler_end(pr);
+    }
+    if (e == hipSuccess && plan->hl) {
+        craw_profiler_begin(pr, "highlight_field");
+        e = craw_launch_hl_field(plan->hl_k, stream);
+        craw_profiler_end(pr);
+        if (e == hipSuccess) {
+            craw_profiler_begin(pr, "highlight_push");
+            e = craw_launch_hl_push(plan->hl_k, stream);
+            craw_profiler_end(pr);
+        }
+        if (e == hipSuccess) {
+            craw_profiler_begin(pr, "highlight_fill");
+            e = craw_launch_hl_fill(plan->hl_k, stream);
+            craw_profiler_end(pr);
+        }
     }
     if (e == hipSuccess) {
         static const char *names[3] = { "debayer_bilinear", "debayer_mhc", "rcd" };

@@ -24,14 +24,15 @@
 #define CD_MIN_SIZE 20
 
 static const uint32_t unsupported_flags =
-    CRAW_PIPELINE_FLAG_LENS | CRAW_PIPELINE_FLAG_HIGHLIGHT_RECONSTRUCTION | CRAW_PIPELINE_FLAG_DEFRINGE |
+    CRAW_PIPELINE_FLAG_LENS | CRAW_PIPELINE_FLAG_DEFRINGE |
     CRAW_PIPELINE_FLAG_SHARPEN;
 
 static const uint32_t known_flags =
     CRAW_PIPELINE_FLAG_AWB_HISTOGRAM | CRAW_PIPELINE_FLAG_DENOISE |
     CRAW_PIPELINE_FLAG_CHROMA_DENOISE | CRAW_PIPELINE_FLAG_TEMPORAL_DENOISE |
     CRAW_PIPELINE_FLAG_PROFILE | CRAW_PIPELINE_FLAG_NLM_FP16 | CRAW_PIPELINE_FLAG_NLM_FAST |
-    CRAW_PIPELINE_FLAG_FALSE_COLOR | unsupported_flags;
+    CRAW_PIPELINE_FLAG_FALSE_COLOR | CRAW_PIPELINE_FLAG_HIGHLIGHT_RECONSTRUCTION |
+    unsupported_flags;
 
 int craw_stage_config(CRAWContext *ctx, uint32_t flags, enum CRAWDebayer debayer,
                       enum CRAWColorSpace out_space, enum CRAWFormat out_format,
@@ -59,6 +60,7 @@ int craw_stage_config(CRAWContext *ctx, uint32_t flags, enum CRAWDebayer debayer
     cfg->has_temporal = !!(flags & CRAW_PIPELINE_FLAG_TEMPORAL_DENOISE);
     cfg->has_chroma = !!(flags & CRAW_PIPELINE_FLAG_CHROMA_DENOISE);
     cfg->has_fc = !!(flags & CRAW_PIPELINE_FLAG_FALSE_COLOR);
+    cfg->has_hl = !!(flags & CRAW_PIPELINE_FLAG_HIGHLIGHT_RECONSTRUCTION);
     cfg->fp16 = !!(flags & CRAW_PIPELINE_FLAG_NLM_FP16);
     cfg->fast = !!(flags & CRAW_PIPELINE_FLAG_NLM_FAST);
     return 0;
@@ -80,6 +82,32 @@ static bool overlaps(const CRAWImage *a, const CRAWImage *b)
     uintptr_t b0 = (uintptr_t)b->data, b1 = b0 + b->pitch * b->height;
 
     return a0 < b1 && b0 < a1;
+}
+
+int craw_hl_levels(uint32_t w, uint32_t h, int lw[KHL_LEVELS], int lh[KHL_LEVELS],
+                   size_t off[KHL_LEVELS], size_t *total)
+{
+    /* lib/pipeline.c:885-912: level 0 a texel per 4x4 block, halved
+     * (rounding up) to 1x1 */
+    uint32_t x = (w + 3) / 4, y = (h + 3) / 4;
+    size_t at = 0;
+    int n = 0;
+
+    while (1) {
+        if (n == KHL_LEVELS)
+            return CRAW_ERROR(EINVAL);
+        lw[n] = (int)x;
+        lh[n] = (int)y;
+        off[n] = at;
+        at += (size_t)x * y;
+        n++;
+        if (x == 1 && y == 1)
+            break;
+        x = (x + 1) / 2;
+        y = (y + 1) / 2;
+    }
+    *total = at;
+    return n;
 }
 
 int craw_frame_plan(CRAWContext *ctx, const CRAWStageConfig &cfg, const CRAWFrameBuffers &buf,
@@ -320,6 +348,37 @@ int craw_frame_plan(CRAWContext *ctx, const CRAWStageConfig &cfg, const CRAWFram
     } else {
         plan->f.src = (const uint16_t *)src->data;
         plan->f.src_pitch = (int)(src->pitch / 2);
+    }
+
+    /* lib/pipeline.c:2047-2062: the rebuilt mosaic (balanced, unclipped,
+     * R16F) replaces the one the debayer would read, whose prep becomes the
+     * identity */
+    plan->hl = cfg.has_hl && (fp->flags & CRAW_FRAME_FLAG_HIGHLIGHT_RECONSTRUCTION);
+    if (plan->hl) {
+        KHighlight &k = plan->hl_k;
+        size_t off[KHL_LEVELS], total;
+
+        if (!buf.hl_mosaic || !buf.hl_levels)
+            return CRAW_ERROR(EINVAL);
+        k.nl = craw_hl_levels(w, h, k.lw, k.lh, off, &total);
+        if (k.nl < 0)
+            return k.nl;
+        for (int l = 0; l < k.nl; l++)
+            k.level[l] = buf.hl_levels + off[l];
+        k.src = plan->f.src;
+        k.src_pitch = plan->f.src_pitch;
+        k.mosaic = buf.hl_mosaic;
+        k.W = (int)w;
+        k.H = (int)h;
+        for (int s = 0; s < 4; s++) {
+            k.black[s] = fp->profile->black_level[s];
+            k.inv_range[s] = 1.0f / (fp->profile->white_level[s] - fp->profile->black_level[s]);
+        }
+        for (int c = 0; c < 3; c++)
+            k.wb[c] = frame.wb[c];
+        plan->f.src = buf.hl_mosaic;
+        plan->f.src_pitch = (int)w;
+        plan->f.prep.half = 1;
     }
     return 0;
 }

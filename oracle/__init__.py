@@ -124,6 +124,17 @@ class Mode:
                                      ctypes.c_uint32(passes))
         return ret, rgb
 
+    def highlights(self, bayer, profile, wb, stride=None):
+        """The rebuilt, balanced mosaic (R16F-rounded in the float modes)."""
+        h, w = bayer.shape
+        out = np.empty((h, w), self.dtype)
+        wb = np.asarray(wb, np.float32)
+        ret = self._f("highlights")(_ptr(np.ascontiguousarray(bayer, np.uint16)),
+                                    ctypes.c_uint32(w), ctypes.c_uint32(h),
+                                    ctypes.c_size_t(stride if stride else w),
+                                    ctypes.byref(profile), _ptr(wb), _ptr(out))
+        return ret, out
+
     def color(self, rgb, fp, out_space):
         rgb = np.ascontiguousarray(rgb, self.dtype).copy()
         h, w, _ = rgb.shape
