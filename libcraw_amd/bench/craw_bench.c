@@ -7,13 +7,16 @@
  *   craw_bench [--raw FILE] [--width W] [--height H] [--frames N]
  *              [--warmup K] [--debayer rcd|mhc|bilinear]
  *              [--space xyz|709|2020|p3|ap0|ap1] [--denoise S]
- *              [--chroma S] [--temporal A] [--false-color N] [--awb]
- *              [--fp16] [--half] [--cold] [--estimate-noise] [--out FILE]
+ *              [--chroma S] [--temporal A] [--false-color N] [--highlights]
+ *              [--awb] [--fp16] [--half] [--cold] [--estimate-noise]
+ *              [--out FILE]
  *
  * The frame is a raw little-endian uint16 RGGB dump of W x H samples
  * (12-bit data, black 512, white 4095 of 65535, the format of the
  * reference's test frames), or without --raw a synthetic one: smooth
- * gradients with Gaussian noise of variance 2e-4 u + 4e-6. It is uploaded
+ * gradients with Gaussian noise of variance 2e-4 u + 4e-6 (with
+ * --highlights, 1.7 times brighter, so that its brighter part clips and
+ * the highlight stage has work). It is uploaded
  * once; the frames are recorded back to back on one stream (warm), or
  * with --cold each waited for before the next. --estimate-noise fits the
  * noise model on the device first and develops with it.
@@ -93,7 +96,7 @@ static double gaussian(void)
     return sqrt(-2.0 * log(u)) * cos(2.0 * M_PI * v);
 }
 
-static void synthetic_frame(uint16_t *m, int w, int h)
+static void synthetic_frame(uint16_t *m, int w, int h, double gain)
 {
     static const double wb[4] = { 2.0, 1.0, 1.0, 1.6 };
 
@@ -101,8 +104,8 @@ static void synthetic_frame(uint16_t *m, int w, int h)
         for (int x = 0; x < w; x++) {
             const int s = (y & 1) * 2 + (x & 1);
             const double fx = (double)x / w, fy = (double)y / h;
-            double u = (0.05 + 0.6 * fx * (0.4 + 0.6 * fy) +
-                        0.05 * sin(x * 0.21) * sin(y * 0.13)) / wb[s];
+            double u = gain * (0.05 + 0.6 * fx * (0.4 + 0.6 * fy) +
+                               0.05 * sin(x * 0.21) * sin(y * 0.13)) / wb[s];
             double raw;
 
             u += gaussian() * sqrt(2e-4 * (u > 0 ? u : 0) + 4e-6);
@@ -115,7 +118,8 @@ static void usage(void)
 {
     fprintf(stderr, "usage: craw_bench [--raw FILE] [--width W] [--height H] [--frames N] "
                     "[--warmup K] [--debayer rcd|mhc|bilinear] [--space xyz|709|2020|p3|ap0|ap1] "
-                    "[--denoise S] [--chroma S] [--temporal A] [--false-color N] [--awb] [--fp16] [--half] "
+                    "[--denoise S] [--chroma S] [--temporal A] [--false-color N] [--highlights] [--awb] "
+                    "[--fp16] [--half] "
                     "[--cold] [--estimate-noise] [--out FILE]\n");
 }
 
@@ -132,7 +136,7 @@ int main(int argc, char **argv)
 {
     const char *raw_path = NULL, *out_path = NULL;
     int w = 5760, h = 3240, frames = 100, warmup = 50, cold = 0, awb = 0, fp16 = 0, half = 0;
-    int estimate_noise = 0, false_color = 0;
+    int estimate_noise = 0, false_color = 0, highlights = 0;
     float denoise = 1.0f, chroma = 0.5f, temporal = 0.0f;
     enum CRAWDebayer debayer = CRAW_DEBAYER_RCD;
     enum CRAWColorSpace space = CRAW_CS_ACES_AP1_LINEAR;
@@ -167,6 +171,7 @@ int main(int argc, char **argv)
         else if (!strcmp(a, "--temporal") && v) temporal = (float)atof(argv[++i]);
         else if (!strcmp(a, "--false-color") && v) false_color = atoi(argv[++i]);
         else if (!strcmp(a, "--awb")) awb = 1;
+        else if (!strcmp(a, "--highlights")) highlights = 1;
         else if (!strcmp(a, "--fp16")) fp16 = 1;
         else if (!strcmp(a, "--half")) half = 1;
         else if (!strcmp(a, "--cold")) cold = 1;
@@ -204,7 +209,7 @@ int main(int argc, char **argv)
         }
         fclose(f);
     } else {
-        synthetic_frame(mosaic, w, h);
+        synthetic_frame(mosaic, w, h, highlights ? 1.7 : 1.0);
     }
 
     memset(&hp, 0, sizeof(hp));
@@ -250,6 +255,7 @@ int main(int argc, char **argv)
     fp.denoise_chroma_strength = chroma;
     fp.denoise_temporal_strength = temporal;
     fp.false_color_passes = (uint32_t)false_color;
+    fp.flags = highlights ? CRAW_FRAME_FLAG_HIGHLIGHT_RECONSTRUCTION : 0;
     fp.noise_override.a = 2e-4f;
     fp.noise_override.b = 4e-6f;
     if (estimate_noise && (denoise > 0.0f || temporal > 0.0f)) {
@@ -286,6 +292,7 @@ int main(int argc, char **argv)
                (chroma > 0.0f ? CRAW_PIPELINE_FLAG_CHROMA_DENOISE : 0) |
                (temporal > 0.0f ? CRAW_PIPELINE_FLAG_TEMPORAL_DENOISE : 0) |
                (false_color > 0 ? CRAW_PIPELINE_FLAG_FALSE_COLOR : 0) |
+               (highlights ? CRAW_PIPELINE_FLAG_HIGHLIGHT_RECONSTRUCTION : 0) |
                (fp16 ? CRAW_PIPELINE_FLAG_NLM_FP16 : 0);
     if ((ret = craw_pipeline_create(ctx, &pp, &pipe)) < 0) {
         fprintf(stderr, "craw_pipeline_create: %d\n", ret);
