@@ -73,6 +73,98 @@ KDEV float fc_mean9(float a0, float a1, float a2, float a3, float a4, float a5, 
     return ((((((((a0 + a1) + a2) + a3) + a4) + a5) + a6) + a7) + a8) * (1.0f / 9.0f);
 }
 
+/* Sweep S of a tile whose staged region lies wholly inside the frame (no
+ * mirror): a work item is a column and a run of FC_K rows, which load
+ * FC_K + 2 rows of three once; the median sorts each row's three once
+ * (min3 / med3 / max3) for the up to three outputs that use it, the box
+ * sum starts from the top row's (a + b) + c, the prefix of the stencil
+ * order, and adds the other six in that order. */
+#define FC_K 4
+
+template <int P, int S>
+KDEV void fc_sweep_inside(const float2 *cur, float2 *nxt)
+{
+    constexpr int R = 2 * P, LW = FC_TW + 2 * R;
+    constexpr int CW = FC_TW + 2 * (R - S), CH = FC_TH + 2 * (R - S);
+    constexpr int NCH = (CH + FC_K - 1) / FC_K;
+
+    for (int it = threadIdx.x; it < CW * NCH; it += FC_NT) {
+        const int lx = S + it % CW, ly0 = S + (it / CW) * FC_K;
+        float2 v[FC_K + 2][3];
+
+#pragma unroll
+        for (int j = 0; j < FC_K + 2; j++) {
+            /* a short last run reads no further than the sweep's last row
+             * below the region (which is inside the staged tile) */
+            const int r = min(ly0 - 1 + j, S + CH);
+            const float2 *row = cur + r * LW + lx;
+
+            v[j][0] = row[-1];
+            v[j][1] = row[0];
+            v[j][2] = row[1];
+        }
+        if constexpr (S & 1) {
+            float lo[FC_K + 2][2], md[FC_K + 2][2], hi[FC_K + 2][2];
+
+#pragma unroll
+            for (int j = 0; j < FC_K + 2; j++) {
+                lo[j][0] = fminf(fminf(v[j][0].x, v[j][1].x), v[j][2].x);
+                md[j][0] = __builtin_amdgcn_fmed3f(v[j][0].x, v[j][1].x, v[j][2].x);
+                hi[j][0] = fmaxf(fmaxf(v[j][0].x, v[j][1].x), v[j][2].x);
+                lo[j][1] = fminf(fminf(v[j][0].y, v[j][1].y), v[j][2].y);
+                md[j][1] = __builtin_amdgcn_fmed3f(v[j][0].y, v[j][1].y, v[j][2].y);
+                hi[j][1] = fmaxf(fmaxf(v[j][0].y, v[j][1].y), v[j][2].y);
+            }
+#pragma unroll
+            for (int j = 0; j < FC_K; j++) {
+                float o[2];
+
+                if (ly0 + j >= S + CH)
+                    break;
+#pragma unroll
+                for (int c = 0; c < 2; c++)
+                    o[c] = __builtin_amdgcn_fmed3f(
+                        fmaxf(fmaxf(lo[j][c], lo[j + 1][c]), lo[j + 2][c]),
+                        __builtin_amdgcn_fmed3f(md[j][c], md[j + 1][c], md[j + 2][c]),
+                        fminf(fminf(hi[j][c], hi[j + 1][c]), hi[j + 2][c]));
+                nxt[(ly0 + j) * LW + lx] = make_float2(o[0], o[1]);
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < FC_K; j++) {
+                float2 t;
+
+                if (ly0 + j >= S + CH)
+                    break;
+                t.x = (v[j][0].x + v[j][1].x) + v[j][2].x;
+                t.y = (v[j][0].y + v[j][1].y) + v[j][2].y;
+#pragma unroll
+                for (int k = 1; k < 3; k++)
+#pragma unroll
+                    for (int i = 0; i < 3; i++) {
+                        t.x = t.x + v[j + k][i].x;
+                        t.y = t.y + v[j + k][i].y;
+                    }
+                nxt[(ly0 + j) * LW + lx] = make_float2(t.x * (1.0f / 9.0f), t.y * (1.0f / 9.0f));
+            }
+        }
+    }
+}
+
+template <int P, int S>
+struct FcSweeps {
+    static KDEV void run(float2 *&cur, float2 *&nxt)
+    {
+        fc_sweep_inside<P, S>(cur, nxt);
+        __syncthreads();
+        float2 *t = cur;
+        cur = nxt;
+        nxt = t;
+        if constexpr (S < 2 * P)
+            FcSweeps<P, S + 1>::run(cur, nxt);
+    }
+};
+
 template <int P, bool CHROMA>
 __global__ __launch_bounds__(FC_NT) void k_false_color(KFrame f, KInter to)
 {
@@ -87,43 +179,62 @@ __global__ __launch_bounds__(FC_NT) void k_false_color(KFrame f, KInter to)
     const size_t plane = (size_t)W * H;
     const float *rp = f.inter.rgb, *gp = rp + plane, *bp = rp + 2 * plane;
 
-    /* local (lx, ly) stands for frame (x0 - R + lx, y0 - R + ly) */
-    for (int i = tid; i < G::BUF; i += FC_NT) {
-        const int lx = i % LW, ly = i / LW;
-        const size_t o = (size_t)fc_mirror(y0 - R + ly, H) * W + fc_mirror(x0 - R + lx, W);
-        const float g = gp[o];
+    /* the staged region (tile and apron) wholly inside the frame */
+    const bool inside = x0 >= R && y0 >= R && x0 + FC_TW + R <= W && y0 + FC_TH + R <= H;
 
-        cur[i] = make_float2(rp[o] - g, bp[o] - g);
+    /* local (lx, ly) stands for frame (x0 - R + lx, y0 - R + ly) */
+    if (inside) {
+        for (int i = tid; i < G::BUF; i += FC_NT) {
+            const int lx = i % LW, ly = i / LW;
+            const size_t o = (size_t)(y0 - R + ly) * W + x0 - R + lx;
+            const float g = gp[o];
+
+            cur[i] = make_float2(rp[o] - g, bp[o] - g);
+        }
+    } else {
+        for (int i = tid; i < G::BUF; i += FC_NT) {
+            const int lx = i % LW, ly = i / LW;
+            const size_t o = (size_t)fc_mirror(y0 - R + ly, H) * W + fc_mirror(x0 - R + lx, W);
+            const float g = gp[o];
+
+            cur[i] = make_float2(rp[o] - g, bp[o] - g);
+        }
     }
     __syncthreads();
 
+    if (inside) {
+        FcSweeps<P, 1>::run(cur, nxt);
+    } else {
+        /* near the frame's edges: a position outside the frame takes the
+         * sweep at the pixel it mirrors, on that pixel's neighbourhood */
 #pragma unroll
-    for (int s = 1; s <= 2 * P; s++) {
-        const int cw = xe + 2 * (R - s), ch = ye + 2 * (R - s);
+        for (int s = 1; s <= 2 * P; s++) {
+            const int cw = xe + 2 * (R - s), ch = ye + 2 * (R - s);
 
-        for (int i = tid; i < cw * ch; i += FC_NT) {
-            const int lx = s + i % cw, ly = s + i / cw;
-            const int cx = fc_mirror(x0 - R + lx, W) - x0 + R;
-            const int cy = fc_mirror(y0 - R + ly, H) - y0 + R;
-            const float2 *c = cur + cy * LW + cx;
-            const float2 v0 = c[-LW - 1], v1 = c[-LW], v2 = c[-LW + 1];
-            const float2 v3 = c[-1], v4 = c[0], v5 = c[1];
-            const float2 v6 = c[LW - 1], v7 = c[LW], v8 = c[LW + 1];
-            float2 r;
+            for (int i = tid; i < cw * ch; i += FC_NT) {
+                const int lx = s + i % cw, ly = s + i / cw;
+                const int cx = fc_mirror(x0 - R + lx, W) - x0 + R;
+                const int cy = fc_mirror(y0 - R + ly, H) - y0 + R;
+                const float2 *c = cur + cy * LW + cx;
+                const float2 v0 = c[-LW - 1], v1 = c[-LW], v2 = c[-LW + 1];
+                const float2 v3 = c[-1], v4 = c[0], v5 = c[1];
+                const float2 v6 = c[LW - 1], v7 = c[LW], v8 = c[LW + 1];
+                float2 r;
 
-            if (s & 1) {
-                r.x = fc_med9(v0.x, v1.x, v2.x, v3.x, v4.x, v5.x, v6.x, v7.x, v8.x);
-                r.y = fc_med9(v0.y, v1.y, v2.y, v3.y, v4.y, v5.y, v6.y, v7.y, v8.y);
-            } else {
-                r.x = fc_mean9(v0.x, v1.x, v2.x, v3.x, v4.x, v5.x, v6.x, v7.x, v8.x);
-                r.y = fc_mean9(v0.y, v1.y, v2.y, v3.y, v4.y, v5.y, v6.y, v7.y, v8.y);
+                if (s & 1) {
+                    r.x = fc_med9(v0.x, v1.x, v2.x, v3.x, v4.x, v5.x, v6.x, v7.x, v8.x);
+                    r.y = fc_med9(v0.y, v1.y, v2.y, v3.y, v4.y, v5.y, v6.y, v7.y, v8.y);
+                } else {
+                    r.x = fc_mean9(v0.x, v1.x, v2.x, v3.x, v4.x, v5.x, v6.x, v7.x, v8.x);
+                    r.y = fc_mean9(v0.y, v1.y, v2.y, v3.y, v4.y, v5.y, v6.y, v7.y, v8.y);
+                }
+                nxt[ly * LW + lx] = r;
             }
-            nxt[ly * LW + lx] = r;
+            __syncthreads();
+            float2 *t = cur;
+            cur = nxt;
+            nxt = t;
         }
-        __syncthreads();
-        float2 *t = cur;
-        cur = nxt;
-        nxt = t;
     }
 
     /* 2P sweeps leave the result in the first buffer; the tile's camera
