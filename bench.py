@@ -434,8 +434,10 @@ def measure_forms(args, ctx, capi, hip, P, scene, plain_value):
 
     b58 = {}
     for name, extra in (("fp32", 0), ("fp16", capi.CRAW_PIPELINE_FLAG_NLM_FP16)):
+        # warm-up: about 50 ms of work, as the plain form's, so that the
+        # GPU's clocks have risen before the timed batches
         r = time_batched(ctx, capi, hip, P, [(W, H)] * 4, mos, [fp_5p8k] * 4, f32, extra,
-                         max(3, args.steps // 4), 1, ap1)
+                         max(3, args.steps // 4), 12, ap1)
         r["vs_plain"] = round(r["mpix_s"] / plain_value, 2)
         b58[name] = r
         print("bench: batched 5.8K %s %.0f Mpix/s" % (name, r["mpix_s"]), file=sys.stderr,

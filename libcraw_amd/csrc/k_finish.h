@@ -13,6 +13,35 @@
 
 /* ------------------------------------------------------------ finish */
 
+/* One channel of an AWB block summed row by row from 0 (lib/awb.c:249-256).
+ * A whole 8x8 block is unrolled, so its 64 LDS reads are issued ahead of
+ * the adds rather than one behind each; the adds keep their order. */
+template <int TW, int TH>
+KDEV float kblock_sum(const float (&ch)[TH][TW], int bx, int by, int nx, int ny)
+{
+    float sum = 0.0f;
+
+    if (nx == 8 && ny == 8) {
+        float v[8][8];
+
+#pragma unroll
+        for (int y = 0; y < 8; y++)
+#pragma unroll
+            for (int x = 0; x < 8; x++)
+                v[y][x] = ch[by + y][bx + x];
+#pragma unroll
+        for (int y = 0; y < 8; y++)
+#pragma unroll
+            for (int x = 0; x < 8; x++)
+                sum += v[y][x];
+    } else {
+        for (int y = 0; y < ny; y++)
+            for (int x = 0; x < nx; x++)
+                sum += ch[by + y][bx + x];
+    }
+    return sum;
+}
+
 /* AWB samples of the tile's camera RGB (SoA in LDS) and the colour stage
  * into the destination. Expects the LDS tile complete (after a barrier).
  * Blocks are summed row by row as lib/awb.c:249-256 sums them, one lane of
@@ -38,9 +67,7 @@ KDEV void kfinish(const float (*rgbs)[TH][TW], int x0, int y0, int W, int H,
 
             if (gx0 < W && gy0 < H) {
                 for (int c = 0; c < 3; c++)
-                    for (int y = gy0; y < gy1; y++)
-                        for (int x = gx0; x < gx1; x++)
-                            acc[c] += rgbs[c][y - y0][x - x0];
+                    acc[c] = kblock_sum<TW, TH>(rgbs[c], gx0 - x0, gy0 - y0, gx1 - gx0, gy1 - gy0);
                 const bool valid = craw_awb_sample(acc, (float)((gy1 - gy0) * (gx1 - gx0)),
                                                    out.awb_wb0, out.awb_wb2, &smp);
                 out.samples[(size_t)(gy0 / 8) * ((W + 7) / 8) + gx0 / 8] =
@@ -59,11 +86,8 @@ KDEV void kfinish(const float (*rgbs)[TH][TW], int x0, int y0, int W, int H,
         CRAWAwbSample smp = { 0u, 0u, 0u };
         bool valid = false;
 
-        if (inside) {
-            for (int y = gy0; y < gy1; y++)
-                for (int x = gx0; x < gx1; x++)
-                    sum += rgbs[c][y - y0][x - x0];
-        }
+        if (inside)
+            sum = kblock_sum<TW, TH>(rgbs[c], gx0 - x0, gy0 - y0, gx1 - gx0, gy1 - gy0);
         acc[0] = sum;
         acc[1] = __shfl(sum, tid + NB, 64);
         acc[2] = __shfl(sum, tid + 2 * NB, 64);
