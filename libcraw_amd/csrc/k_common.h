@@ -160,6 +160,17 @@ struct KOut {
  * estimator skips. */
 #define KAWB_NONE 0xffffffffu
 
+/* A pointer to global memory as such (address space 1), for loads: a
+ * pointer read from memory rather than from the kernel's arguments has no
+ * known address space, and its loads would be flat ones, which every LDS
+ * wait then waits for too. */
+#define KGLOBAL(T) T __attribute__((address_space(1)))
+template <class T>
+KDEV KGLOBAL(T) *kglobal(T *p)
+{
+    return (KGLOBAL(T) *)p;
+}
+
 KDEV void kstore_rgba(const KOut &o, int x, int y, const float v[3])
 {
     char *row = (char *)o.dst + (size_t)y * o.pitch;

@@ -492,7 +492,7 @@ KDEV void dev_chroma_up_finish(SmUp &sm, const KFrame &f, int x0, int y0)
         const int c = i / (CU_QH * CU_QW), r = i % (CU_QH * CU_QW);
         const int lj = r / CU_QW, li = r % CU_QW;
         int gx = qbx + li, gy = qby + lj;
-        const float *srcp = (c < 2 ? it.down : it.blur) + (c & 1) * qplane;
+        const KGLOBAL(float) *srcp = kglobal(c < 2 ? it.down : it.blur) + (c & 1) * qplane;
 
         gx = gx > 0 ? (gx < it.wq - 1 ? gx : it.wq - 1) : 0;
         gy = gy > 0 ? (gy < it.hq - 1 ? gy : it.hq - 1) : 0;
@@ -509,9 +509,9 @@ KDEV void dev_chroma_up_finish(SmUp &sm, const KFrame &f, int x0, int y0)
 
         if (x >= W || y >= H)
             continue;
-        r = it.rgb[(size_t)y * W + x];
-        g = it.rgb[plane + (size_t)y * W + x];
-        b = it.rgb[2 * plane + (size_t)y * W + x];
+        r = kglobal(it.rgb)[(size_t)y * W + x];
+        g = kglobal(it.rgb)[plane + (size_t)y * W + x];
+        b = kglobal(it.rgb)[2 * plane + (size_t)y * W + x];
 
         /* lib/ref/ref_chroma.c:128-140: the taps' weights and (clamped)
          * cells; the clamp is in the staged cells */

@@ -100,10 +100,16 @@ KDEV void kfinish(const float (*rgbs)[TH][TW], int x0, int y0, int W, int H,
         }
     }
 
+    /* With the histogram the first wave's block sums are a serial chain,
+     * so the other waves take its share of the colour stage: the tile
+     * ends when the slower of the two is done rather than after both. */
+    const int first = out.hist ? 64 : 0;
+
+    if (tid < first)
+        return;
 #pragma unroll 1
-    for (int k = 0; k < TW * TH / NT; k++) {
-        int p = tid + k * NT, px = p % TW, py = p / TW;
-        int x = x0 + px, y = y0 + py;
+    for (int p = tid - first; p < TW * TH; p += NT - first) {
+        const int px = p % TW, py = p / TW, x = x0 + px, y = y0 + py;
         float o[3];
 
         if (x >= W || y >= H)
