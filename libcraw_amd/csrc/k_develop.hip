@@ -755,22 +755,6 @@ __global__ __launch_bounds__(CB_TX * CB_TY) void kb_chroma_blur(const KFrame *__
     }
 }
 
-__global__ __launch_bounds__(CU_NT) void kb_chroma_up_finish(const KFrame *__restrict__ frames,
-                                                             const uint32_t *__restrict__ tiles,
-                                                             uint32_t ntiles)
-{
-    __shared__ SmUp sm;
-
-    for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
-        const uint32_t d = tiles[t];
-        const KFrame &f = frames[KTILE_FRAME(d)];
-        const int ntx = (f.W + CU_TW - 1) / CU_TW, i = KTILE_IDX(d);
-
-        __syncthreads();
-        dev_chroma_up_finish(sm, f, (i % ntx) * CU_TW, (i / ntx) * CU_TH);
-    }
-}
-
 __global__ void k_zero_u32(uint32_t *p, size_t n)
 {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -914,12 +898,3 @@ hipError_t craw_launch_batch_awb_hist(const KFrame *frames, const uint32_t *tile
     return hipGetLastError();
 }
 
-hipError_t craw_launch_batch_chroma_up(const KFrame *frames, const uint32_t *tiles,
-                                       uint32_t ntiles, int grid, hipStream_t s)
-{
-    if (!ntiles)
-        return hipSuccess;
-    KLAUNCH(kb_chroma_up_finish, dim3(grid), dim3(CU_NT), 0, s, frames, tiles,
-                       ntiles);
-    return hipGetLastError();
-}

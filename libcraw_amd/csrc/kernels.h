@@ -404,7 +404,5 @@ hipError_t craw_launch_batch_chroma_blur(const KFrame *frames, const uint32_t *t
                                          uint32_t ntiles, int grid, hipStream_t s);
 hipError_t craw_launch_batch_awb_hist(const KFrame *frames, const uint32_t *tiles,
                                       uint32_t ntiles, int grid, hipStream_t s);
-hipError_t craw_launch_batch_chroma_up(const KFrame *frames, const uint32_t *tiles,
-                                       uint32_t ntiles, int grid, hipStream_t s);
 
 #endif /* CRAW_KERNELS_H */
