@@ -415,15 +415,21 @@ static int grid_for(CRAWBatch *b, int kind, size_t ntiles)
     return (int)g;
 }
 
-/* The stage launches of a planned recording, in order. */
+/* The stage launches of a planned recording, in order: with `part` 1 those
+ * a graph may hold (every launch reads the recording's tables, which are
+ * uploaded again before each replay), with 2 the rest (the chroma-up
+ * launches, which take a frame's constants as arguments, and the
+ * histogram after them), with 0 all. */
 static hipError_t batch_launch(CRAWBatch *b, const BatchPlan &bp, const size_t *list_off,
-                               hipStream_t s, CRAWProfRun *pr)
+                               hipStream_t s, CRAWProfRun *pr, int part = 0)
 {
     const uint32_t nb = (uint32_t)bp.plans.size();
     hipError_t e = hipSuccess;
 
 #define PB(name) do { if (pr) craw_profiler_begin(*pr, name); } while (0)
 #define PE() do { if (pr) craw_profiler_end(*pr); } while (0)
+    if (part == 2)
+        goto rest;
     if (b->cfg.awb) {
         PB("awb_clear");
         e = craw_launch_zero_u32(b->hist, (size_t)nb * CRAW_HIST_WORDS, s);
@@ -465,10 +471,14 @@ static hipError_t batch_launch(CRAWBatch *b, const BatchPlan &bp, const size_t *
                                           grid_for(b, 2, n), s);
         PE();
     }
+    if (part == 1)
+        return e;
+rest:
     if (e == hipSuccess && bp.lists[L_CU].size()) {
         /* one plain launch a frame, its constants in the kernel's
          * arguments: the persistent form of this stage, reading them from
-         * memory, ran at 0.26 ms a 5.8K frame against 0.16 */
+         * memory, ran at 0.26 ms a 5.8K frame against 0.16. A graph never
+         * holds these launches: their arguments change with the frames */
         PB("chroma_up_color");
         for (uint32_t i = 0; i < nb && e == hipSuccess; i++)
             if (bp.plans[i].chroma)
@@ -536,7 +546,7 @@ static int batch_record(CRAWBatch *b, hipStream_t s, const CRAWBatchFrame *frame
 
             e = hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed);
             if (e == hipSuccess) {
-                hipError_t le = batch_launch(b, bp, list_off, s, NULL);
+                hipError_t le = batch_launch(b, bp, list_off, s, NULL, 1);
                 e = hipStreamEndCapture(s, &g);
                 if (le != hipSuccess)
                     e = le;
@@ -563,9 +573,13 @@ static int batch_record(CRAWBatch *b, hipStream_t s, const CRAWBatchFrame *frame
             craw_profiler_begin(pr, "batch_graph", false);
             e = hipGraphLaunch(b->exec, s);
             craw_profiler_end(pr);
+            if (e == hipSuccess)
+                e = batch_launch(b, bp, list_off, s, &pr, 2);
             craw_profiler_finish(pr);
         } else {
             e = hipGraphLaunch(b->exec, s);
+            if (e == hipSuccess)
+                e = batch_launch(b, bp, list_off, s, NULL, 2);
         }
     } else if (b->prof.on) {
         CRAWProfRun pr = craw_profiler_start(&b->prof, s);

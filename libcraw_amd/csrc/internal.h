@@ -55,7 +55,7 @@ struct CRAWStageConfig {
 struct CRAWFrameBuffers {
     uint16_t *clean;            /* W x H, when denoising */
     float *inter_rgb;           /* 3 x W x H, with chroma denoise or false colour */
-    float *inter_rgb2;          /* the same, false colour's output for the chroma stage */
+    float *inter_rgb2;          /* the debayer's planes for false colour before chroma */
     uint16_t *hl_mosaic;        /* W x H R16F, with highlight reconstruction */
     float4 *hl_levels;          /* the colour field's levels, one after another */
     float *down, *blur;         /* 3 and 2 quarter planes */
@@ -73,7 +73,7 @@ struct CRAWFramePlan {
     int fc_passes;              /* false colour passes, 0: none */
     bool hl;                    /* highlight reconstruction before the debayer */
     KHighlight hl_k;
-    KInter fc_to;               /* where the false colour stage writes for the chroma stage */
+    KInter fc_to;               /* the false colour stage's output: the chroma stage's inputs */
     int t_read, t_write;        /* temporal state sets, -1: none */
     bool temporal, replay;
     CRAWTemporalState *ts;

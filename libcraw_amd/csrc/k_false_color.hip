@@ -177,7 +177,7 @@ __global__ __launch_bounds__(FC_NT) void k_false_color(KFrame f, KInter to)
     const int x0 = blockIdx.x * FC_TW, y0 = blockIdx.y * FC_TH;
     const int xe = min(x0 + FC_TW, W) - x0, ye = min(y0 + FC_TH, H) - y0;
     const size_t plane = (size_t)W * H;
-    const float *rp = f.inter.rgb, *gp = rp + plane, *bp = rp + 2 * plane;
+    const float *rp = f.inter.rgb_out, *gp = rp + plane, *bp = rp + 2 * plane;
 
     /* the staged region (tile and apron) wholly inside the frame */
     const bool inside = x0 >= R && y0 >= R && x0 + FC_TW + R <= W && y0 + FC_TH + R <= H;

@@ -120,9 +120,9 @@ KDEV void kfinish(const float (*rgbs)[TH][TW], int x0, int y0, int W, int H,
 }
 
 /* Camera RGB of the tile to the chroma stage's inputs: the three planes
- * and (unless it.down is NULL: a stage before the chroma stage's own input
- * comes next) the 4x4 block means of (R - G, B - G, G), summed row by row
- * over the pixels inside the frame (lib/ref/ref_chroma.c:59-82). */
+ * (to it.rgb_out) and, with it.means, the 4x4 block means of (R - G,
+ * B - G, G), summed row by row over the pixels inside the frame
+ * (lib/ref/ref_chroma.c:59-82). */
 template <int TW, int TH, int NT>
 KDEV void kstore_inter(const float (*rgbs)[TH][TW], int x0, int y0, int W, int H,
                        const KInter &it)
@@ -137,12 +137,12 @@ KDEV void kstore_inter(const float (*rgbs)[TH][TW], int x0, int y0, int W, int H
 
         if (x >= W || y >= H)
             continue;
-        it.rgb[(size_t)y * W + x] = rgbs[0][py][px];
-        it.rgb[plane + (size_t)y * W + x] = rgbs[1][py][px];
-        it.rgb[2 * plane + (size_t)y * W + x] = rgbs[2][py][px];
+        it.rgb_out[(size_t)y * W + x] = rgbs[0][py][px];
+        it.rgb_out[plane + (size_t)y * W + x] = rgbs[1][py][px];
+        it.rgb_out[2 * plane + (size_t)y * W + x] = rgbs[2][py][px];
     }
 
-    if (!it.down)
+    if (!it.means)
         return;
     for (int b = tid; b < (TW / 4) * (TH / 4); b += NT) {
         int qx = x0 / 4 + b % (TW / 4), qy = y0 / 4 + b / (TW / 4);

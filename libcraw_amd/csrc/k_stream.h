@@ -103,8 +103,8 @@ KDEV void rs_finish(float (&acc)[3], const KFrame &f, int oy, bool out_cols, int
  * the 4x4 block sums of R - G, B - G, G in raster order (the block's first
  * lane, column X0 % 4 == 0, adds its two columns and then its right
  * neighbour's; lib/ref/ref_chroma.c:59-82), stored as means on the
- * block's last row unless f.inter.down is NULL (the false colour stage,
- * not the chroma stage, reads the planes next). */
+ * block's last row with f.inter.means (without it the false colour
+ * stage, not the chroma stage, reads the planes next). */
 KDEV void rs_store_inter(float (&acc)[3], const KFrame &f, int oy, bool out_cols, int X0,
                          const float r[2], const float g[2], const float b[2])
 {
@@ -113,9 +113,9 @@ KDEV void rs_store_inter(float (&acc)[3], const KFrame &f, int oy, bool out_cols
 
     if (out_cols) {
         const size_t o = (size_t)oy * f.W + X0;
-        *(float2 *)&it.rgb[o] = make_float2(r[0], r[1]);
-        *(float2 *)&it.rgb[plane + o] = make_float2(g[0], g[1]);
-        *(float2 *)&it.rgb[2 * plane + o] = make_float2(b[0], b[1]);
+        *(float2 *)&it.rgb_out[o] = make_float2(r[0], r[1]);
+        *(float2 *)&it.rgb_out[plane + o] = make_float2(g[0], g[1]);
+        *(float2 *)&it.rgb_out[2 * plane + o] = make_float2(b[0], b[1]);
     }
     {
         const float d0[2] = { r[0] - g[0], r[1] - g[1] };
@@ -137,7 +137,7 @@ KDEV void rs_store_inter(float (&acc)[3], const KFrame &f, int oy, bool out_cols
         acc[2] = acc[2] + g[1];
         acc[2] = acc[2] + n20;
         acc[2] = acc[2] + n21;
-        if ((oy & 3) == 3 && out_cols && (X0 & 3) == 0 && it.down) {
+        if ((oy & 3) == 3 && out_cols && (X0 & 3) == 0 && it.means) {
             const size_t q = (size_t)(oy / 4) * it.wq + X0 / 4;
             const size_t qplane = (size_t)it.wq * it.hq;
 

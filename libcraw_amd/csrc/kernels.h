@@ -38,12 +38,17 @@ void craw_launch_events(hipEvent_t *start, hipEvent_t *stop);
 
 /* The chroma stage's buffers: camera RGB as three float planes of the
  * frame, the quarter grid (ceil(W/4) x ceil(H/4)) of block means as three
- * planes (R - G, B - G, G), and its bilateral result as two. */
+ * planes (R - G, B - G, G), and its bilateral result as two. The debayer
+ * writes its planes to rgb_out, and the quarter means only with `means`:
+ * with the false colour stage after it, rgb_out is that stage's input and
+ * the stage writes rgb and the means (rgb_out == rgb otherwise). */
 struct KInter {
     float *rgb;
     float *down;
     float *blur;
     int wq, hq;
+    float *rgb_out;
+    int means;
 };
 
 /* Constants of the raw-domain denoise (lib/pipeline.c:1414-1467 and

@@ -269,17 +269,12 @@ static int pipeline_enqueue(CRAWPipeline *p, const CRAWFramePlan *plan, hipStrea
         craw_profiler_end(pr);
     }
     if (e == hipSuccess && plan->chroma) {
-        KFrame fc = f;
-
-        /* after the false colour stage the chroma stage reads what it wrote */
-        if (plan->fc_passes)
-            fc.inter = plan->fc_to;
         craw_profiler_begin(pr, "chroma_blur");
-        e = craw_launch_chroma_blur(fc, stream);
+        e = craw_launch_chroma_blur(f, stream);
         craw_profiler_end(pr);
         if (e == hipSuccess) {
             craw_profiler_begin(pr, "chroma_up_color");
-            e = craw_launch_chroma_up_finish(fc, stream);
+            e = craw_launch_chroma_up_finish(f, stream);
             craw_profiler_end(pr);
         }
     }

@@ -283,6 +283,8 @@ int craw_frame_plan(CRAWContext *ctx, const CRAWStageConfig &cfg, const CRAWFram
     plan->f.inter.blur = buf.blur;
     plan->f.inter.wq = (int)((w + 3) / 4);
     plan->f.inter.hq = (int)((h + 3) / 4);
+    plan->f.inter.rgb_out = buf.inter_rgb;
+    plan->f.inter.means = 1;
 
     /* lib/pipeline.c:1820-1836: zero passes leave the frame alone, with or
      * without the stage; more than the stage has are refused */
@@ -296,12 +298,13 @@ int craw_frame_plan(CRAWContext *ctx, const CRAWStageConfig &cfg, const CRAWFram
         if (!buf.inter_rgb || (plan->chroma && !buf.inter_rgb2))
             return CRAW_ERROR(EINVAL);
         plan->fc_passes = (int)fp->false_color_passes;
-        /* the debayer leaves the planes alone (no quarter means); the
-         * false colour stage writes the chroma stage's inputs, which the
-         * chroma stage then reads */
+        /* the debayer writes its planes (no quarter means) for the false
+         * colour stage, which writes the chroma stage's inputs (the planes
+         * and the means) where the chroma stage reads them */
         plan->fc_to = plan->f.inter;
-        plan->fc_to.rgb = buf.inter_rgb2;
-        plan->f.inter.down = NULL;
+        if (plan->chroma)
+            plan->f.inter.rgb_out = buf.inter_rgb2;
+        plan->f.inter.means = 0;
     }
 
     /* Colour: the frame's matrix and scale, then XYZ to the output space,

@@ -170,6 +170,25 @@ def test_graph_replays_follow_new_inputs(gpu_available):
     check_against_plain(b, fr, fps, mos, FULL, capi.CRAW_DEBAYER_RCD, "after reshape")
 
 
+def test_graph_replays_follow_new_buffers_and_params(gpu_available):
+    """Replays of one captured shape with other destination buffers and
+    other frame parameters each time (what a server's batches see): every
+    frame still develops as the plain form develops it."""
+    prof = scene.make_profile()
+    sizes = SIZES[:3]
+    b = new_batch(sizes, FULL, graphs=True)
+    stream = hip.Stream()
+    for rep in range(3):
+        fr = Frames(sizes)
+        fps = [params_for(prof, i + rep) for i in range(3)]
+        mos = mosaics(sizes, 400 + 10 * rep)
+        fr.upload(mos)
+        frames = [batch_frame(*fr.images(i), fps[i]) for i in range(3)]
+        assert b.record(stream, frames) == 0
+        stream.synchronize()
+        check_against_plain(b, fr, fps, mos, FULL, capi.CRAW_DEBAYER_RCD, "replay %d" % rep)
+
+
 def test_temporal_streams(gpu_available):
     """Two clips of different sizes, interleaved in one batch per time step,
     each equal to a plain pipeline developing that clip alone; a reset and a
