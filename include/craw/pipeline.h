@@ -28,9 +28,7 @@
  *
  * The reference's lens, defringe and sharpening stages are not part of
  * this library; their pipeline flags make craw_pipeline_create() fail with
- * CRAW_ERROR(ENOTSUP) and their frame fields are ignored. The highlight and
- * false colour stages are in the plain form only (craw_batch_create()
- * refuses their flags with CRAW_ERROR(ENOTSUP)).
+ * CRAW_ERROR(ENOTSUP) and their frame fields are ignored.
  */
 #ifndef CRAW_PIPELINE_H
 #define CRAW_PIPELINE_H

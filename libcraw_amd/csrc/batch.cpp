@@ -47,7 +47,9 @@ struct CRAWBatch {
 
     /* arenas */
     uint16_t *clean;
-    float *inter_rgb, *down, *blur;
+    float *inter_rgb, *inter_rgb2, *down, *blur;
+    uint16_t *hl_mosaic;
+    float4 *hl_levels;
     uint32_t *hist;
     uint2 *awb_samples;
     uint32_t *hist_host;    /* pinned, max_frames histograms */
@@ -134,6 +136,9 @@ CRAW_API void craw_batch_free(CRAWBatch **pb)
     }
     CRAW_IGNORE(hipFree(b->clean));
     CRAW_IGNORE(hipFree(b->inter_rgb));
+    CRAW_IGNORE(hipFree(b->inter_rgb2));
+    CRAW_IGNORE(hipFree(b->hl_mosaic));
+    CRAW_IGNORE(hipFree(b->hl_levels));
     CRAW_IGNORE(hipFree(b->down));
     CRAW_IGNORE(hipFree(b->blur));
     CRAW_IGNORE(hipFree(b->hist));
@@ -151,12 +156,13 @@ CRAW_API void craw_batch_free(CRAWBatch **pb)
 
 /* Bytes of each device buffer a batch of `params` allocates. */
 struct BatchSizes {
-    size_t clean, inter_rgb, down, blur, hist, awb_samples, frames, tiles_cap;
+    size_t clean, inter_rgb, inter_rgb2, down, blur, hl_mosaic, hl_levels, hist, awb_samples,
+        frames, tiles_cap;
 
     size_t device() const
     {
-        return clean + inter_rgb + down + blur + hist + awb_samples + frames +
-               tiles_cap * sizeof(uint32_t);
+        return clean + inter_rgb + inter_rgb2 + down + blur + hl_mosaic + hl_levels + hist +
+               awb_samples + frames + tiles_cap * sizeof(uint32_t);
     }
 };
 
@@ -175,11 +181,6 @@ static int batch_sizes(CRAWContext *ctx, const CRAWBatchParams *params, CRAWStag
                             params->out_format, cfg);
     if (ret < 0)
         return ret;
-    if (cfg->has_fc || cfg->has_hl) {
-        craw_log(ctx, CRAW_LOG_ERROR, "The batched form has no false colour or highlight "
-                 "stage; record such frames with craw_pipeline_record()\n");
-        return CRAW_ERROR(ENOTSUP);
-    }
     fast = cfg->fp16 || cfg->fast;
     px = params->max_pixels;
     /* quarter grid of every frame, with a row and a column of rounding */
@@ -188,9 +189,20 @@ static int batch_sizes(CRAWContext *ctx, const CRAWBatchParams *params, CRAWStag
     if (cfg->has_denoise || cfg->has_temporal)
         sz->clean = px * sizeof(uint16_t);
     if (cfg->has_chroma) {
-        sz->inter_rgb = 3 * px * sizeof(float);
         sz->down = 3 * qpx * sizeof(float);
         sz->blur = 2 * qpx * sizeof(float);
+    }
+    if (cfg->has_chroma || cfg->has_fc)
+        sz->inter_rgb = 3 * px * sizeof(float);
+    if (cfg->has_chroma && cfg->has_fc)
+        sz->inter_rgb2 = 3 * px * sizeof(float);
+    if (cfg->has_hl) {
+        /* a frame's field levels hold at most w h / 12 + (w + h) / 2 + 16
+         * texels (each level a quarter of the one below, plus a row and a
+         * column of rounding) */
+        sz->hl_mosaic = px * sizeof(uint16_t);
+        sz->hl_levels = (px / 12 + (size_t)params->max_frames *
+                         ((params->max_width + params->max_height) / 2 + 16)) * sizeof(float4);
     }
     if (cfg->awb) {
         sz->hist = (size_t)params->max_frames * CRAW_HIST_WORDS * 4;
@@ -255,6 +267,8 @@ CRAW_API int craw_batch_create(CRAWContext *ctx, const CRAWBatchParams *params, 
     ret = CRAW_ERROR(ENOMEM);
 #define DALLOC(ptr, bytes) ((bytes) && hipMalloc((void **)&(ptr), (bytes)) != hipSuccess)
     if (DALLOC(b->clean, sz.clean) || DALLOC(b->inter_rgb, sz.inter_rgb) ||
+        DALLOC(b->inter_rgb2, sz.inter_rgb2) || DALLOC(b->hl_mosaic, sz.hl_mosaic) ||
+        DALLOC(b->hl_levels, sz.hl_levels) ||
         DALLOC(b->down, sz.down) || DALLOC(b->blur, sz.blur) || DALLOC(b->hist, sz.hist) ||
         DALLOC(b->awb_samples, sz.awb_samples) ||
         DALLOC(b->frames, sz.frames) || DALLOC(b->tiles, sz.tiles_cap * sizeof(uint32_t)))
@@ -300,7 +314,7 @@ struct BatchPlan {
 
 static int batch_plan(CRAWBatch *b, const CRAWBatchFrame *frames, uint32_t nb, BatchPlan &bp)
 {
-    size_t off_px = 0, off_q = 0, off_awb = 0;
+    size_t off_px = 0, off_q = 0, off_awb = 0, off_lv = 0;
 
     if (!nb || nb > b->params.max_frames)
         return CRAW_ERROR(ENOSPC);
@@ -319,6 +333,9 @@ static int batch_plan(CRAWBatch *b, const CRAWBatchFrame *frames, uint32_t nb, B
             return CRAW_ERROR(ENOSPC);
         buf.clean = b->clean ? b->clean + off_px : NULL;
         buf.inter_rgb = b->inter_rgb ? b->inter_rgb + 3 * off_px : NULL;
+        buf.inter_rgb2 = b->inter_rgb2 ? b->inter_rgb2 + 3 * off_px : NULL;
+        buf.hl_mosaic = b->hl_mosaic ? b->hl_mosaic + off_px : NULL;
+        buf.hl_levels = b->hl_levels ? b->hl_levels + off_lv : NULL;
         buf.down = b->down ? b->down + 3 * off_q : NULL;
         buf.blur = b->blur ? b->blur + 2 * off_q : NULL;
         buf.hist = b->hist ? b->hist + (size_t)i * CRAW_HIST_WORDS : NULL;
@@ -330,6 +347,14 @@ static int batch_plan(CRAWBatch *b, const CRAWBatchFrame *frames, uint32_t nb, B
             return ret;
         off_px += (size_t)w * h;
         off_q += q;
+        if (b->hl_levels) {
+            int lw[KHL_LEVELS], lh[KHL_LEVELS];
+            size_t off[KHL_LEVELS], texels;
+
+            if (craw_hl_levels(w, h, lw, lh, off, &texels) < 0)
+                return CRAW_ERROR(EINVAL);
+            off_lv += texels;
+        }
         off_awb += craw_awb_blocks(w, h);
     }
     /* one temporal state may appear once per recording */
@@ -340,6 +365,8 @@ static int batch_plan(CRAWBatch *b, const CRAWBatchFrame *frames, uint32_t nb, B
 
     for (uint32_t i = 0; i < nb; i++) {
         const CRAWFramePlan &pl = bp.plans[i];
+        /* the debayer writes camera RGB planes for a later stage */
+        const bool inter = pl.chroma || pl.fc_passes;
         int W = pl.f.W, H = pl.f.H, ntx, n;
 
         if (pl.dn_mode) {
@@ -356,15 +383,15 @@ static int batch_plan(CRAWBatch *b, const CRAWBatchFrame *frames, uint32_t nb, B
             /* RCD: ring tiles and interior strips */
             n = craw_stage_tiles(KSTAGE_RCD_RING, W, H, 2, 0, &ntx);
             for (int t = 0; t < n; t++)
-                bp.lists[pl.chroma ? L_DB_INTER : L_DB_OUT].push_back(KTILE(i, t));
-            n = craw_stage_tiles(pl.chroma ? KSTAGE_RCD_STREAM : KSTAGE_RCD_STREAM_F, W, H, 2, 0,
+                bp.lists[inter ? L_DB_INTER : L_DB_OUT].push_back(KTILE(i, t));
+            n = craw_stage_tiles(inter ? KSTAGE_RCD_STREAM : KSTAGE_RCD_STREAM_F, W, H, 2, 0,
                                  &ntx);
             for (int t = 0; t < n; t++)
-                bp.lists[pl.chroma ? L_RS : L_RS_F].push_back(KTILE(i, t));
+                bp.lists[inter ? L_RS : L_RS_F].push_back(KTILE(i, t));
         } else {
             n = craw_stage_tiles(KSTAGE_DEBAYER, W, H, pl.f.method, 0, &ntx);
             for (int t = 0; t < n; t++)
-                bp.lists[pl.chroma ? L_DB_INTER : L_DB_OUT].push_back(KTILE(i, t));
+                bp.lists[inter ? L_DB_INTER : L_DB_OUT].push_back(KTILE(i, t));
         }
         if (pl.chroma) {
             n = craw_stage_tiles(KSTAGE_CHROMA_BLUR, W, H, pl.f.method, 0, &ntx);
@@ -381,7 +408,8 @@ static int batch_plan(CRAWBatch *b, const CRAWBatchFrame *frames, uint32_t nb, B
         }
         bp.key.push_back((uint32_t)W);
         bp.key.push_back((uint32_t)H);
-        bp.key.push_back((uint32_t)pl.dn_mode | (pl.chroma ? 0x100u : 0u));
+        bp.key.push_back((uint32_t)pl.dn_mode | (pl.chroma ? 0x100u : 0u) |
+                         (pl.hl ? 0x200u : 0u) | ((uint32_t)pl.fc_passes << 12));
     }
     {
         size_t total = 0;
@@ -445,6 +473,27 @@ static hipError_t batch_launch(CRAWBatch *b, const BatchPlan &bp, const size_t *
                                       (uint32_t)n, grid_for(b, 0, n), s);
         PE();
     }
+    /* highlight reconstruction and false colour: plain launches a frame
+     * (such recordings are never captured into a graph), each step over
+     * all frames one stage */
+    {
+        bool any = false;
+
+        for (uint32_t i = 0; i < nb; i++)
+            any = any || bp.plans[i].hl;
+        for (int step = 0; step < 3 && any && e == hipSuccess; step++) {
+            static const char *names[3] = { "highlight_field", "highlight_push",
+                                            "highlight_fill" };
+
+            PB(names[step]);
+            for (uint32_t i = 0; i < nb && e == hipSuccess; i++)
+                if (bp.plans[i].hl)
+                    e = step == 0 ? craw_launch_hl_field(bp.plans[i].hl_k, s) :
+                        step == 1 ? craw_launch_hl_push(bp.plans[i].hl_k, s) :
+                                    craw_launch_hl_fill(bp.plans[i].hl_k, s);
+            PE();
+        }
+    }
     for (int l = L_DB_OUT; l <= L_DB_INTER && e == hipSuccess; l++) {
         size_t n = bp.lists[l].size();
         static const char *names[3] = { "debayer_bilinear", "debayer_mhc", "rcd" };
@@ -463,6 +512,22 @@ static hipError_t batch_launch(CRAWBatch *b, const BatchPlan &bp, const size_t *
         e = craw_launch_batch_rcd_stream(l == L_RS, b->frames, b->tiles + list_off[l],
                                          (uint32_t)n, grid_for(b, 1, n), s);
         PE();
+    }
+    {
+        bool any = false;
+
+        for (uint32_t i = 0; i < nb; i++)
+            any = any || bp.plans[i].fc_passes;
+        if (any && e == hipSuccess) {
+            PB("false_color");
+            for (uint32_t i = 0; i < nb && e == hipSuccess; i++) {
+                const CRAWFramePlan &pl = bp.plans[i];
+
+                if (pl.fc_passes)
+                    e = craw_launch_false_color(pl.fc_passes, pl.chroma, pl.f, pl.fc_to, s);
+            }
+            PE();
+        }
     }
     if (e == hipSuccess && bp.lists[L_CB].size()) {
         size_t n = bp.lists[L_CB].size();
@@ -539,6 +604,11 @@ static int batch_record(CRAWBatch *b, hipStream_t s, const CRAWBatchFrame *frame
     stg->used = true;
     b->ring_pos = (b->ring_pos + 1) % RING;
 
+    /* a recording with a stage that takes a frame's constants as launch
+     * arguments before the chroma-up stage is launched directly */
+    for (uint32_t i = 0; i < nb && graphs; i++)
+        if (bp.plans[i].hl || bp.plans[i].fc_passes)
+            graphs = false;
     if (graphs) {
         if (!b->exec || bp.key != b->graph_key) {
             hipGraph_t g = NULL;

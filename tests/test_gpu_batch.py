@@ -189,6 +189,36 @@ def test_graph_replays_follow_new_buffers_and_params(gpu_available):
         check_against_plain(b, fr, fps, mos, FULL, capi.CRAW_DEBAYER_RCD, "replay %d" % rep)
 
 
+@pytest.mark.parametrize("graphs", [False, True])
+def test_highlights_and_false_colour(gpu_available, graphs):
+    """Frames of uneven sizes with and without highlight reconstruction and
+    with 0-2 false colour passes, before and without the chroma stage, in
+    one recording (a graph is not used for such a recording even when
+    asked for): each frame as the plain form develops it."""
+    flags = (capi.CRAW_PIPELINE_FLAG_CHROMA_DENOISE | capi.CRAW_PIPELINE_FLAG_AWB_HISTOGRAM |
+             capi.CRAW_PIPELINE_FLAG_HIGHLIGHT_RECONSTRUCTION |
+             capi.CRAW_PIPELINE_FLAG_FALSE_COLOR)
+    prof = scene.make_profile()
+    sizes = SIZES[:4]
+    fr = Frames(sizes)
+    mos = mosaics(sizes, 500)
+    for m in mos:
+        h, w = m.shape
+        m[h // 4:h // 2, w // 3:w // 2] = 4095           # blown
+    fps = [frame_params(prof, flags=capi.CRAW_FRAME_FLAG_HIGHLIGHT_RECONSTRUCTION if i % 2 == 0
+                        else 0, false_color_passes=i % 3,
+                        denoise_chroma_strength=0.5 if i != 1 else 0.0)
+           for i in range(len(sizes))]
+    b = new_batch(sizes, flags, graphs=graphs)
+    stream = hip.Stream()
+    frames = [batch_frame(*fr.images(i), fps[i]) for i in range(len(sizes))]
+    for rep in range(2):
+        fr.upload(mos)
+        assert b.record(stream, frames) == 0
+        stream.synchronize()
+        check_against_plain(b, fr, fps, mos, flags, capi.CRAW_DEBAYER_RCD, "rep %d" % rep)
+
+
 def test_temporal_streams(gpu_available):
     """Two clips of different sizes, interleaved in one batch per time step,
     each equal to a plain pipeline developing that clip alone; a reset and a

@@ -4,16 +4,16 @@ the AWB histogram bit for bit the oracle's (oracle.f32d.process) for one to
 five passes, every debayer, with and without the chroma stage after it, at
 sizes that leave partial 64x32 tiles and at the 8x8 minimum (the apron then
 mirrors more than once); both ways of switching it off reproduce the plain
-chain; six passes are refused; the batched form refuses the flag."""
+chain; six passes are refused (the batched form: test_gpu_batch.py)."""
 import ctypes
 
 import numpy as np
 import pytest
 
 from libcraw_amd import capi, scene
-from libcraw_amd.pipeline import Batch, CrawError, batch_params, frame_params
+from libcraw_amd.pipeline import frame_params
 
-from gpu_util import Runner, assert_bit_exact, context
+from gpu_util import Runner, assert_bit_exact
 from test_gpu_parity import _check
 
 pytestmark = pytest.mark.gpu
@@ -90,9 +90,3 @@ def test_too_many_passes(gpu_available):
     run = Runner(w, h, flags=FC)
     assert run.run(bayer, frame_params(prof, false_color_passes=6)) == -22
     _check(run, bayer, frame_params(prof, false_color_passes=5), "after a refused record")
-
-
-def test_batched_form_refuses(gpu_available):
-    with pytest.raises(CrawError) as e:
-        Batch(context(), batch_params(2, 128, 128, 2 * 128 * 128, flags=FC))
-    assert e.value.code == -95

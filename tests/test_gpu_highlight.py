@@ -5,16 +5,16 @@ reference's highlight scene (G-only and fully blown blobs), with every
 debayer, after the denoise, before the false colour and chroma stages, at
 sizes from the 8x8 minimum (a two-level field) to ones with ten levels
 and partial tiles everywhere; a scene with nothing clipped; both ways of
-switching it off reproduce the plain chain; the batched form refuses the
-flag."""
+switching it off reproduce the plain chain (the batched form:
+test_gpu_batch.py)."""
 import numpy as np
 import pytest
 
 from libcraw_amd import capi, scene
-from libcraw_amd.pipeline import Batch, CrawError, batch_params, frame_params
+from libcraw_amd.pipeline import frame_params
 
 import hl_scene as sc
-from gpu_util import Runner, assert_bit_exact, context
+from gpu_util import Runner, assert_bit_exact
 from test_gpu_parity import _check
 
 pytestmark = pytest.mark.gpu
@@ -80,9 +80,3 @@ def test_off_both_ways(gpu_available):
         run = Runner(sc.W, sc.H, debayer=capi.CRAW_DEBAYER_MHC, flags=flags)
         assert run.run(m, frame_params(prof, flags=fflags)) == 0
         assert_bit_exact(run.output(), want, "off: flags %d frame flags %d" % (flags, fflags))
-
-
-def test_batched_form_refuses(gpu_available):
-    with pytest.raises(CrawError) as e:
-        Batch(context(), batch_params(2, 128, 128, 2 * 128 * 128, flags=HL))
-    assert e.value.code == -95
