@@ -103,6 +103,33 @@ def test_still_alone_and_in_the_mix(gpu_available):
     srv.close()
 
 
+def test_highlights_and_false_colour_served(gpu_available):
+    """Requests with highlight reconstruction and false colour, mixed with
+    ones without, in one server's batches: each frame as the plain form
+    develops it."""
+    prof = scene.make_profile()
+    flags = (capi.CRAW_PIPELINE_FLAG_CHROMA_DENOISE | capi.CRAW_PIPELINE_FLAG_AWB_HISTOGRAM |
+             capi.CRAW_PIPELINE_FLAG_HIGHLIGHT_RECONSTRUCTION |
+             capi.CRAW_PIPELINE_FLAG_FALSE_COLOR)
+    jobs = []
+    for i, (w, h) in enumerate([(202, 122), (96, 64), (256, 160), (130, 66)]):
+        m = scene.make_frame(w, h, seed=600 + i)
+        m[h // 4:h // 2, w // 3:w // 2] = 4095
+        fp = frame_params(prof, flags=capi.CRAW_FRAME_FLAG_HIGHLIGHT_RECONSTRUCTION
+                          if i % 2 == 0 else 0, false_color_passes=(i + 1) % 3,
+                          denoise_chroma_strength=0.5)
+        jobs.append(Job(m, fp))
+    srv = new_server(flags=flags)
+    assert srv.submit_n([j.request(i) for i, j in enumerate(jobs)]) == 0
+    done = wait_all(srv, len(jobs))
+    assert all(c["status"] == 0 for c in done.values())
+    for i, j in enumerate(jobs):
+        out, hist = plain_of(j, flags)
+        assert_bit_exact(j.output(), out, "request %d" % i)
+        assert np.array_equal(j.hist, hist)
+    srv.close()
+
+
 def test_clip_streams_in_order(gpu_available):
     """Three clips, five frames each, all submitted before any completes;
     one clip resets at its third frame. Each clip equals a plain pipeline
