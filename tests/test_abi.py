@@ -106,3 +106,17 @@ def test_null_arguments():
     h = capi.CRAWAwbHistogram()
     assert lib.craw_pipeline_get_awb_histogram(None, ctypes.byref(h)) == -capi.EINVAL
     assert lib.craw_pipeline_get_stage_times(None, None, 0) == -capi.EINVAL
+
+
+def test_bench_driver_built_and_parses_arguments():
+    """The command-line bench driver is built beside the library and refuses
+    bad arguments with its usage before it touches a device."""
+    import os
+    import subprocess
+
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                       "libcraw_amd", "bench", "craw_bench")
+    assert os.path.exists(exe), "craw_bench is not built (make -C libcraw_amd/csrc bench)"
+    for args in (["--no-such-flag"], ["--width", "7"], ["--frames", "0"]):
+        r = subprocess.run([exe, *args], capture_output=True, text=True, timeout=30)
+        assert r.returncode == 2 and "usage: craw_bench" in r.stderr, (args, r.stderr)
