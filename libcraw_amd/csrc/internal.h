@@ -119,13 +119,14 @@ int craw_stage_config(CRAWContext *ctx, uint32_t flags, enum CRAWDebayer debayer
                       enum CRAWColorSpace out_space, enum CRAWFormat out_format,
                       CRAWStageConfig *cfg);
 
-/* Plans one frame of W x H (validation, constants, state selection); 0 or
- * a negative errno, nothing changed on failure. */
 /* The highlight field's pyramid for a W x H frame: level sizes and
  * offsets (in texels) into one array; the level count, or
  * CRAW_ERROR(EINVAL) past KHL_LEVELS. */
 int craw_hl_levels(uint32_t w, uint32_t h, int lw[KHL_LEVELS], int lh[KHL_LEVELS],
                    size_t off[KHL_LEVELS], size_t *total);
+
+/* Plans one frame of W x H (validation, constants, state selection); 0 or
+ * a negative errno, nothing changed on failure. */
 
 int craw_frame_plan(CRAWContext *ctx, const CRAWStageConfig &cfg, const CRAWFrameBuffers &buf,
                     uint32_t W, uint32_t H, const CRAWImage *src, const CRAWImage *dst,
