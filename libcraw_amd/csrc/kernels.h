@@ -121,7 +121,7 @@ struct KFrame {
 #define KG_XH_TH 30             /* the same kernel with a temporal history */
 #define KG_RS_OUT 116           /* streamed RCD: output columns per strip */
 #define KG_RS_OUT_F 112         /* the same without the chroma stage */
-#define KG_RS_R   64            /* and rows */
+#define KG_RS_R   40            /* and rows */
 #define KG_RS_NW  4             /* strips per workgroup */
 
 enum KStage {
