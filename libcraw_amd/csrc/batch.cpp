@@ -50,6 +50,7 @@ struct CRAWBatch {
     float *inter_rgb, *inter_rgb2, *down, *blur;
     uint16_t *hl_mosaic;
     float4 *hl_levels;
+    size_t hl_levels_cap;   /* texels */
     uint32_t *hist;
     uint2 *awb_samples;
     uint32_t *hist_host;    /* pinned, max_frames histograms */
@@ -202,7 +203,7 @@ static int batch_sizes(CRAWContext *ctx, const CRAWBatchParams *params, CRAWStag
          * column of rounding) */
         sz->hl_mosaic = px * sizeof(uint16_t);
         sz->hl_levels = (px / 12 + (size_t)params->max_frames *
-                         ((params->max_width + params->max_height) / 2 + 16)) * sizeof(float4);
+                         ((params->max_width + params->max_height) / 2 + 64)) * sizeof(float4);
     }
     if (cfg->awb) {
         sz->hist = (size_t)params->max_frames * CRAW_HIST_WORDS * 4;
@@ -280,6 +281,7 @@ CRAW_API int craw_batch_create(CRAWContext *ctx, const CRAWBatchParams *params, 
         goto fail;
     }
     b->tiles_cap = sz.tiles_cap;
+    b->hl_levels_cap = sz.hl_levels / sizeof(float4);
     b->device_bytes = sz.device();
     staging = sz.frames + b->tiles_cap * sizeof(uint32_t);
     for (int i = 0; i < RING; i++) {
@@ -353,6 +355,10 @@ static int batch_plan(CRAWBatch *b, const CRAWBatchFrame *frames, uint32_t nb, B
 
             if (craw_hl_levels(w, h, lw, lh, off, &texels) < 0)
                 return CRAW_ERROR(EINVAL);
+            /* the arena's bound holds for any frames within the batch's
+             * limits; checked all the same before anything is written */
+            if (off_lv + texels > b->hl_levels_cap)
+                return CRAW_ERROR(ENOSPC);
             off_lv += texels;
         }
         off_awb += craw_awb_blocks(w, h);
