@@ -10,7 +10,9 @@ import ctypes
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libcraw.so")
+# CRAW_LIB names another build of the library (the sanitizer build of the
+# csrc Makefile's `sanitize` target, for a host-only test run)
+LIB_PATH = os.environ.get("CRAW_LIB") or os.path.join(HERE, "libcraw.so")
 
 c_float = ctypes.c_float
 c_u32 = ctypes.c_uint32

@@ -64,8 +64,8 @@ struct CRAWBatch {
     Staging ring[RING];
     uint32_t ring_pos;
 
-    /* persistent workgroups per CU for each list kind (denoise, debayer,
-     * blur, up); 0: one workgroup per tile */
+    /* persistent workgroups per CU for each list kind (0 denoise, 1
+     * debayer, 2 chroma blur, 3 AWB histogram); 0: one workgroup per tile */
     int wg_per_cu[4];
 
     /* graph of the last captured shape */
@@ -767,11 +767,14 @@ CRAW_API int craw_batch_autotune(CRAWBatch *b, CRAWStream stream, const CRAWBatc
                 continue;   /* the first run of each candidate is a warm-up */
             n = craw_profiler_times(b->ctx, &b->prof, 0, st, CRAW_MAX_STAGES);
             for (int j = 0; j < n && j < CRAW_MAX_STAGES; j++) {
+                /* the stages whose persistent grid wg_per_cu[kind] sets;
+                 * the one-launch-a-frame stages (chroma up + colour,
+                 * highlights, false colour) and the clear are not tuned */
                 int kind = !strncmp(st[j].name, "denoise", 7) ? 0 :
+                           (!strcmp(st[j].name, "rcd") ||
+                            !strncmp(st[j].name, "debayer_", 8)) ? 1 :
                            !strcmp(st[j].name, "chroma_blur") ? 2 :
-                           !strcmp(st[j].name, "chroma_up_color") ? 3 :
-                           !strcmp(st[j].name, "awb_clear") ? -1 :
-                           !strcmp(st[j].name, "awb_hist") ? -1 : 1;
+                           !strcmp(st[j].name, "awb_hist") ? 3 : -1;
                 if (kind >= 0)
                     acc[kind] += st[j].ms;
             }

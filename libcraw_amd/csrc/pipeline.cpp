@@ -189,13 +189,10 @@ CRAW_API int craw_pipeline_create(CRAWContext *ctx, const CRAWPipelineParams *pa
                 goto fail;
             }
         }
-        /* the device histogram starts at zero; each recording's export
-         * leaves it zero again */
-        if (hipMemset(p->buf.hist, 0, CRAW_HIST_WORDS * sizeof(uint32_t)) != hipSuccess ||
-            hipStreamSynchronize(nullptr) != hipSuccess) {
-            ret = CRAW_ERROR(EIO);
-            goto fail;
-        }
+        /* the device histogram is cleared by the first recording, on the
+         * caller's stream (each recording's export leaves it zero again),
+         * so that creating a pipeline synchronises no stream */
+        p->hist_dirty = true;
     }
     if ((params->flags & CRAW_PIPELINE_FLAG_PROFILE) && (ret = craw_profiler_init(&p->prof)) < 0)
         goto fail;

@@ -204,7 +204,10 @@ int craw_frame_plan(CRAWContext *ctx, const CRAWStageConfig &cfg, const CRAWFram
     plan->t_write = t_write;
     plan->temporal = temporal;
     plan->replay = replay;
-    plan->ts = ts;
+    /* lib/pipeline.c:2308: only a recording of a pipeline (or batch) built
+     * with the temporal pass moves a state; one without it leaves an
+     * attached state as it was */
+    plan->ts = cfg.has_temporal ? ts : NULL;
 
     if (spatial || temporal) {
         const uint32_t dn_min = 2 * (DN_SEARCH_R + DN_PATCH_R + 1 + (cfg.has_temporal ? 1 : 0));

@@ -18,7 +18,9 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB = os.path.join(HERE, "liboracle.so")
+# CRAW_ORACLE_LIB names another build of the oracle (the Makefile's
+# `sanitize` target)
+LIB = os.environ.get("CRAW_ORACLE_LIB") or os.path.join(HERE, "liboracle.so")
 REF_LIB = os.path.join(HERE, "_ref", "libcraw_ref.so")
 
 _lib = None
@@ -39,12 +41,35 @@ def lib():
     return _lib
 
 
+_ref = None
+
+
+def ref_available():
+    """Whether oracle/_ref was built (the reference tree was present at
+    build time). Loads nothing."""
+    return os.path.exists(REF_LIB)
+
+
 def ref_lib():
     """The reference's own lib/awb.c, colorspace.c, profile.c, gamut.c and
-    noise.c, or None where they were not built (no reference tree)."""
-    if not os.path.exists(REF_LIB):
-        return None
-    return ctypes.CDLL(REF_LIB)
+    noise.c, or None where they were not built (no reference tree). Loaded
+    on the first call only, so that merely importing a test module maps
+    nothing of it into the process."""
+    global _ref
+    if _ref is None and ref_available():
+        _ref = ctypes.CDLL(REF_LIB)
+    return _ref
+
+
+class LazyRef:
+    """Attribute access on the reference's host library, loading it on
+    first use (tests that need it are skipped where it is absent)."""
+
+    def __getattr__(self, name):
+        lib_ = ref_lib()
+        if lib_ is None:
+            raise RuntimeError("oracle/_ref is not built (no reference tree)")
+        return getattr(lib_, name)
 
 
 def _ptr(a):
@@ -52,25 +77,86 @@ def _ptr(a):
 
 
 class Mode:
-    def __init__(self, prefix, dtype):
+    def __init__(self, prefix, dtype, path=None):
+        """One mode's set of functions; `path` names another build of the
+        oracle (a planted-mistake variant of a test) instead of liboracle.so."""
         self.prefix = prefix
         self.dtype = np.dtype(dtype)
+        self.path = path
+        self._handle = None
+
+    def lib(self):
+        if self.path is None:
+            return lib()
+        if self._handle is None:
+            self._handle = ctypes.CDLL(self.path)
+        return self._handle
 
     def _f(self, name):
-        f = getattr(lib(), self.prefix + name)
+        f = getattr(self.lib(), self.prefix + name)
         f.restype = ctypes.c_int
         return f
 
+    def _real(self):
+        return ctypes.c_float if self.dtype == np.float32 else ctypes.c_double
+
     def set_threads(self, n):
-        f = getattr(lib(), self.prefix + "set_threads")
+        f = getattr(self.lib(), self.prefix + "set_threads")
         f.restype = None
         f(ctypes.c_int(n))
+
+    # -- test hooks: the denoise and chroma weights one value at a time
+    def gat(self, a, b, u, t_in=None):
+        """T(u) for the array u, I(t) for the array t_in (default T(u)), and
+        the constants (a, c, sc, fwd, inv, off, t0)."""
+        u = np.array(u, self.dtype).ravel()
+        t = np.empty_like(u)
+        consts = np.empty(7, self.dtype)
+        f = getattr(self.lib(), self.prefix + "gat_hook")
+        f.restype = None
+        first = np.empty_like(u)
+        f(ctypes.c_float(a), ctypes.c_float(b), _ptr(u), _ptr(t), _ptr(u), _ptr(first),
+          ctypes.c_size_t(u.size), _ptr(consts))
+        # the inverse, of T(u) or of the given levels (one array length each)
+        t_in = np.array(t if t_in is None else t_in, self.dtype).ravel()
+        scratch = np.zeros_like(t_in)
+        back = np.empty_like(t_in)
+        f(ctypes.c_float(a), ctypes.c_float(b), _ptr(scratch), _ptr(np.empty_like(t_in)),
+          _ptr(t_in), _ptr(back), ctypes.c_size_t(t_in.size), _ptr(consts))
+        return t, back, consts
+
+    def nlm_weight(self, d2, inv_h2, vc=-1.0, vq=-1.0):
+        f = getattr(self.lib(), self.prefix + "nlm_weight_hook")
+        r = self._real()
+        f.restype, f.argtypes = r, [r, r, r, r]
+        return f(d2, inv_h2, vc, vq)
+
+    def alpha_weight(self, d2, alpha_max, inv_h2):
+        f = getattr(self.lib(), self.prefix + "alpha_weight_hook")
+        r = self._real()
+        f.restype, f.argtypes = r, [r, r, r]
+        return f(d2, alpha_max, inv_h2)
+
+    def blend(self, tc, tp, a, v_s):
+        """(the blended level, its variance, the variance as stored)"""
+        f = getattr(self.lib(), self.prefix + "blend_hook")
+        r = self._real()
+        out = (r * 3)()
+        f.restype, f.argtypes = None, [r, r, r, r, ctypes.c_void_p]
+        f(tc, tp, a, v_s, out)
+        return tuple(out)
+
+    def cd_weight(self, dx, dy, g, dg, dd0, dd1):
+        f = getattr(self.lib(), self.prefix + "cd_weight_hook")
+        r = self._real()
+        f.restype, f.argtypes = r, [ctypes.c_int, ctypes.c_int, r, r, r, r]
+        return f(dx, dy, g, dg, dd0, dd1)
 
     def prep(self, bayer, profile, wb):
         h, w = bayer.shape
         v = np.empty((h, w), self.dtype)
         wb = np.asarray(wb, np.float32)
-        f = getattr(lib(), self.prefix + "prep")
+        f = getattr(self.lib(), self.prefix + "prep")
         f.restype = None
         f(_ptr(bayer), ctypes.c_uint32(w), ctypes.c_uint32(h), ctypes.c_size_t(w),
           ctypes.byref(profile), _ptr(wb), _ptr(v))
@@ -144,7 +230,7 @@ class Mode:
 
     def gamut_compress(self, rgb3):
         rgb3 = np.ascontiguousarray(rgb3, self.dtype).copy()
-        f = getattr(lib(), self.prefix + "gamut_compress")
+        f = getattr(self.lib(), self.prefix + "gamut_compress")
         f.restype = None
         f(_ptr(rgb3), ctypes.c_size_t(rgb3.size // 3))
         return rgb3

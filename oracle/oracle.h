@@ -28,12 +28,27 @@
  *       tests/test_ref_debayer.c (constants, ramps, directional step,
  *       hostile inputs, 4x4 frame, prep arithmetic) and brute-force
  *       restatements on small frames in tests/;
- *   GAT: closed form and igat(gat(u)) = u;
- *   NLM: constant planes are fixed points, h -> 0 is the identity, flat
- *       and checkerboard noise ratios, an independent direct summation;
- *   temporal: static-scene variance tends to (1 - a) / (1 + a), state
- *       rules, first frame equals the spatial filter;
- *   chroma: G untouched, flat colour preserved, blotch reduction;
+ *   GAT and inverse: the closed forms of the reference's DESIGN.md
+ *       (c = 3a^2/8 + b, T(u) = (2/a) sqrt(a u + c), its tangent below 0,
+ *       the algebraic inverse) at several (a, b, u), u < 0 included;
+ *   NLM: the weight exp(-max(d2/9/h^2 - 2, 0)) against its closed form;
+ *       noise-free two-level planes whose output is the count-weighted
+ *       mean of the two levels; a numpy direct summation over the 7x7
+ *       window on noisy frames; constant planes are fixed points, h -> 0
+ *       is the identity, flat and checkerboard noise ratios;
+ *   temporal: a uniform offset D between the previous level and this
+ *       frame's gives alpha_max exp(-max(D^2/ht^2 - 2, 0)), the blend and
+ *       the stored variance; with a history, a two-level plane of two
+ *       known variances gives the weight exp(-max(d2/9/h^2 - (vp + vq),
+ *       0) / vp); static-scene variance tends to (1 - a) / (1 + a), first
+ *       frame equals the spatial filter;
+ *   chroma: a numpy statement of the reference's definition (block means,
+ *       9x9 bilateral with sigma_s 2, K = Kc = 0.35, eps 0.02, mirrored,
+ *       bilinear up) on a G edge crossing a colour edge; G untouched, flat
+ *       colour preserved, blotch reduction;
+ *   each of those closed-form checks is shown, by tests/test_oracle_pins.py,
+ *       to fail on a variant of this file with one constant planted wrong
+ *       (ORACLE_MUTANT);
  *   false colour: each sweep against scipy's median filter and a double
  *       box mean (mirror mode), G and alpha untouched, and the reference's
  *       spike known answers (tests/test_pipeline.c:3994-4050: a site spike's
@@ -104,7 +119,14 @@ void oracle_gamut_params_d(double thr[3], double scl[3], double *pwr, double *in
     int P##process(const uint16_t *bayer, uint32_t w, uint32_t h, size_t stride,     \
                    const CRAWPipelineParams *pp, const CRAWFrameParams *fp,          \
                    const real *prev, real *next, uint16_t *clean, real *inter,       \
-                   uint32_t *count, uint32_t *weight, uint32_t *bright, real *out);
+                   uint32_t *count, uint32_t *weight, uint32_t *bright, real *out);  \
+    /* test hooks: the denoise and chroma weights one value at a time */            \
+    void P##gat_hook(float a, float b, const real *u, real *t, const real *tin,      \
+                     real *uout, size_t n, real consts[7]);                          \
+    real P##nlm_weight_hook(real d2, real inv_h2, real vc, real vq);                 \
+    real P##alpha_weight_hook(real d2, real alpha_max, real inv_h2);                 \
+    void P##blend_hook(real tc, real tp, real a, real v_s, real out[3]);             \
+    real P##cd_weight_hook(int dx, int dy, real g, real dg, real dd0, real dd1);
 
 ORACLE_DECLARE(oracle_f32l_, float)
 ORACLE_DECLARE(oracle_f32d_, float)

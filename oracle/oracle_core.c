@@ -442,6 +442,40 @@ int PFX(debayer_wb)(const uint16_t *bayer, uint32_t width, uint32_t height, size
 #define O_DN_GUARD_LO K(0.90f, 0.90)
 #define O_DN_GUARD_HI K(1.00f, 1.00)
 #define O_DN_VAR_MIN  K(1.0e-4f, 1.0e-4)
+/* 2 sigma^2 with sigma = 1 in the stabilised domain: the part of a patch
+ * distance that two noisy samples of one true value carry
+ * (reference DESIGN.md:1679-1700, 2149-2160). */
+#define O_DN_ALLOW    K(2.0f, 2.0)
+/* 1 / 9: the 3x3 patch mean of squared differences, and the same 3x3 for
+ * the temporal weight (ref_denoise.c:82-97, 114-154). */
+#define O_DN_INV9     K(1.0f / 9.0f, 1.0 / 9.0)
+
+/* ORACLE_MUTANT plants one mistake, for the test that shows each closed-form
+ * pin catches it (tests/test_oracle_pins.py builds these variants in a
+ * temporary directory; the oracle proper is built without it). */
+#ifndef ORACLE_MUTANT
+#define ORACLE_MUTANT 0
+#endif
+#if ORACLE_MUTANT == 1          /* NLM allowance 2 -> 1 */
+#define O_MUT_NLM_ALLOW K(1.0f, 1.0)
+#else
+#define O_MUT_NLM_ALLOW O_DN_ALLOW
+#endif
+#if ORACLE_MUTANT == 2          /* patch mean 1/9 -> 1/8 */
+#define O_MUT_NLM_INV_NP K(1.0f / 8.0f, 1.0 / 8.0)
+#else
+#define O_MUT_NLM_INV_NP O_DN_INV9
+#endif
+#if ORACLE_MUTANT == 4          /* temporal 3x3 mean 1/9 -> 1/8 */
+#define O_MUT_ALPHA_INV9 K(1.0f / 8.0f, 1.0 / 8.0)
+#else
+#define O_MUT_ALPHA_INV9 O_DN_INV9
+#endif
+#if ORACLE_MUTANT == 5          /* temporal allowance 2 -> 1 */
+#define O_MUT_ALPHA_ALLOW K(1.0f, 1.0)
+#else
+#define O_MUT_ALPHA_ALLOW O_DN_ALLOW
+#endif
 
 /* lib/ref/ref_denoise.c:26-58: the generalised Anscombe transform
  * with its tangent below black, and the algebraic inverse. */
@@ -454,9 +488,18 @@ static void o_gat_init(struct o_gat *g, float a, float b)
     real ra = (real)a, rb = (real)b;
 
     g->a = ra;
+#if ORACLE_MUTANT == 3          /* c without b */
+    g->c = K(3.0f, 3.0) * ra * ra / K(8.0f, 8.0);
+    (void)rb;
+#else
     g->c = K(3.0f, 3.0) * ra * ra / K(8.0f, 8.0) + OMAX(rb, K(0.0f, 0.0));
+#endif
     g->sc = SQRTF(g->c);
+#if ORACLE_MUTANT == 7          /* forward scale 2/a -> 1/a */
+    g->fwd = K(1.0f, 1.0) / ra;
+#else
     g->fwd = K(2.0f, 2.0) / ra;
+#endif
     g->inv = ra / K(4.0f, 4.0);
     g->off = g->c / ra;
     g->t0 = g->fwd * g->sc;
@@ -466,7 +509,11 @@ static inline real o_gat(const struct o_gat *g, real u)
 {
     if (u >= K(0.0f, 0.0))
         return g->fwd * SQRTF(g->a * u + g->c);
+#if ORACLE_MUTANT == 8          /* tangent slope 1/sc -> sc */
+    return g->t0 + u * g->sc;
+#else
     return g->t0 + u / g->sc;
+#endif
 }
 
 static inline real o_igat(const struct o_gat *g, real t)
@@ -480,6 +527,14 @@ static inline real o_igat(const struct o_gat *g, real t)
 static inline int o_dn_mirror(int v, int n)
 {
     return o_mirror(v, n);
+}
+
+/* ref_denoise.c:82-97: the temporal weight of a 3x3 sum of squared
+ * differences d2: alpha_max exp(-max(d2 / 9 / ht^2 - 2, 0)). */
+static inline real o_alpha_weight(real d2, real alpha_max, real inv_h2)
+{
+    return alpha_max * EXPF(-OMAX(d2 * O_MUT_ALPHA_INV9 * inv_h2 - O_MUT_ALPHA_ALLOW,
+                                  K(0.0f, 0.0)));
 }
 
 /* ref_denoise.c:82-97: the temporal weight, 3x3 of (prev - cur)^2 summed
@@ -496,8 +551,50 @@ static real o_alpha(const real *cur, const real *prev, int w2, int h2, int i, in
             d2 += e * e;
         }
     }
-    return alpha_max * EXPF(-OMAX(d2 * K(1.0f / 9.0f, 1.0 / 9.0) * inv_h2 - K(2.0f, 2.0),
-                                  K(0.0f, 0.0)));
+    return o_alpha_weight(d2, alpha_max, inv_h2);
+}
+
+/* ref_denoise.c:268-296: the blended level and the variance it carries,
+ * v = max((1 - a)^2 + a^2 v_s, 1e-4). */
+static inline real o_blend(real tc, real tp, real a)
+{
+#if ORACLE_MUTANT == 9          /* blend towards the wrong frame */
+    return tp * (K(1.0f, 1.0) - a) + tc * a;
+#else
+    return tc * (K(1.0f, 1.0) - a) + tp * a;
+#endif
+}
+
+static inline real o_blend_var(real a, real v_s)
+{
+#if ORACLE_MUTANT == 10         /* (1 - a) for (1 - a)^2 */
+    return OMAX((K(1.0f, 1.0) - a) + a * a * v_s, O_DN_VAR_MIN);
+#else
+    return OMAX((K(1.0f, 1.0) - a) * (K(1.0f, 1.0) - a) + a * a * v_s, O_DN_VAR_MIN);
+#endif
+}
+
+/* ref_denoise.c:114-154: the NLM weight of a patch's sum of squared
+ * differences d2. Without history (vc < 0): exp(-max(d2 / 9 / h^2 - 2, 0));
+ * with it, the noise the two samples carry, vc + vq, in place of 2, and the
+ * strength in units of the centre's noise: exp(-max(d2 / 9 / h^2 - (vc +
+ * vq), 0) / vc). */
+static inline real o_nlm_weight(real d2, real inv_h2, real vc, real vq)
+{
+    if (vc >= K(0.0f, 0.0)) {
+#if ORACLE_MUTANT == 11         /* the centre's variance twice for vc + vq */
+        real v2 = vc + vc;
+        (void)vq;
+#else
+        real v2 = vc + vq;
+#endif
+#if ORACLE_MUTANT == 12         /* no division by the centre's variance */
+        return EXPF(-OMAX(d2 * O_MUT_NLM_INV_NP * inv_h2 - v2, K(0.0f, 0.0)));
+#else
+        return EXPF(-OMAX(d2 * O_MUT_NLM_INV_NP * inv_h2 - v2, K(0.0f, 0.0)) / vc);
+#endif
+    }
+    return EXPF(-OMAX(d2 * O_MUT_NLM_INV_NP * inv_h2 - O_MUT_NLM_ALLOW, K(0.0f, 0.0)));
 }
 
 /* ref_denoise.c:100-105: the variance as the state keeps it, its square
@@ -515,7 +612,6 @@ static real o_nlm(const real *plane, const real *var, int w2, const int *mx, con
                   real inv_h2)
 {
     const int sr = O_DN_SEARCH_R, pr = O_DN_PATCH_R;
-    const real inv_np = K(1.0f, 1.0) / (real)((2 * pr + 1) * (2 * pr + 1));
     real cp[(2 * O_DN_PATCH_R + 1) * (2 * O_DN_PATCH_R + 1)];
     real acc = K(0.0f, 0.0), wsum = K(0.0f, 0.0);
     int n = 0;
@@ -535,13 +631,11 @@ static real o_nlm(const real *plane, const real *var, int w2, const int *mx, con
                     d2 += e * e;
                 }
             }
-            if (var) {
-                real vc = var[(size_t)my[0] * w2 + mx[0]];
-                real v2 = vc + var[(size_t)my[dy] * w2 + mx[dx]];
-                wgt = EXPF(-OMAX(d2 * inv_np * inv_h2 - v2, K(0.0f, 0.0)) / vc);
-            } else {
-                wgt = EXPF(-OMAX(d2 * inv_np * inv_h2 - K(2.0f, 2.0), K(0.0f, 0.0)));
-            }
+            if (var)
+                wgt = o_nlm_weight(d2, inv_h2, var[(size_t)my[0] * w2 + mx[0]],
+                                   var[(size_t)my[dy] * w2 + mx[dx]]);
+            else
+                wgt = o_nlm_weight(d2, inv_h2, K(-1.0f, -1.0), K(-1.0f, -1.0));
             sv = plane[(size_t)my[dy] * w2 + mx[dx]];
             wsum += wgt;
             acc += wgt * sv;
@@ -646,9 +740,8 @@ int PFX(denoise_temporal)(const uint16_t *bayer, uint32_t width, uint32_t height
                     size_t mp = (size_t)(2 * k + py) * width + 2 * i + px;
                     real a = o_alpha(cur, tprev, w2, h2, i, k, ralpha, inv_h2);
 
-                    plane[c] = cur[c] * (K(1.0f, 1.0) - a) + tprev[c] * a;
-                    var[c] = OMAX((K(1.0f, 1.0) - a) * (K(1.0f, 1.0) - a) + a * a * prev_var[mp],
-                                  O_DN_VAR_MIN);
+                    plane[c] = o_blend(cur[c], tprev[c], a);
+                    var[c] = o_blend_var(a, prev_var[mp]);
                 }
             }
         }
@@ -698,13 +791,86 @@ done:
     return ret;
 }
 
+/* Test hooks: the denoise arithmetic above, one value at a time, for the
+ * closed-form checks of the tests (the same helpers the filters call). */
+void PFX(gat_hook)(float a, float b, const real *u, real *t, const real *tin, real *uout,
+                   size_t n, real consts[7])
+{
+    struct o_gat g;
+
+    o_gat_init(&g, a, b);
+    for (size_t i = 0; i < n; i++) {
+        t[i] = o_gat(&g, u[i]);
+        uout[i] = o_igat(&g, tin[i]);
+    }
+    consts[0] = g.a; consts[1] = g.c; consts[2] = g.sc; consts[3] = g.fwd;
+    consts[4] = g.inv; consts[5] = g.off; consts[6] = g.t0;
+}
+
+real PFX(nlm_weight_hook)(real d2, real inv_h2, real vc, real vq)
+{
+    return o_nlm_weight(d2, inv_h2, vc, vq);
+}
+
+real PFX(alpha_weight_hook)(real d2, real alpha_max, real inv_h2)
+{
+    return o_alpha_weight(d2, alpha_max, inv_h2);
+}
+
+void PFX(blend_hook)(real tc, real tp, real a, real v_s, real out[3])
+{
+    out[0] = o_blend(tc, tp, a);
+    out[1] = o_blend_var(a, v_s);
+    out[2] = o_var_store(out[1]);
+}
+
 /* ------------------------------------------------------------- chroma */
 
+/* ref_chroma.c:14-22: radius 4, sigma_s 2 quarter texels, K = Kc = 0.35,
+ * eps 0.02 (reference DESIGN.md:1947-1967). */
 #define O_CD_RADIUS    4
+#if ORACLE_MUTANT == 13         /* 1 / (2 sigma_s^2) with sigma_s = 1 */
+#define O_CD_INV_2S2   K(0.5f, 0.5)
+#else
 #define O_CD_INV_2S2   K(1.0f / (2.0f * 2.0f * 2.0f), 1.0 / 8.0)
+#endif
+#if ORACLE_MUTANT == 14         /* K 0.35 -> 0.25 */
+#define O_CD_RANGE_K   K(0.25f, 0.25)
+#else
 #define O_CD_RANGE_K   K(0.35f, 0.35)
+#endif
+#if ORACLE_MUTANT == 15         /* Kc 0.35 -> 0.25 */
+#define O_CD_CHROMA_K  K(0.25f, 0.25)
+#else
 #define O_CD_CHROMA_K  K(0.35f, 0.35)
+#endif
+#if ORACLE_MUTANT == 16         /* eps 0.02 -> 0.05 */
+#define O_CD_RANGE_EPS K(0.05f, 0.05)
+#else
 #define O_CD_RANGE_EPS K(0.02f, 0.02)
+#endif
+
+/* ref_chroma.c:100-125: the bilateral's scales at a centre of level G:
+ * 1 / (K L) and 1 / (Kc L), L = max(G, 0) + eps. */
+static inline void o_cd_scales(real g, real *inv_range, real *inv_chroma)
+{
+    real level = OMAX(g, K(0.0f, 0.0)) + O_CD_RANGE_EPS;
+
+    *inv_range = K(1.0f, 1.0) / (O_CD_RANGE_K * level);
+    *inv_chroma = K(1.0f, 1.0) / (O_CD_CHROMA_K * level);
+}
+
+/* ref_chroma.c:100-125: the joint bilateral weight of a quarter-grid
+ * offset (dx, dy) whose G, R - G and B - G differ from the centre's by
+ * dg, dd0, dd1: exp(-(dx^2 + dy^2) / (2 sigma_s^2) - (dg / (K L))^2 -
+ * (dd0 / (Kc L))^2 - (dd1 / (Kc L))^2). */
+static inline real o_cd_weight(int dx, int dy, real dg, real dd0, real dd1, real inv_range,
+                               real inv_chroma)
+{
+    real g = dg * inv_range, d0 = dd0 * inv_chroma, d1 = dd1 * inv_chroma;
+
+    return EXPF(-(real)(dx * dx + dy * dy) * O_CD_INV_2S2 - g * g - d0 * d0 - d1 * d1);
+}
 
 /* ref_chroma.c:24-31: one reflection on the quarter grid. */
 static inline int o_cd_mirror(int v, int n)
@@ -770,10 +936,10 @@ int PFX(chroma_denoise)(real *rgb, uint32_t width, uint32_t height, float streng
         for (int qx = 0; qx < wq; qx++) {
             const real *c = &down[3 * ((size_t)qy * wq + qx)];
             real *o = &blur[3 * ((size_t)qy * wq + qx)];
-            real level = OMAX(c[2], K(0.0f, 0.0)) + O_CD_RANGE_EPS;
-            real inv_range = K(1.0f, 1.0) / (O_CD_RANGE_K * level);
-            real inv_chroma = K(1.0f, 1.0) / (O_CD_CHROMA_K * level);
+            real inv_range, inv_chroma;
             real acc[2] = { K(0.0f, 0.0), K(0.0f, 0.0) }, wsum = K(0.0f, 0.0);
+
+            o_cd_scales(c[2], &inv_range, &inv_chroma);
 
             for (int dy = -O_CD_RADIUS; dy <= O_CD_RADIUS; dy++) {
                 int sy = o_cd_mirror(qy + dy, hq);
@@ -781,10 +947,8 @@ int PFX(chroma_denoise)(real *rgb, uint32_t width, uint32_t height, float streng
                 for (int dx = -O_CD_RADIUS; dx <= O_CD_RADIUS; dx++) {
                     int sx = o_cd_mirror(qx + dx, wq);
                     const real *s = &down[3 * ((size_t)sy * wq + sx)];
-                    real dg = (s[2] - c[2]) * inv_range;
-                    real dd0 = (s[0] - c[0]) * inv_chroma, dd1 = (s[1] - c[1]) * inv_chroma;
-                    real wgt = EXPF(-(real)(dx * dx + dy * dy) * O_CD_INV_2S2 - dg * dg
-                                    - dd0 * dd0 - dd1 * dd1);
+                    real wgt = o_cd_weight(dx, dy, s[2] - c[2], s[0] - c[0], s[1] - c[1],
+                                           inv_range, inv_chroma);
 
                     acc[0] += wgt * s[0];
                     acc[1] += wgt * s[1];
@@ -833,6 +997,14 @@ fail:
     free(blur);
     free(down);
     return ret;
+}
+
+real PFX(cd_weight_hook)(int dx, int dy, real g, real dg, real dd0, real dd1)
+{
+    real inv_range, inv_chroma;
+
+    o_cd_scales(g, &inv_range, &inv_chroma);
+    return o_cd_weight(dx, dy, dg, dd0, dd1, inv_range, inv_chroma);
 }
 
 /* ------------------------------------------ highlight reconstruction */

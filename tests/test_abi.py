@@ -31,15 +31,14 @@ def test_exports_nothing_undeclared():
     assert craw == capi.declared_symbols()
 
 
-def test_struct_sizes_match_the_reference():
+def test_struct_sizes_match_the_reference(tmp_path):
     # lib/pipeline.c:252-253 of the reference asserts 256 and 264.
     assert ctypes.sizeof(capi.CRAWPipelineParams) == 256
     assert ctypes.sizeof(capi.CRAWFrameParams) == 264
     assert ctypes.sizeof(capi.CRAWProfile) == 1264
     assert ctypes.sizeof(capi.CRAWLensCorrection) == 64
-    src = os.path.join(ROOT, "_work") if os.path.isdir(os.path.join(ROOT, "_work")) else "/tmp"
-    c = os.path.join(src, "abi_sizes.c")
-    exe = os.path.join(src, "abi_sizes")
+    c = str(tmp_path / "abi_sizes.c")
+    exe = str(tmp_path / "abi_sizes")
     with open(c, "w") as f:
         f.write('#include <craw/serve.h>\n#include <stdio.h>\n#include <stddef.h>\n'
                 'int main(void){printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\\n", '
@@ -57,15 +56,15 @@ def test_struct_sizes_match_the_reference():
                      ctypes.sizeof(capi.CRAWServeRequest), ctypes.sizeof(capi.CRAWServeCompletion)]
 
 
-def test_headers_compile_as_c_and_cpp():
+def test_headers_compile_as_c_and_cpp(tmp_path):
     for compiler, ext in (("gcc", "c"), ("g++", "cpp")):
-        path = "/tmp/craw_hdr_test.%s" % ext
+        path = str(tmp_path / ("craw_hdr_test.%s" % ext))
         with open(path, "w") as f:
             f.write("#include <craw/pipeline.h>\n#include <craw/batch.h>\n"
                     "#include <craw/serve.h>\n"
                     "int main(void){return 0;}\n")
         subprocess.run([compiler, "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"),
-                        "-c", path, "-o", "/tmp/craw_hdr_test.o"], check=True)
+                        "-c", path, "-o", str(tmp_path / "craw_hdr_test.o")], check=True)
 
 
 def test_defaults_and_host_entry_points():

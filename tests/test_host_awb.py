@@ -21,8 +21,9 @@ import oracle
 from libcraw_amd import capi
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-REF = oracle.ref_lib()
-needs_ref = pytest.mark.skipif(REF is None, reason="oracle/_ref not built (no reference tree)")
+REF = oracle.LazyRef()
+HAVE_REF = oracle.ref_available()
+needs_ref = pytest.mark.skipif(not HAVE_REF, reason="oracle/_ref not built (no reference tree)")
 P = ctypes.POINTER
 EAGAIN, EINVAL, ENOTSUP = 11, 22, 95
 
@@ -72,7 +73,7 @@ def bind_ref():
 
 def impls():
     out = [("libcraw", capi.lib())]
-    if REF is not None:
+    if HAVE_REF:
         out.append(("reference", bind_ref()))
     return out
 
@@ -164,6 +165,7 @@ def hist_struct(arrs, bright=True):
     hs.count = arrs[0].ctypes.data_as(P(ctypes.c_uint32))
     hs.weight = arrs[1].ctypes.data_as(P(ctypes.c_uint32))
     hs.weight_bright = arrs[2].ctypes.data_as(P(ctypes.c_uint32)) if bright else None
+    hs._keep = arrs     # the struct holds raw pointers: keep the arrays alive with it
     return hs
 
 

@@ -19,8 +19,9 @@ from libcraw_amd import capi
 GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden",
                                    "reference_vectors.json")))
 F9 = ctypes.c_float * 9
-REF = oracle.ref_lib()
-needs_ref = pytest.mark.skipif(REF is None, reason="oracle/_ref not built (no reference tree)")
+REF = oracle.LazyRef()
+HAVE_REF = oracle.ref_available()
+needs_ref = pytest.mark.skipif(not HAVE_REF, reason="oracle/_ref not built (no reference tree)")
 
 
 def _mat(fn, cs):
@@ -31,7 +32,7 @@ def _mat(fn, cs):
 
 def impls():
     out = [("libcraw", capi.lib(), "craw_"), ("oracle", oracle.lib(), "oracle_")]
-    if REF is not None:
+    if HAVE_REF:
         out.append(("reference", REF, "craw_"))
     return out
 
@@ -205,7 +206,7 @@ def test_gamut_constants():
     f(thr, scl, ctypes.byref(pw), ctypes.byref(ipw))
     assert np.abs(np.array(scl[:]) - np.array(GOLD["rgc_scales"]["scl"])).max() < 1e-6
     assert np.allclose(thr[:], [0.815, 0.803, 0.880]) and abs(pw.value - 1.2) < 1e-7
-    if REF is not None:
+    if HAVE_REF:
         # CRAWGamutCompress: thr[3], scl[3], pwr, inv_pwr
         buf = (ctypes.c_float * 8)()
         REF.craw_gamut_compress_params.restype = None

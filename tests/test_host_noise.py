@@ -13,8 +13,9 @@ import pytest
 import oracle
 from libcraw_amd import capi, scene
 
-REF = oracle.ref_lib()
-needs_ref = pytest.mark.skipif(REF is None, reason="oracle/_ref not built (no reference tree)")
+REF = oracle.LazyRef()
+HAVE_REF = oracle.ref_available()
+needs_ref = pytest.mark.skipif(not HAVE_REF, reason="oracle/_ref not built (no reference tree)")
 P = ctypes.POINTER
 EINVAL, ENOTSUP, ENODATA = 22, 95, 61
 

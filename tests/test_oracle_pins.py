@@ -1,0 +1,585 @@
+"""Closed-form pins of the oracle's denoise, temporal and chroma arithmetic.
+
+Every expected value here comes from the reference's own definitions
+(DESIGN.md of cyanreg/libcraw, sections 14.3, 14.4 and 14.7, lines
+1641-1714, 1947-1967 and 2149-2160), evaluated in float64 by numpy, never by
+the oracle:
+
+  GAT       c = 3a^2/8 + b, T(u) = (2/a) sqrt(a u + c) for u >= 0, the tangent
+            t0 + u/sc below (sc = sqrt(c), t0 = 2 sc / a); the inverse
+            I(t) = a t^2/4 - c/a for t >= t0, (t - t0) sc below;
+  NLM       d2 = (1/9) sum over the 3x3 patch of (t(p+k) - t(q+k))^2,
+            w = exp(-max(d2/h^2 - 2, 0)), t' = sum w t / sum w, borders
+            mirrored in plane coordinates; with a history
+            w = exp(-max(d2/h^2 - (v_p + v_q), 0) / v_p);
+  temporal  a = alpha_max exp(-max(d2_3x3/ht^2 - 2, 0)), t' = (1-a) tc + a tp,
+            v = max((1-a)^2 + a^2 v_s, 1e-4), stored as round(sqrt(v) 255)/255
+            squared;
+  chroma    4x4 block means of (R-G, B-G, G), a 9x9 joint bilateral with
+            sigma_s = 2, K = Kc = 0.35, eps = 0.02, mirrored on the quarter
+            grid, recombined through a bilinear up at (p + 0.5)/4 - 0.5.
+
+The scenes are chosen so that the weights sit where a wrong constant moves
+them: two-level planes whose step D in the stabilised domain puts a patch's
+distance on either side of the allowance; a uniform offset D between the
+previous level and this frame's; two known variances; a luminance edge across
+a colour edge. The last test builds variants of the oracle with one constant
+planted wrong (ORACLE_MUTANT in oracle_core.c) and shows that each of these
+checks fails on the variant it is meant to catch.
+"""
+import math
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle
+from libcraw_amd import scene
+
+MODES = [oracle.f32l, oracle.f32d, oracle.f64]
+IDS = lambda m: m.prefix  # noqa: E731
+
+F32 = np.float32
+BLACK = float(F32(scene.BLACK_12))
+WHITE = float(F32(scene.WHITE_12))
+RANGE = WHITE - BLACK
+
+
+def _f(x):
+    """The float a ctypes float field holds, as a double."""
+    return float(F32(x))
+
+
+# ---------------------------------------------------------------- closed forms
+
+def gat_consts(a, b):
+    a = _f(a)
+    c = 3.0 * a * a / 8.0 + max(_f(b), 0.0)
+    sc = math.sqrt(c)
+    return a, c, sc, 2.0 * sc / a
+
+
+def gat(a, b, u):
+    a, c, sc, t0 = gat_consts(a, b)
+    u = np.asarray(u, np.float64)
+    return np.where(u >= 0, (2.0 / a) * np.sqrt(np.maximum(a * u + c, 0.0)), t0 + u / sc)
+
+
+def igat(a, b, t):
+    a, c, sc, t0 = gat_consts(a, b)
+    t = np.asarray(t, np.float64)
+    return np.where(t >= t0, a * t * t / 4.0 - c / a, (t - t0) * sc)
+
+
+def code_u(code):
+    return (np.asarray(code, np.float64) / 65535.0 - BLACK) / RANGE
+
+
+def u_level(u, u0):
+    """The re-encoded level (before rounding) of a denoised u at a sample
+    that came in as u0: the highlight guard, the clamp, times 65535."""
+    g = np.clip((u0 - 0.90) / 0.10, 0.0, 1.0)
+    u = u + (u0 - u) * g
+    return np.clip(BLACK + u * RANGE, 0.0, 1.0) * 65535.0
+
+
+def var_store(v):
+    return (np.rint(np.sqrt(np.clip(v, 0.0, 1.0)) * 255.0) / 255.0) ** 2
+
+
+def mosaic(planes):
+    """An RGGB mosaic from four (h2, w2) planes of codes."""
+    h2, w2 = planes[0].shape
+    m = np.empty((2 * h2, 2 * w2), np.uint16)
+    for s, p in enumerate(planes):
+        m[s >> 1::2, s & 1::2] = p
+    return m
+
+
+def planes_of(m):
+    return [m[s >> 1::2, s & 1::2] for s in range(4)]
+
+
+def nlm_direct(t, h, var=None):
+    """Non-local means of one plane by direct summation over the 7x7 search
+    and the 3x3 patch, mirrored borders (numpy 'reflect' repeats no edge
+    sample, as the plane mirror does)."""
+    r = 4
+    tp = np.pad(t, r, mode="reflect")
+    vp = None if var is None else np.pad(var, r, mode="reflect")
+    H, W = t.shape
+    inv_h2 = 1.0 / (h * h)
+    num = np.zeros_like(t)
+    den = np.zeros_like(t)
+    for dy in range(-3, 4):
+        for dx in range(-3, 4):
+            d2 = np.zeros_like(t)
+            for ky in range(-1, 2):
+                for kx in range(-1, 2):
+                    c = tp[r + ky:r + ky + H, r + kx:r + kx + W]
+                    q = tp[r + dy + ky:r + dy + ky + H, r + dx + kx:r + dx + kx + W]
+                    d2 += (c - q) ** 2
+            tq = tp[r + dy:r + dy + H, r + dx:r + dx + W]
+            if var is None:
+                w = np.exp(-np.maximum(d2 / 9.0 * inv_h2 - 2.0, 0.0))
+            else:
+                vq = vp[r + dy:r + dy + H, r + dx:r + dx + W]
+                w = np.exp(-np.maximum(d2 / 9.0 * inv_h2 - (var + vq), 0.0) / var)
+            num += w * tq
+            den += w
+    return num / den
+
+
+def temporal_direct(tc, tp, v_s, alpha_max, ht):
+    """The temporal blend of one plane: (t', v)."""
+    H, W = tc.shape
+    e = np.pad(tp - tc, 1, mode="reflect") ** 2
+    d2 = sum(e[1 + ky:1 + ky + H, 1 + kx:1 + kx + W] for ky in (-1, 0, 1) for kx in (-1, 0, 1))
+    a = alpha_max * np.exp(-np.maximum(d2 / 9.0 / (ht * ht) - 2.0, 0.0))
+    return (1.0 - a) * tc + a * tp, np.maximum((1.0 - a) ** 2 + a * a * v_s, 1e-4)
+
+
+def chroma_direct(rgb, strength):
+    """The chroma stage of the reference's definition on an (H, W, 4) image:
+    R and B after the stage."""
+    rgb = np.asarray(rgb, np.float64)
+    H, W = rgb.shape[:2]
+    wq, hq = (W + 3) // 4, (H + 3) // 4
+    d = np.stack([rgb[..., 0] - rgb[..., 1], rgb[..., 2] - rgb[..., 1], rgb[..., 1]], -1)
+    pad = np.zeros((4 * hq, 4 * wq, 3))
+    cnt = np.zeros((4 * hq, 4 * wq))
+    pad[:H, :W] = d
+    cnt[:H, :W] = 1.0
+    down = pad.reshape(hq, 4, wq, 4, 3).sum(axis=(1, 3)) / \
+        cnt.reshape(hq, 4, wq, 4).sum(axis=(1, 3))[..., None]
+    R = 4
+    dp = np.pad(down, ((R, R), (R, R), (0, 0)), mode="reflect")
+    L = np.maximum(down[..., 2], 0.0) + 0.02
+    num = np.zeros((hq, wq, 2))
+    den = np.zeros((hq, wq))
+    for dy in range(-R, R + 1):
+        for dx in range(-R, R + 1):
+            s = dp[R + dy:R + dy + hq, R + dx:R + dx + wq]
+            w = np.exp(-(dx * dx + dy * dy) / (2.0 * 2.0 * 2.0)
+                       - ((s[..., 2] - down[..., 2]) / (0.35 * L)) ** 2
+                       - ((s[..., 0] - down[..., 0]) / (0.35 * L)) ** 2
+                       - ((s[..., 1] - down[..., 1]) / (0.35 * L)) ** 2)
+            num += w[..., None] * s[..., :2]
+            den += w
+    filt = num / den[..., None]
+
+    def up(img):
+        fy = (np.arange(H) + 0.5) / 4.0 - 0.5
+        fx = (np.arange(W) + 0.5) / 4.0 - 0.5
+        y0f, x0f = np.floor(fy), np.floor(fx)
+        ty, tx = (fy - y0f)[:, None, None], (fx - x0f)[None, :, None]
+        y0 = np.clip(y0f.astype(int), 0, hq - 1)
+        y1 = np.clip(y0f.astype(int) + 1, 0, hq - 1)
+        x0 = np.clip(x0f.astype(int), 0, wq - 1)
+        x1 = np.clip(x0f.astype(int) + 1, 0, wq - 1)
+        return ((1 - tx) * (1 - ty) * img[y0][:, x0] + tx * (1 - ty) * img[y0][:, x1] +
+                (1 - tx) * ty * img[y1][:, x0] + tx * ty * img[y1][:, x1])
+
+    dn = d[..., :2] + strength * (up(filt) - up(down[..., :2]))
+    return rgb[..., 1] + dn[..., 0], rgb[..., 1] + dn[..., 1]
+
+
+# ------------------------------------------------------------------- the pins
+
+NOISE = [(2e-4, 4e-6), (4e-4, 1.6e-5), (1e-4, 2e-6), (8e-4, 3e-5), (2e-4, 0.0), (3e-4, -1e-6)]
+U = np.array([-0.05, -0.01, -1e-4, 0.0, 1e-5, 1e-3, 0.01, 0.1, 0.5, 1.0, 1.3])
+
+
+def _tol(mode, f64, f32):
+    return f64 if mode.dtype == np.float64 else f32
+
+
+def _wtol(mode, want):
+    """Relative tolerance of an exp weight: in float the argument carries an
+    absolute rounding error of a few ulp of its own size."""
+    return _tol(mode, 1e-12, 1e-6 * (1.0 + abs(math.log(max(want, 1e-300)))))
+
+
+def pin_gat(mode):
+    """T, I and the constants against the closed forms, below black too."""
+    for a, b in NOISE:
+        t, back, consts = mode.gat(a, b, U)
+        a_, c, sc, t0 = gat_consts(a, b)
+        rt = _tol(mode, 1e-13, 2e-6)
+        np.testing.assert_allclose(t, gat(a, b, U), rtol=rt, err_msg="T at %r" % ((a, b),))
+        np.testing.assert_allclose([consts[1], consts[2], consts[6]], [c, sc, t0], rtol=rt)
+        # the inverse at levels on both sides of t0, and the round trip
+        tin = np.array([t0 - 30.0, t0 - 1.0, t0, t0 + 0.5, t0 + 40.0, 2.5 * t0 + 100.0])
+        _, u_back, _ = mode.gat(a, b, U, t_in=tin)
+        np.testing.assert_allclose(u_back, igat(a, b, tin), rtol=rt,
+                                   atol=_tol(mode, 1e-14, 2e-7) * (1.0 + c / a_))
+        np.testing.assert_allclose(back, U, rtol=rt, atol=_tol(mode, 1e-13, 2e-6))
+
+
+def pin_nlm_weight(mode):
+    """The weight against exp(-max(d2/9/h^2 - 2, 0)), and with a history
+    exp(-max(d2/9/h^2 - (vp + vq), 0)/vp). A noise-free step of D = 2 over a
+    whole patch keeps exp(-2); with noise its expected distance is D^2 + 2,
+    where the weight is exp(-D^2): 1.8% at two sigma, as the reference
+    states (DESIGN.md:1700-1706)."""
+    rt = _tol(mode, 1e-14, 3e-7)
+    for inv_h2 in (1.0, 1.0 / 0.49, 4.0):
+        for d2 in (0.0, 3.0, 9.0, 17.9, 18.1, 27.0, 36.0, 60.0, 200.0):
+            want = math.exp(-max(d2 / 9.0 * inv_h2 - 2.0, 0.0))
+            assert mode.nlm_weight(d2, inv_h2) == pytest.approx(want, rel=_wtol(mode, want)), \
+                (d2, inv_h2)
+            for vc, vq in ((1.0, 1.0), (0.2, 0.68), (0.68, 0.2), (0.11, 0.11)):
+                vc, vq = _f(vc), _f(vq)
+                want = math.exp(-max(d2 / 9.0 * inv_h2 - (vc + vq), 0.0) / vc)
+                got = mode.nlm_weight(d2, inv_h2, vc, vq)
+                assert got == pytest.approx(want, rel=_wtol(mode, want), abs=1e-40), \
+                    (d2, vc, vq)
+    assert mode.nlm_weight(9.0 * 4.0, 1.0) == pytest.approx(math.exp(-2.0), rel=rt)
+    two_sigma = mode.nlm_weight(9.0 * (4.0 + 2.0), 1.0)
+    assert two_sigma == pytest.approx(math.exp(-4.0), rel=rt)
+    assert round(100 * two_sigma, 1) == 1.8
+
+
+def pin_alpha_weight(mode):
+    rt = _tol(mode, 1e-14, 3e-7)
+    for am in (0.8, 0.5, 1.0):
+        for inv_h2 in (1.0, 1.0 / 0.49):
+            for d2 in (0.0, 9.0, 18.0, 20.25, 36.0, 81.0):
+                want = _f(am) * math.exp(-max(d2 / 9.0 * inv_h2 - 2.0, 0.0))
+                got = mode.alpha_weight(d2, _f(am), inv_h2)
+                assert got == pytest.approx(want, rel=_wtol(mode, want / _f(am))), \
+                    (am, inv_h2, d2)
+
+
+def pin_blend(mode):
+    for tc, tp, a, vs in ((30.0, 31.5, 0.8, 0.5), (120.0, 100.0, 0.3, 1.0),
+                          (25.0, 25.0, 1.0, 0.0), (60.0, 58.0, 0.05, 0.11)):
+        lvl, v, stored = mode.blend(tc, tp, a, vs)
+        want_v = max((1 - a) ** 2 + a * a * vs, 1e-4)
+        assert lvl == pytest.approx((1 - a) * tc + a * tp, rel=_tol(mode, 1e-15, 3e-7))
+        assert v == pytest.approx(want_v, rel=_tol(mode, 1e-15, 3e-7))
+        assert stored == pytest.approx(var_store(want_v), rel=_tol(mode, 1e-15, 3e-7))
+
+
+def pin_cd_weight(mode):
+    rt = _tol(mode, 1e-12, 2e-6)
+    for g in (0.4, 0.05, -0.01):
+        L = max(g, 0.0) + 0.02
+        for dx, dy in ((0, 0), (1, 0), (2, -3), (4, 4)):
+            for dg, dd0, dd1 in ((0.0, 0.0, 0.0), (0.02, 0.0, 0.0), (0.0, 0.03, -0.01),
+                                 (-0.05, 0.01, 0.04)):
+                want = math.exp(-(dx * dx + dy * dy) / 8.0 - (dg / (0.35 * L)) ** 2 -
+                                (dd0 / (0.35 * L)) ** 2 - (dd1 / (0.35 * L)) ** 2)
+                got = mode.cd_weight(dx, dy, g, dg, dd0, dd1)
+                assert got == pytest.approx(want, rel=_wtol(mode, want), abs=1e-300), \
+                    (g, dx, dy, dg, dd0, dd1)
+
+
+def _steps(a, b, lo, targets):
+    """Codes above `lo` whose stabilised step from it is nearest each target."""
+    t_lo = float(gat(a, b, code_u(lo)))
+    codes = np.arange(lo + 1, lo + 400)
+    dt = gat(a, b, code_u(codes)) - t_lo
+    return [int(codes[np.argmin(np.abs(dt - d))]) for d in targets]
+
+
+def pin_nlm_two_level(mode):
+    """Noise-free planes, each of two levels split at a column, whose steps
+    in the stabilised domain are D = 1.2, 1.8, 2.4 and 3.2: the output of
+    every sample is the mean of the two levels weighted by the number of
+    patch cells that straddle the step, w = exp(-max(n D^2/9/h^2 - 2, 0))."""
+    a, b = 2e-4, 4e-6
+    h2, w2, x0 = 12, 20, 9
+    lo = 542
+    hi = _steps(a, b, lo, [1.2, 1.8, 2.4, 3.2])
+    planes = []
+    for s in range(4):
+        p = np.full((h2, w2), lo, np.uint16)
+        p[:, x0:] = hi[s]
+        planes.append(p)
+    bayer = mosaic(planes)
+    prof = scene.make_profile()
+    mx = lambda v: -v if v < 0 else (2 * w2 - 2 - v if v >= w2 else v)  # noqa: E731
+    for h in (1.0, _f(0.7)):
+        ret, _, _, lvl = mode.denoise_temporal(bayer, prof, a, b, h, want_level=True)
+        assert ret == 0
+        got = planes_of(lvl)
+        for s in range(4):
+            t = {False: float(gat(a, b, code_u(lo))), True: float(gat(a, b, code_u(hi[s])))}
+            D = t[True] - t[False]
+            want = np.empty(w2)
+            for i in range(w2):
+                num = den = 0.0
+                for dx in range(-3, 4):
+                    n = 3 * sum((mx(i + kx) >= x0) != (mx(i + dx + kx) >= x0) for kx in (-1, 0, 1))
+                    w = math.exp(-max(n * D * D / 9.0 / (h * h) - 2.0, 0.0))
+                    num += 7 * w * t[mx(i + dx) >= x0]
+                    den += 7 * w
+                u0 = code_u(hi[s] if i >= x0 else lo)
+                want[i] = u_level(igat(a, b, num / den), u0)
+            np.testing.assert_allclose(got[s], np.broadcast_to(want, (h2, w2)),
+                                       rtol=_tol(mode, 1e-11, 0), atol=_tol(mode, 0, 3e-3),
+                                       err_msg="plane %d, D %.3f, h %g" % (s, D, h))
+
+
+def pin_nlm_direct(mode):
+    """A noisy frame of the seeded scene (clipped rectangle included) against
+    the direct 7x7 x 3x3 summation, at two strengths."""
+    a, b = 2e-4, 4e-6
+    bayer = scene.make_frame(48, 40, seed=9)
+    prof = scene.make_profile()
+    for h in (1.0, _f(0.6)):
+        ret, _, _, lvl = mode.denoise_temporal(bayer, prof, a, b, h, want_level=True)
+        assert ret == 0
+        for s, (pl, got) in enumerate(zip(planes_of(bayer), planes_of(lvl))):
+            u0 = code_u(pl)
+            want = u_level(igat(a, b, nlm_direct(gat(a, b, u0), h)), u0)
+            np.testing.assert_allclose(got, want, rtol=_tol(mode, 1e-10, 0),
+                                       atol=_tol(mode, 1e-9, 4e-3), err_msg="plane %d h %g" % (s, h))
+
+
+def _prev_state(levels, variances):
+    """A state (level, variance), indexed like the mosaic, from four planes
+    of each."""
+    return np.stack([mosaic_f(levels), mosaic_f(variances)])
+
+
+def mosaic_f(planes):
+    h2, w2 = planes[0].shape
+    m = np.empty((2 * h2, 2 * w2))
+    for s, p in enumerate(planes):
+        m[s >> 1::2, s & 1::2] = p
+    return m
+
+
+def pin_temporal_offset(mode):
+    """Flat planes whose previous level sits a uniform D above this frame's
+    in the stabilised domain: the 3x3 distance is D^2, the weight
+    a = alpha_max exp(-max(D^2/ht^2 - 2, 0)), the next state's level
+    I((1-a) tc + a tp) and its variance the stored max((1-a)^2 + a^2 v_s,
+    1e-4). With the spatial filter on, a flat plane is its fixed point, so
+    the output is that level too."""
+    a, b = 2e-4, 4e-6
+    am = 0.8
+    h2, w2 = 8, 10
+    codes = [560, 700, 900, 1400]
+    offs = [0.6, 1.5, 2.0, 2.8]
+    vs = [1.0, 0.5, 0.25, 0.11]
+    prof = scene.make_profile()
+    bayer = mosaic([np.full((h2, w2), c, np.uint16) for c in codes])
+    tc = [float(gat(a, b, code_u(c))) for c in codes]
+    prev = _prev_state([np.full((h2, w2), float(igat(a, b, tc[s] + offs[s]))) for s in range(4)],
+                       [np.full((h2, w2), vs[s]) for s in range(4)])
+    for strength in (0.0, _f(0.7), 1.0):
+        ht = strength if strength > 0 else 1.0
+        ret, _, nxt, lvl = mode.denoise_temporal(bayer, prof, a, b, strength, alpha_max=am,
+                                                 prev=prev, want_state=True, want_level=True)
+        assert ret == 0
+        for s in range(4):
+            al = _f(am) * math.exp(-max(offs[s] ** 2 / (ht * ht) - 2.0, 0.0))
+            tn = (1 - al) * tc[s] + al * (tc[s] + offs[s])
+            v = max((1 - al) ** 2 + al * al * vs[s], 1e-4)
+            what = "plane %d D %g strength %g" % (s, offs[s], strength)
+            np.testing.assert_allclose(planes_of(nxt[0])[s], float(igat(a, b, tn)),
+                                       rtol=_tol(mode, 1e-12, 2e-5), err_msg=what)
+            np.testing.assert_allclose(planes_of(nxt[1])[s], var_store(v),
+                                       rtol=_tol(mode, 1e-14, 1e-6), err_msg=what)
+            np.testing.assert_allclose(planes_of(lvl)[s],
+                                       float(u_level(igat(a, b, tn), code_u(codes[s]))),
+                                       rtol=_tol(mode, 1e-11, 2e-6), err_msg=what)
+
+
+def pin_history_two_level(mode):
+    """With a history: planes of two levels split at a column (D near 1.2, so
+    patch distances straddle the allowance), a previous state equal to this
+    frame (the temporal weight is then alpha_max) whose variance is 1 in the
+    top rows and 0.25 below, so v_p, v_q in {0.68, 0.2} at alpha_max 0.8:
+    the output is the mean weighted by exp(-max(n D^2/9/h^2 - (v_p + v_q),
+    0) / v_p)."""
+    a, b = 2e-4, 4e-6
+    am = 0.8
+    h2, w2, x0, y0 = 14, 16, 7, 6
+    lo = 600
+    hi = _steps(a, b, lo, [0.9, 1.2, 1.5, 2.1])
+    planes = []
+    for s in range(4):
+        p = np.full((h2, w2), lo, np.uint16)
+        p[:, x0:] = hi[s]
+        planes.append(p)
+    bayer = mosaic(planes)
+    v_s = np.where(np.arange(h2)[:, None] < y0, 1.0, 0.25) * np.ones((h2, w2))
+    prev = _prev_state([igat(a, b, gat(a, b, code_u(p))) for p in planes], [v_s] * 4)
+    prof = scene.make_profile()
+    mir = lambda v, n: -v if v < 0 else (2 * n - 2 - v if v >= n else v)  # noqa: E731
+    al = _f(am)
+    var = lambda k: (1 - al) ** 2 + al * al * (1.0 if k < y0 else 0.25)  # noqa: E731
+    for h in (1.0, _f(0.8)):
+        ret, _, _, lvl = mode.denoise_temporal(bayer, prof, a, b, h, alpha_max=am, prev=prev,
+                                               want_level=True)
+        assert ret == 0
+        for s in range(4):
+            t = {False: float(gat(a, b, code_u(lo))), True: float(gat(a, b, code_u(hi[s])))}
+            D = t[True] - t[False]
+            want = np.empty((h2, w2))
+            for k in range(h2):
+                vp = var(k)
+                for i in range(w2):
+                    num = den = 0.0
+                    for dy in range(-3, 4):
+                        vq = var(mir(k + dy, h2))
+                        for dx in range(-3, 4):
+                            n = 3 * sum((mir(i + kx, w2) >= x0) != (mir(i + dx + kx, w2) >= x0)
+                                        for kx in (-1, 0, 1))
+                            w = math.exp(-max(n * D * D / 9.0 / (h * h) - (vp + vq), 0.0) / vp)
+                            num += w * t[mir(i + dx, w2) >= x0]
+                            den += w
+                    u0 = code_u(hi[s] if i >= x0 else lo)
+                    want[k, i] = u_level(igat(a, b, num / den), u0)
+            np.testing.assert_allclose(planes_of(lvl)[s], want, rtol=_tol(mode, 1e-10, 0),
+                                       atol=_tol(mode, 0, 3e-3),
+                                       err_msg="plane %d D %.3f h %g" % (s, D, h))
+
+
+def pin_temporal_direct(mode):
+    """Two frames of the moving clip: the second's output and next state
+    against the direct temporal blend and history NLM."""
+    a, b = 4e-4, 1.6e-5
+    w, h = 48, 40
+    prof = scene.make_profile()
+    f0 = scene.clip_frame(w, h, 0, square=16, speed=3)
+    f1 = scene.clip_frame(w, h, 1, square=16, speed=3)
+    ret, _, st, _ = mode.denoise_temporal(f0, prof, a, b, 1.0, alpha_max=0.8, want_state=True)
+    assert ret == 0
+    for strength in (1.0, 0.0):
+        ret, _, nxt, lvl = mode.denoise_temporal(f1, prof, a, b, strength, alpha_max=0.8,
+                                                 prev=st, want_state=True, want_level=True)
+        assert ret == 0
+        ht = strength if strength > 0 else 1.0
+        for s, (pl, sl, sv, nl, nv, got) in enumerate(zip(
+                planes_of(f1), planes_of(st[0]), planes_of(st[1]), planes_of(nxt[0]),
+                planes_of(nxt[1]), planes_of(lvl))):
+            u0 = code_u(pl)
+            tc, tp = gat(a, b, u0), gat(a, b, np.asarray(sl, np.float64))
+            tn, v = temporal_direct(tc, tp, np.asarray(sv, np.float64), _f(0.8), ht)
+            out = nlm_direct(tn, ht, v) if strength > 0 else tn
+            what = "plane %d strength %g" % (s, strength)
+            np.testing.assert_allclose(nl, igat(a, b, tn), rtol=_tol(mode, 1e-11, 0),
+                                       atol=_tol(mode, 1e-14, 2e-6), err_msg=what)
+            np.testing.assert_allclose(nv, var_store(v), atol=_tol(mode, 1e-13, 1e-6),
+                                       err_msg=what)
+            np.testing.assert_allclose(got, u_level(igat(a, b, out), u0),
+                                       rtol=_tol(mode, 1e-10, 0), atol=_tol(mode, 1e-9, 4e-3),
+                                       err_msg=what)
+
+
+def _chroma_scenes():
+    rng = np.random.default_rng(12)
+    out = []
+    # a luminance edge (rows) crossing a colour edge (columns), block aligned
+    h, w = 40, 48
+    img = np.ones((h, w, 4))
+    g = np.where(np.arange(h)[:, None] < 20, 0.25, 0.3125) * np.ones((h, w))
+    dr = np.where(np.arange(w)[None, :] < 24, 0.03125, 0.0625) * np.ones((h, w))
+    img[..., 1] = g
+    img[..., 0] = g + dr
+    img[..., 2] = g - 0.015625
+    out.append(("edges", img, 1.0))
+    # flat G, a one-cell chroma step
+    img = np.ones((h, w, 4))
+    img[..., 1] = 0.375
+    img[..., 0] = 0.375 + 0.0234375
+    img[..., 2] = 0.375 - 0.01171875
+    img[16:20, 28:32, 0] += 0.046875
+    out.append(("cell", img, 0.5))
+    # noisy colour over smooth shading, partial blocks
+    h, w = 38, 46
+    yy, xx = np.mgrid[0:h, 0:w]
+    img = np.ones((h, w, 4))
+    img[..., 1] = 0.2 + 0.004 * xx + 0.003 * yy
+    img[..., 0] = img[..., 1] * 1.3 + rng.normal(0, 0.01, (h, w))
+    img[..., 2] = img[..., 1] * 0.8 + rng.normal(0, 0.01, (h, w))
+    img[:, 30:, 1] += 0.1
+    out.append(("noisy", img.astype(np.float32).astype(np.float64), _f(0.7)))
+    return out
+
+
+def pin_chroma_definition(mode):
+    for name, img, strength in _chroma_scenes():
+        ret, got = mode.chroma_denoise(img, strength)
+        assert ret == 0
+        r, b_ = chroma_direct(np.asarray(np.asarray(img, mode.dtype), np.float64), strength)
+        tol = dict(rtol=_tol(mode, 1e-12, 0), atol=_tol(mode, 1e-14, 3e-7))
+        np.testing.assert_allclose(got[..., 0], r, err_msg=name + " R", **tol)
+        np.testing.assert_allclose(got[..., 2], b_, err_msg=name + " B", **tol)
+        assert np.array_equal(got[..., 1], np.asarray(img, mode.dtype)[..., 1])
+
+
+PINS = [pin_gat, pin_nlm_weight, pin_alpha_weight, pin_blend, pin_cd_weight, pin_nlm_two_level,
+        pin_nlm_direct, pin_temporal_offset, pin_history_two_level, pin_temporal_direct,
+        pin_chroma_definition]
+
+
+@pytest.mark.parametrize("pin", PINS, ids=lambda p: p.__name__)
+@pytest.mark.parametrize("mode", MODES, ids=IDS)
+def test_pin(mode, pin):
+    pin(mode)
+
+
+# ORACLE_MUTANT -> what it plants, and the pins that must catch it
+MUTANTS = {
+    1: ("NLM allowance 2 -> 1", [pin_nlm_weight, pin_nlm_two_level, pin_nlm_direct]),
+    2: ("patch mean 1/9 -> 1/8", [pin_nlm_weight, pin_nlm_two_level, pin_nlm_direct]),
+    3: ("GAT c without b", [pin_gat, pin_nlm_two_level, pin_nlm_direct]),
+    4: ("temporal 3x3 mean 1/9 -> 1/8", [pin_alpha_weight, pin_temporal_offset,
+                                         pin_temporal_direct]),
+    5: ("temporal allowance 2 -> 1", [pin_alpha_weight, pin_temporal_offset,
+                                      pin_temporal_direct]),
+    7: ("GAT scale 2/a -> 1/a", [pin_gat, pin_nlm_two_level]),
+    8: ("GAT tangent slope 1/sc -> sc", [pin_gat]),
+    9: ("blend towards the wrong frame", [pin_blend, pin_temporal_offset, pin_temporal_direct]),
+    10: ("(1 - a) for (1 - a)^2 in the variance", [pin_blend, pin_temporal_offset,
+                                                   pin_temporal_direct]),
+    11: ("v_p + v_p for v_p + v_q", [pin_nlm_weight, pin_history_two_level,
+                                     pin_temporal_direct]),
+    12: ("no division by v_p", [pin_nlm_weight, pin_history_two_level, pin_temporal_direct]),
+    13: ("sigma_s 2 -> 1", [pin_cd_weight, pin_chroma_definition]),
+    14: ("K 0.35 -> 0.25", [pin_cd_weight, pin_chroma_definition]),
+    15: ("Kc 0.35 -> 0.25", [pin_cd_weight, pin_chroma_definition]),
+    16: ("eps 0.02 -> 0.05", [pin_cd_weight, pin_chroma_definition]),
+}
+
+
+@pytest.fixture(scope="module")
+def mutant_libs(tmp_path_factory):
+    """The oracle's double mode rebuilt once per planted mistake."""
+    d = tmp_path_factory.mktemp("mutants")
+    here = os.path.dirname(oracle.__file__)
+    root = os.path.dirname(here)
+    inc = ["-I", os.path.join(root, "include"), "-I", os.path.join(root, "libcraw_amd", "csrc"),
+           "-I", here]
+    flags = ["-O1", "-fPIC", "-std=gnu11", "-mfma", "-ffp-contract=off"]
+    host = str(d / "host.o")
+    subprocess.run(["gcc", *flags, *inc, "-c", os.path.join(here, "oracle_host.c"), "-o", host],
+                   check=True)
+    procs, libs = [], {}
+    for n in MUTANTS:
+        so = str(d / ("mutant%d.so" % n))
+        procs.append(subprocess.Popen(
+            ["gcc", *flags, *inc, "-shared", "-DORACLE_MODE=3", "-DORACLE_MUTANT=%d" % n,
+             os.path.join(here, "oracle_core.c"), host, "-o", so, "-lm"]))
+        libs[n] = so
+    for p in procs:
+        assert p.wait() == 0
+    return libs
+
+
+@pytest.mark.parametrize("n", sorted(MUTANTS), ids=lambda n: "%d-%s" % (n, MUTANTS[n][0]))
+def test_pins_catch_planted_mistakes(mutant_libs, n):
+    """Each closed-form pin meant for a constant fails on the oracle with
+    that constant planted wrong (and the same pins pass on the oracle
+    proper, test_pin)."""
+    mode = oracle.Mode("oracle_f64_", np.float64, path=mutant_libs[n])
+    for pin in MUTANTS[n][1]:
+        with pytest.raises(AssertionError):
+            pin(mode)
