@@ -108,11 +108,11 @@ class Pipeline:
                      for p in (h.count, h.weight, h.weight_bright))
 
     def stage_times(self, frames_back=0):
-        out = (capi.CRAWStageTime * 8)()
-        n = self._lib.craw_pipeline_get_stage_times_n(self.handle, frames_back, out, 8)
+        out = (capi.CRAWStageTime * 32)()
+        n = self._lib.craw_pipeline_get_stage_times_n(self.handle, frames_back, out, 32)
         if n < 0:
             raise CrawError("get_stage_times", n)
-        return [(out[i].name.decode(), out[i].count, out[i].ms) for i in range(min(n, 8))]
+        return [(out[i].name.decode(), out[i].count, out[i].ms) for i in range(min(n, 32))]
 
     def close(self):
         if self.handle.value:

@@ -41,27 +41,37 @@ def make_profile(wb=(2.0, 1.0, 1.6), black=BLACK_12, white=WHITE_12, matrix=CAME
 
 
 def scene_rgb(width, height, seed, x_off=0, y_off=0):
-    """Scene-linear RGB, float64 (H, W, 3), deterministic in `seed`."""
+    """Scene-linear RGB, float64 (H, W, 3), deterministic in `seed`.
+
+    The texture is six plane waves sin(kx x + ky y + phase); each is split as
+    sin(kx x) cos(ky y + phase) + cos(kx x) sin(ky y + phase), so that the
+    sum of all six for one channel is one (H x 12) by (12 x W) product."""
     rng = np.random.Generator(np.random.Philox(seed))
-    y, x = np.mgrid[0:height, 0:width].astype(np.float64)
-    x = x + x_off
-    y = y + y_off
+    x = np.arange(width, dtype=np.float64) + x_off
+    y = np.arange(height, dtype=np.float64) + y_off
     fx = x / max(width - 1, 1)
     fy = y / max(height - 1, 1)
-    rgb = np.empty((height, width, 3))
-    rgb[..., 0] = 0.08 + 0.55 * fx + 0.10 * fy
-    rgb[..., 1] = 0.10 + 0.45 * fy + 0.08 * fx
-    rgb[..., 2] = 0.12 + 0.30 * fx + 0.30 * (1.0 - fy)
+    rgb = np.empty((3, height, width))
+    rgb[0] = 0.08 + 0.55 * fx[None, :] + 0.10 * fy[:, None]
+    rgb[1] = 0.10 + 0.45 * fy[:, None] + 0.08 * fx[None, :]
+    rgb[2] = 0.12 + 0.30 * fx[None, :] + 0.30 * (1.0 - fy[:, None])
 
     # sinusoidal texture, periods 2..40 px, mixed orientations
+    cols, rows, tints = [], [], []
     for _ in range(6):
         period = rng.uniform(2.0, 40.0)
         theta = rng.uniform(0.0, np.pi)
         amp = rng.uniform(0.02, 0.08)
         phase = rng.uniform(0.0, 2 * np.pi)
-        wave = amp * np.sin(2 * np.pi * (x * np.cos(theta) + y * np.sin(theta)) / period + phase)
+        ax = 2 * np.pi * x * np.cos(theta) / period
+        by = 2 * np.pi * y * np.sin(theta) / period + phase
+        cols += [np.sin(ax), np.cos(ax)]
+        rows += [amp * np.cos(by), amp * np.sin(by)]
         tint = rng.uniform(0.5, 1.0, size=3)
-        rgb += wave[..., None] * tint
+        tints += [tint, tint]
+    cols, rows, tints = np.array(cols), np.array(rows), np.array(tints)
+    for c in range(3):
+        rgb[c] += (rows.T * tints[:, c]) @ cols
 
     # hard-edged rectangles, one of them above the clip
     n_rect = 12
@@ -73,8 +83,9 @@ def scene_rgb(width, height, seed, x_off=0, y_off=0):
         col = rng.uniform(0.02, 0.9, size=3)
         if i == 0:
             col = np.array([1.3, 1.2, 1.25])
-        rgb[y0:y0 + h, x0:x0 + w] = col
-    return np.clip(rgb, 0.0, 1.5)
+        rgb[:, y0:y0 + h, x0:x0 + w] = col[:, None, None]
+    np.clip(rgb, 0.0, 1.5, out=rgb)
+    return np.ascontiguousarray(np.moveaxis(rgb, 0, -1))
 
 
 def mosaic_from_scene(rgb, wb=(2.0, 1.0, 1.6), a=2e-4, b=4e-6, seed=0, black=BLACK_12,
@@ -82,12 +93,9 @@ def mosaic_from_scene(rgb, wb=(2.0, 1.0, 1.6), a=2e-4, b=4e-6, seed=0, black=BLA
     """RGGB uint16 mosaic: the scene with the gains divided out, noise
     N(0, a*u + b) added, re-encoded to the 12-bit range."""
     height, width, _ = rgb.shape
-    chan = np.zeros((height, width), dtype=np.int64)
-    chan[0::2, 1::2] = 1
-    chan[1::2, 0::2] = 1
-    chan[1::2, 1::2] = 2
-    gains = np.asarray(wb, dtype=np.float64)[chan]
-    u = np.take_along_axis(rgb, chan[..., None], axis=2)[..., 0] / gains
+    u = np.empty((height, width))
+    for s, c in enumerate((0, 1, 1, 2)):
+        u[s >> 1::2, s & 1::2] = rgb[s >> 1::2, s & 1::2, c] / float(wb[c])
     rng = np.random.Generator(np.random.Philox(seed ^ 0x5EED))
     sigma = np.sqrt(a * np.maximum(u, 0.0) + b)
     u = u + sigma * rng.standard_normal(u.shape)

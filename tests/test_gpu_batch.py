@@ -253,6 +253,31 @@ def test_temporal_streams(gpu_available):
             assert_bit_exact(fr.output(c), runs[c].output(), "step %d clip %d" % (t, c))
 
 
+def test_state_untouched_by_a_batch_without_the_pass(gpu_available):
+    """A clip's state attached to a recording of a batch built without the
+    temporal pass is left as it was (lib/pipeline.c:2308 moves the state
+    only on a pipeline with the pass): the clip's next frame still reads the
+    frame before."""
+    flags = capi.CRAW_PIPELINE_FLAG_DENOISE | capi.CRAW_PIPELINE_FLAG_TEMPORAL_DENOISE
+    size = (128, 96)
+    prof = scene.make_profile()
+    fr = Frames([size])
+    with_pass, without = new_batch([size], flags), new_batch([size], capi.CRAW_PIPELINE_FLAG_DENOISE)
+    state = TemporalState(context(), *size)
+    run = Runner(*size, flags=flags)
+    stream = hip.Stream()
+    fp = frame_params(prof, denoise_strength=1.0, denoise_temporal_strength=0.7,
+                      noise_override=(4e-4, 1.6e-5))
+    for t, b in enumerate((with_pass, without, with_pass)):
+        mos = [scene.clip_frame(*size, t, seed=61)]
+        fr.upload(mos)
+        assert b.record(stream, [batch_frame(*fr.images(0), fp, state)]) == 0
+        stream.synchronize()
+        if b is with_pass:
+            assert run.run(mos[0], fp) == 0
+            assert_bit_exact(fr.output(0), run.output(), "step %d" % t)
+
+
 def test_autotune_keeps_outputs(gpu_available):
     prof = scene.make_profile()
     fr = Frames(SIZES)

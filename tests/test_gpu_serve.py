@@ -216,3 +216,42 @@ def test_budget_and_errors(gpu_available):
     for sid in sids:
         srv.close_stream(sid)
     srv.close()
+
+
+def test_fp16_temporal_streams_equal_plain_fp16(gpu_available):
+    """The served workload's own form: packed-FP16 NLM with the temporal
+    pass on clip streams (and stills beside them), each frame equal to a
+    plain FP16 pipeline developing that clip alone."""
+    prof = scene.make_profile()
+    flags = FULL | capi.CRAW_PIPELINE_FLAG_NLM_FP16
+    sizes = [(160, 96), (200, 136)]
+    srv = new_server(flags=flags, max_frames=3)
+    sids = [srv.open_stream(w, h) for w, h in sizes]
+    jobs, reqs = {}, []
+    for t in range(4):
+        for c, (w, h) in enumerate(sizes):
+            fp = frame_params(prof, denoise_strength=1.0, denoise_chroma_strength=0.5,
+                              denoise_temporal_strength=0.8,
+                              noise_override=(2e-4 * (1 + c), 4e-6 * (1 + c)))
+            jobs[(c, t)] = Job(scene.clip_frame(w, h, t, seed=70 + c, square=40, speed=4), fp,
+                               stream=sids[c])
+            reqs.append(jobs[(c, t)].request(100 * c + t))
+    st = [still(128, 96, 900 + i, prof) for i in range(2)]
+    reqs += [s.request(1000 + i) for i, s in enumerate(st)]
+    assert srv.submit_n(reqs) == 0
+    done = wait_all(srv, len(reqs))
+    assert all(c["status"] == 0 for c in done.values())
+    for c, (w, h) in enumerate(sizes):
+        run = Runner(w, h, flags=flags)
+        for t in range(4):
+            j = jobs[(c, t)]
+            assert run.run(j.mosaic, j.fp) == 0
+            assert_bit_exact(j.output(), run.output(), "fp16 clip %d frame %d" % (c, t))
+            assert np.array_equal(j.hist, np.concatenate(run.pipe.histogram()))
+    for i, s in enumerate(st):
+        out, hist = plain_of(s, flags)
+        assert_bit_exact(s.output(), out, "fp16 still %d" % i)
+        assert np.array_equal(s.hist, hist)
+    for sid in sids:
+        srv.close_stream(sid)
+    srv.close()
