@@ -14,7 +14,8 @@ same K frames are then recorded through a pipeline created with
 CRAW_PIPELINE_FLAG_PROFILE, whose per-stage hipEvents ride on the stages'
 kernel dispatches (hipExtLaunchKernel) on the same stream: `roofline` is
 the dominant kernel's algorithmic bytes per launch over its average launch
-time from those events, `kernels` the same for every kernel, and
+time from those events, `kernels` the same for every kernel (each against the roof it is nearer to:
+HBM bytes, or FP32 FLOP for the NLM counted on its definition), and
 `ms_per_step_profiled` that run's frame time (the dispatch timestamps add
 a few microseconds per kernel). `roofline.traffic` is the dominant
 kernel's HBM bytes a launch (read + write) from the committed rocprofv3
@@ -52,7 +53,8 @@ sys.path.insert(0, ROOT)
 import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-VALU_PEAK_TOPS = 39.3   # fp32 lane operations/s, non-packed: 256 CU x 64 lanes x 2.4 GHz
+FP32_PEAK_TFLOPS = 157.3      # MI355X published dense FP32 (packed FMA)
+FP32_NONPACKED_TFLOPS = 78.6  # the same without packed instructions (half)
 
 W, H = 5760, 3240
 NOISE = (2e-4, 4e-6)
@@ -68,15 +70,20 @@ ALGO_BYTES_PER_PX = {
     "awb_clear": 0.0,
     "awb_export": 0.0,
 }
-# VALU lane operations per sample of the exact NLM (7x7 search, 3x3 patch)
-# as the pairwise-weight kernel evaluates it: 24 weights a sample (a
-# difference and a square, 8 adds for the raster-order distance, 3 for the
-# argument and 1 for its clamp, 18 for the double-precision exp: convert,
-# reduce in 4, a degree-10 polynomial in 10 FMAs, scale, round to float),
-# 48 neighbours accumulated (3 each: weight sum, product, sum) and the
-# centre (3). The weights of the neighbours before the centre come from
-# other lanes through ds_bpermute, an LDS instruction.
-NLM_OPS_PER_SAMPLE = 24 * (2 + 8 + 3 + 1 + 18) + 48 * 3 + 3
+# Algorithmic floating-point operations per output sample of the NLM 7x7
+# search / 3x3 patch, counted on the definition (no sharing of symmetric
+# pairs, which is how the kernel halves its work): 49 x 9 = 441 squared
+# differences at 2 each, 49 x 8 adds for the patch sums, 49 weight
+# arguments and exps at 5 each (scale, subtract, clamp, negate, exp), 49 x 3
+# accumulations (weight sum, product, sum) and the final division.
+NLM_FLOP_PER_SAMPLE = 441 * 2 + 49 * 8 + 49 * 5 + 49 * 3 + 1
+# Per stage: algorithmic FLOP a pixel where the stage is bound by arithmetic
+# rather than by HBM (the NLM with or without its packed-FP16 form).
+ALGO_FLOP_PER_PX = {
+    "denoise_nlm": NLM_FLOP_PER_SAMPLE,
+    "denoise_nlm_fp16": NLM_FLOP_PER_SAMPLE,
+    "denoise_nlm_fast": NLM_FLOP_PER_SAMPLE,
+}
 
 
 def parse():
@@ -554,28 +561,10 @@ def main():
     pipe_prof.close()
     stage_ms = {k: v[1] / v[0] for k, v in stages.items()}
     dom = max(stage_ms, key=stage_ms.get)
-    algo = ALGO_BYTES_PER_PX.get(dom, 0.0) * W * H
-    achieved = algo / (stage_ms[dom] / 1e3) / 1e9
-    roofline = {
-        "kernel": dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-        "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": measured_traffic(dom),
-        "traffic_source": os.path.relpath(COUNTERS_MD, ROOT),
-        "algo_bytes_per_launch": int(algo), "avg_launch_ms": round(stage_ms[dom], 4),
-    }
-    if dom == "denoise_nlm":
-        ops = NLM_OPS_PER_SAMPLE * W * H
-        tops = ops / (stage_ms[dom] / 1e3) / 1e12
-        roofline["valu"] = {"achieved": round(tops, 2), "peak": VALU_PEAK_TOPS,
-                            "unit": "T fp32 lane-ops/s", "frac": round(tops / VALU_PEAK_TOPS, 3),
-                            "ops_per_sample": NLM_OPS_PER_SAMPLE}
-
-    kernels = {}
-    for k, ms in stage_ms.items():
-        b = ALGO_BYTES_PER_PX.get(k, 0.0) * W * H
-        kernels[k] = {"avg_launch_ms": round(ms, 4), "algo_bytes": int(b),
-                      "achieved_gbs": round(b / (ms / 1e3) / 1e9, 1),
-                      "frac_hbm": round(b / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4)}
+    kernels = {k: kernel_roofs(k, ms) for k, ms in stage_ms.items()}
+    roofline = dict(kernels[dom])
+    roofline.update({"kernel": dom, "traffic": measured_traffic(dom),
+                     "traffic_source": os.path.relpath(COUNTERS_MD, ROOT)})
 
     # plain-form latency: each frame recorded and waited for alone
     lat = []
@@ -633,13 +622,62 @@ def main():
                                        "ms_per_frame": round(ms16, 4), "stage_ms": st16}}
         out["forms"].update(measure_forms(args, ctx, capi, hip, P, scene, value))
     if not args.no_cpu_baseline:
-        threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
+        # every core this process may use, up to the GPU box's CPU share of
+        # 16 threads a GPU (the box shows more cores than one job is given)
+        cpu = host_cpu()
+        threads = args.cpu_threads or min(cpu["usable"] or 1, 16)
         out["cpu_baseline"] = cpu_baseline(threads)
+        out["cpu_baseline"]["host"] = cpu
     print(json.dumps(out))
 
 
 def capi_slots():
     return 4
+
+
+def kernel_roofs(stage, ms):
+    """One stage's achieved rate against each roof it has an algorithmic
+    count for: HBM bytes (ALGO_BYTES_PER_PX) and FP32 FLOP
+    (ALGO_FLOP_PER_PX), at 5760x3240. `bound` is the roof the stage is
+    nearer to; `achieved`/`peak`/`frac` are against it, the other is kept
+    beside it."""
+    px = W * H
+    secs = ms / 1e3
+    b = ALGO_BYTES_PER_PX.get(stage, 0.0) * px
+    hbm = {"achieved": round(b / secs / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": round(b / secs / 1e9 / HBM_PEAK_GBS, 4)}
+    out = {"avg_launch_ms": round(ms, 4), "algo_bytes_per_launch": int(b)}
+    f = ALGO_FLOP_PER_PX.get(stage)
+    if f is None:
+        out.update({"bound": "hbm", **hbm})
+        return out
+    tf = f * px / secs / 1e12
+    valu = {"achieved": round(tf, 2), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(tf / FP32_PEAK_TFLOPS, 4),
+            "frac_of_nonpacked": round(tf / FP32_NONPACKED_TFLOPS, 4)}
+    out["algo_flop_per_launch"] = int(f * px)
+    if valu["frac"] >= hbm["frac"]:
+        out.update({"bound": "valu", **valu, "hbm": hbm})
+    else:
+        out.update({"bound": "hbm", **hbm, "valu": valu})
+    return out
+
+
+def host_cpu():
+    """The host's CPU model and core counts, as the CPU baseline ran on it."""
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count()
+    return {"model": model, "nproc": os.cpu_count(), "usable": usable}
 
 
 # The kernel behind each stage of the plain 5.8K pipeline, as rocprofv3 names it.
