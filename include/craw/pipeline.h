@@ -183,6 +183,31 @@ CRAW_API int craw_pipeline_record(CRAWPipeline *pipe, CRAWStream stream,
                                   const CRAWImage *src, const CRAWImage *dst,
                                   const CRAWFrameParams *fp);
 
+/* One frame developed with no pipeline and no allocation: prep,
+ * debayer, colour (and gamut compression) into `dst`, and, when
+ * `histogram` is not NULL, the AWB histogram into it (3 x 128^2 u32:
+ * count, weight, weight_bright; device-addressable, 16-byte aligned,
+ * complete when `stream` reaches the end of this call's work). The same
+ * kernels and constants as craw_pipeline_record() on a pipeline built with
+ * `debayer`, `out_space`, dst's format and CRAW_PIPELINE_FLAG_AWB_HISTOGRAM
+ * (when `histogram` is set) and no other stage flag: the frame's denoise,
+ * chroma, false-colour and highlight settings are ignored, as such a
+ * pipeline ignores them.
+ *
+ * It has no counterpart in the reference. It is for callers that cannot
+ * allocate device memory: code under HIP graph capture, serving loops and
+ * harnesses that forbid hipMalloc. It calls no allocator: the histogram's
+ * scratch (192 KiB plus 8 bytes an 8x8 block, at most 2^19 blocks, e.g.
+ * 8192x4096 or 6000x4000; 4.2 MiB in all) is a device array each GPU the
+ * library runs on holds until the process ends, shared by every call on
+ * that GPU: calls that overlap on one GPU race. A frame with more blocks,
+ * with a histogram, fails with CRAW_ERROR(EINVAL); the other errors are
+ * craw_pipeline_record()'s. */
+CRAW_API int craw_develop_noalloc(CRAWContext *ctx, CRAWStream stream, const CRAWImage *src,
+                                  const CRAWImage *dst, const CRAWFrameParams *fp,
+                                  enum CRAWDebayer debayer, enum CRAWColorSpace out_space,
+                                  uint32_t *histogram);
+
 /* The histogram of the last recording (or of `frames_back` recordings
  * before it, up to nb_awb_slots - 1), in pinned host memory owned by the
  * pipeline. Valid once the caller has synchronised that recording.

@@ -239,6 +239,8 @@ SIGNATURES = [
     ("craw_pipeline_create", c_i32, [vp, P(CRAWPipelineParams), P(vp)]),
     ("craw_pipeline_free", None, [P(vp)]),
     ("craw_pipeline_record", c_i32, [vp, vp, P(CRAWImage), P(CRAWImage), P(CRAWFrameParams)]),
+    ("craw_develop_noalloc", c_i32, [vp, vp, P(CRAWImage), P(CRAWImage), P(CRAWFrameParams),
+                                     c_i32, c_i32, vp]),
     ("craw_pipeline_get_awb_histogram", c_i32, [vp, P(CRAWAwbHistogram)]),
     ("craw_pipeline_get_awb_histogram_n", c_i32, [vp, c_u32, P(CRAWAwbHistogram)]),
     ("craw_pipeline_get_stage_times", c_i32, [vp, P(CRAWStageTime), c_u32]),

@@ -45,7 +45,7 @@ CRAW_API int craw_context_create(CRAWContext **out, const CRAWHipInitParams *par
 {
     CRAWContext *ctx;
     hipDeviceProp_t prop;
-    int n = 0;
+    int n = 0, ret;
 
     if (!out || !params)
         return CRAW_ERROR(EINVAL);
@@ -71,6 +71,15 @@ CRAW_API int craw_context_create(CRAWContext **out, const CRAWHipInitParams *par
         return CRAW_ERROR(ENOTSUP);
     }
 
+    if (hipSetDevice(ctx->device) != hipSuccess) {
+        free(ctx);
+        return CRAW_ERROR(ENODEV);
+    }
+    ret = craw_noalloc_setup(ctx);
+    if (ret < 0) {
+        free(ctx);
+        return ret;
+    }
     pthread_mutex_init(&ctx->noise_lock, NULL);
     craw_log(ctx, CRAW_LOG_VERBOSE, "Context on device %d (%s, %d CUs)\n", ctx->device,
              ctx->arch, ctx->cu_count);

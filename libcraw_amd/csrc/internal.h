@@ -153,6 +153,10 @@ void craw_profiler_finish(CRAWProfRun &r);
 int craw_profiler_times(CRAWContext *ctx, CRAWProfiler *pr, uint32_t frames_back,
                         CRAWStageTime *out, uint32_t max);
 
+/* Looks up, once per device, the address of the allocation-free entry
+ * point's scratch array on ctx->device (which must be current). */
+int craw_noalloc_setup(CRAWContext *ctx);
+
 /* Blocks of the AWB grid of a W x H frame. */
 static inline size_t craw_awb_blocks(uint32_t w, uint32_t h)
 {
