@@ -19,7 +19,8 @@
  * versions would be the ones to differ.
  *
  *   exp: Cody-Waite reduction x = k ln2 + r, |r| <= ln2/2, with ln2 split
- *        into a 32-bit head and a tail (the fdlibm split), degree-10
+ *        into a 32-bit head and a tail (the fdlibm split) and k rounded by
+ *        the 1.5 * 2^52 shift, degree-10
  *        Chebyshev polynomial of exp(r) in Horner form with FMA, scaled by
  *        2^k through the exponent field.
  *   log: x = m 2^e with m in [sqrt(1/2), sqrt(2)), log m = 2 atanh(s),
@@ -74,7 +75,13 @@ CRAW_DM_FN double craw_dm_exp_d(double x)
     const double inv_ln2 = 0x1.71547652b82fep+0;
     const double ln2_hi  = 0x1.62e42fee00000p-1;
     const double ln2_lo  = 0x1.a39ef35793c76p-33;
-    double k = __builtin_rint(x * inv_ln2);
+    /* k = the integer nearest x / ln2: x / ln2 + 1.5 * 2^52 rounded once
+     * (one FMA) holds k in its low mantissa bits, and taking the shift
+     * back off is exact. Over every float argument in [-200, 90] this k
+     * equals rint(x * inv_ln2) (checked exhaustively). */
+    const double shift = 0x1.8p52;
+    double t = __builtin_fma(x, inv_ln2, shift);
+    double k = t - shift;
     double r = __builtin_fma(-k, ln2_hi, x);
     double p;
     int32_t ki;
@@ -101,7 +108,7 @@ CRAW_DM_FN double craw_dm_exp_d(double x)
     /* p in [0.70, 1.42]; 2^k is a normal double for |k| <= 1010, so the
      * scaling is exact (one ldexp on the device, a multiply by the power
      * built in the exponent field on the host: the same value). */
-    ki = (int32_t)k;
+    ki = (int32_t)(uint32_t)craw_dm_bits(t);
 #if defined(__HIP_DEVICE_COMPILE__)
     return __builtin_ldexp(p, ki);
 #else

@@ -49,8 +49,22 @@ typedef struct CRAWAwbSample {
  * reference does; a NaN lands in bin 0), and the 16.16 weights
  * min(g, 0.5) and g^4 rounded half away from zero. The logarithm is the
  * deterministic one of detmath.h. Returns false for a skipped block. */
+CRAW_DM_FN bool craw_awb_sample_log(const float acc[3], float n, float wb0, float wb2,
+                                    bool float_log, CRAWAwbSample *s);
+
 CRAW_DM_FN bool craw_awb_sample(const float acc[3], float n, float wb0, float wb2,
                                 CRAWAwbSample *s)
+{
+    return craw_awb_sample_log(acc, n, wb0, wb2, false, s);
+}
+
+/* The same with the logarithm chosen: float_log (device code only) takes
+ * it from the hardware's float log2 (v_log_f32) times ln 2, with no double
+ * arithmetic: the precision form's histogram, whose bins then differ from
+ * the correctly rounded logarithm's only for ratios within an ulp or two
+ * of a bin edge. */
+CRAW_DM_FN bool craw_awb_sample_log(const float acc[3], float n, float wb0, float wb2,
+                                    bool float_log, CRAWAwbSample *s)
 {
     const float lmin = -2.5f, lmax = 1.0f, span = lmax - lmin;
     float r = acc[0] / n, g = acc[1] / n, b = acc[2] / n;
@@ -64,8 +78,19 @@ CRAW_DM_FN bool craw_awb_sample(const float acc[3], float n, float wb0, float wb
 
     rr = (r / wb0) / g;
     bb = (b / wb2) / g;
+#if defined(__HIP_DEVICE_COMPILE__)
+    if (float_log) {
+        x = __builtin_amdgcn_logf(rr) * 0.693147180559945309f;
+        y = __builtin_amdgcn_logf(bb) * 0.693147180559945309f;
+    } else {
+        x = craw_dm_logf(rr);
+        y = craw_dm_logf(bb);
+    }
+#else
+    (void)float_log;
     x = craw_dm_logf(rr);
     y = craw_dm_logf(bb);
+#endif
     x = x > lmin ? (x < lmax ? x : lmax) : lmin;
     y = y > lmin ? (y < lmax ? y : lmax) : lmin;
 

@@ -78,6 +78,27 @@ KDEV float kprep(uint32_t raw, int s, const KPrep &p)
  * as the reference applies them. */
 #define KCOLOR_CLAMP_NEGATIVE 1u
 #define KCOLOR_GAMUT_COMPRESS 2u
+/* The precision form (CRAW_PIPELINE_FLAG_NLM_FP16): the transcendentals of
+ * the colour, chroma and histogram stages in float on the hardware's
+ * v_exp_f32 / v_log_f32 (2^x and log2 x, about an ulp each), with no
+ * double arithmetic, instead of the correctly rounded ones of detmath.h. */
+#define KCOLOR_FLOAT_MATH     4u
+
+KDEV float kf_exp(float x)
+{
+    return __builtin_amdgcn_exp2f(x * 1.44269504088896341f);
+}
+
+KDEV float kf_log(float x)
+{
+    return __builtin_amdgcn_logf(x) * 0.693147180559945309f;
+}
+
+/* x^y for x >= 0: 2^(y log2 x), 0 at x = 0 for y > 0. */
+KDEV float kf_pow(float x, float y)
+{
+    return __builtin_amdgcn_exp2f(y * __builtin_amdgcn_logf(x));
+}
 
 struct KColor {
     float cam[3][3];
@@ -111,8 +132,13 @@ KDEV void kgamut(const KColor &c, float &r, float &g, float &b)
         if (d < thr)
             continue;
         nd = (d - thr) / scl;
-        p = craw_dm_powf(nd, c.pwr);
-        v = ach - (thr + scl * nd / craw_dm_powf(1.0f + p, c.inv_pwr)) * mag;
+        if (c.flags & KCOLOR_FLOAT_MATH) {
+            p = kf_pow(nd, c.pwr);
+            v = ach - (thr + scl * nd / kf_pow(1.0f + p, c.inv_pwr)) * mag;
+        } else {
+            p = craw_dm_powf(nd, c.pwr);
+            v = ach - (thr + scl * nd / craw_dm_powf(1.0f + p, c.inv_pwr)) * mag;
+        }
         if (i == 0)
             r = v;
         else if (i == 1)
@@ -153,6 +179,7 @@ struct KOut {
     uint32_t *hist;
     uint2 *samples;
     float awb_wb0, awb_wb2;
+    int float_log;       /* the precision form's float logarithm (craw_awb_sample_log) */
 };
 
 /* A block's histogram sample, packed: x = bin | weight << 14 (bin < 2^14,

@@ -400,6 +400,7 @@ struct SmCb {
     float q[3][CB_LH][CB_LW];
 };
 
+template <bool FLOAT_MATH>
 KDEV void dev_chroma_blur(SmCb &sm, const KFrame &f, int qx0, int qy0)
 {
     const KInter &it = f.inter;
@@ -448,8 +449,9 @@ KDEV void dev_chroma_blur(SmCb &sm, const KFrame &f, int qx0, int qy0)
             const float dg = (s2 - c2) * inv_range;
             const float dd0 = (s0 - c0) * inv_chroma;
             const float dd1 = (s1 - c1) * inv_chroma;
-            const float wgt = craw_dm_expf_nonpos(-(float)(dx * dx + dy * dy) * 0.125f
-                                                  - dg * dg - dd0 * dd0 - dd1 * dd1);
+            const float arg = -(float)(dx * dx + dy * dy) * 0.125f - dg * dg - dd0 * dd0 -
+                              dd1 * dd1;
+            const float wgt = FLOAT_MATH ? kf_exp(arg) : craw_dm_expf_nonpos(arg);
 
             acc0 += wgt * s0;
             acc1 += wgt * s1;
@@ -688,7 +690,10 @@ __global__ __launch_bounds__(RC_NT) void k_debayer_rcd(KFrame f)
 __global__ __launch_bounds__(CB_TX * CB_TY) void k_chroma_blur(KFrame f)
 {
     __shared__ SmCb sm;
-    dev_chroma_blur(sm, f, blockIdx.x * CB_TX, blockIdx.y * CB_TY);
+    if (f.col.flags & KCOLOR_FLOAT_MATH)
+        dev_chroma_blur<true>(sm, f, blockIdx.x * CB_TX, blockIdx.y * CB_TY);
+    else
+        dev_chroma_blur<false>(sm, f, blockIdx.x * CB_TX, blockIdx.y * CB_TY);
 }
 
 /* Seven waves a SIMD: 72 VGPRs without a spill (the compiler's own choice
@@ -755,7 +760,10 @@ __global__ __launch_bounds__(CB_TX * CB_TY) void kb_chroma_blur(const KFrame *__
         const int ntx = (f.inter.wq + CB_TX - 1) / CB_TX, i = KTILE_IDX(d);
 
         __syncthreads();
-        dev_chroma_blur(sm, f, (i % ntx) * CB_TX, (i / ntx) * CB_TY);
+        if (f.col.flags & KCOLOR_FLOAT_MATH)
+            dev_chroma_blur<true>(sm, f, (i % ntx) * CB_TX, (i / ntx) * CB_TY);
+        else
+            dev_chroma_blur<false>(sm, f, (i % ntx) * CB_TX, (i / ntx) * CB_TY);
     }
 }
 

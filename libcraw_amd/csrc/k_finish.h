@@ -68,8 +68,9 @@ KDEV void kfinish(const float (*rgbs)[TH][TW], int x0, int y0, int W, int H,
             if (gx0 < W && gy0 < H) {
                 for (int c = 0; c < 3; c++)
                     acc[c] = kblock_sum<TW, TH>(rgbs[c], gx0 - x0, gy0 - y0, gx1 - gx0, gy1 - gy0);
-                const bool valid = craw_awb_sample(acc, (float)((gy1 - gy0) * (gx1 - gx0)),
-                                                   out.awb_wb0, out.awb_wb2, &smp);
+                const bool valid = craw_awb_sample_log(acc, (float)((gy1 - gy0) * (gx1 - gx0)),
+                                                       out.awb_wb0, out.awb_wb2, out.float_log,
+                                                       &smp);
                 out.samples[(size_t)(gy0 / 8) * ((W + 7) / 8) + gx0 / 8] =
                     valid ? make_uint2(smp.bin | (smp.weight << 14), smp.bright)
                           : make_uint2(KAWB_NONE, 0u);
@@ -92,8 +93,8 @@ KDEV void kfinish(const float (*rgbs)[TH][TW], int x0, int y0, int W, int H,
         acc[1] = __shfl(sum, tid + NB, 64);
         acc[2] = __shfl(sum, tid + 2 * NB, 64);
         if (inside && c == 0) {
-            valid = craw_awb_sample(acc, (float)((gy1 - gy0) * (gx1 - gx0)), out.awb_wb0,
-                                    out.awb_wb2, &smp);
+            valid = craw_awb_sample_log(acc, (float)((gy1 - gy0) * (gx1 - gx0)), out.awb_wb0,
+                                        out.awb_wb2, out.float_log, &smp);
             out.samples[(size_t)(gy0 / 8) * ((W + 7) / 8) + gx0 / 8] =
                 valid ? make_uint2(smp.bin | (smp.weight << 14), smp.bright)
                       : make_uint2(KAWB_NONE, 0u);

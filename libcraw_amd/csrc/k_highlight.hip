@@ -69,6 +69,18 @@ KDEV float hl_ramp(float u)
     return t * t * (3.0f - 2.0f * t);
 }
 
+/* log and exp: correctly rounded (detmath.h), or in the precision form
+ * the hardware's float ones */
+KDEV float hl_log(const KHighlight &h, float x)
+{
+    return h.float_math ? kf_log(x) : craw_dm_logf(x);
+}
+
+KDEV float hl_exp(const KHighlight &h, float x)
+{
+    return h.float_math ? kf_exp(x) : craw_dm_expf(x);
+}
+
 KDEV float hl_wb(const KHighlight &h, int c)
 {
     return c == 0 ? h.wb[0] : (c == 1 ? h.wb[1] : h.wb[2]);
@@ -179,8 +191,8 @@ __global__ __launch_bounds__(HF_NT) void k_hl_field(KHighlight h)
                 m = HMIN(umax, 1.0f);
                 wq = (1.0f - amax) * m * m;
                 g = HMAX(rgb[1], HL_EPS);
-                r = HCLAMP(craw_dm_logf(HMAX(rgb[0], HL_EPS) / g), -HL_LOG_MAX, HL_LOG_MAX);
-                b = HCLAMP(craw_dm_logf(HMAX(rgb[2], HL_EPS) / g), -HL_LOG_MAX, HL_LOG_MAX);
+                r = HCLAMP(hl_log(h, HMAX(rgb[0], HL_EPS) / g), -HL_LOG_MAX, HL_LOG_MAX);
+                b = HCLAMP(hl_log(h, HMAX(rgb[2], HL_EPS) / g), -HL_LOG_MAX, HL_LOG_MAX);
                 acc.x += wq * r;
                 acc.y += wq * b;
                 acc.z += wq;
@@ -408,14 +420,14 @@ __global__ __launch_bounds__(HL_FNT) void k_hl_fill(KHighlight h)
             k2 = c == 2 ? 1 : 2;
             a1 = 1.0f - am[k1];
             a2 = 1.0f - am[k2];
-            l1 = craw_dm_logf(HMAX(sum[k1] / n[k1], HL_EPS)) - lrho[k1];
-            l2 = craw_dm_logf(HMAX(sum[k2] / n[k2], HL_EPS)) - lrho[k2];
-            lc = craw_dm_logf(HMAX(xc, HL_EPS)) - lrho[c];
+            l1 = hl_log(h, HMAX(sum[k1] / n[k1], HL_EPS)) - lrho[k1];
+            l2 = hl_log(h, HMAX(sum[k2] / n[k2], HL_EPS)) - lrho[k2];
+            lc = hl_log(h, HMAX(xc, HL_EPS)) - lrho[c];
             A = a1 + a2;
-            lfull = craw_dm_expf(HMAX(lc, HMAX(l1, l2)));
-            lpart = A > 0.0f ? craw_dm_expf((a1 * l1 + a2 * l2) / A) : lfull;
+            lfull = hl_exp(h, HMAX(lc, HMAX(l1, l2)));
+            lpart = A > 0.0f ? hl_exp(h, (a1 * l1 + a2 * l2) / A) : lfull;
             L = lfull + HMIN(A, 1.0f) * (lpart - lfull);
-            res = xc + a * HMAX(L * craw_dm_expf(lrho[c]) - xc, 0.0f);
+            res = xc + a * HMAX(L * hl_exp(h, lrho[c]) - xc, 0.0f);
         }
         h.mosaic[(size_t)y * h.W + x] = __half_as_ushort(__float2half_rn(res));
     }

@@ -19,7 +19,7 @@
  * exps.
  *
  * Layout: a workgroup's four waves stand side by side over a band of
- * 224 columns of the plane (57 rows, 30 with history), each lane one
+ * 224 columns of the plane (48 rows, 30 with history), each lane one
  * column, and walk down it a row at a time. A lane keeps, per offset, the
  * squared differences of three rows in registers and forms the weights of
  * one row; the patch sums across columns come from the neighbouring lanes
@@ -271,9 +271,10 @@ struct XsBand {
          * the weight exp(-0) = 1), then the offsets after it with this
          * lane's own weights */
         {
-            const float wv = HIST ? 1.0f : xs_weight(0.0f, inv_h2);
-            L.ws[0] += wv;
-            L.a[0] += wv * plane[r][lc];
+            /* exp(-max(0 / 9 / h^2 - 2, 0)) = exp(-0) = 1 for the finite
+             * positive 1 / h^2 the host forms */
+            L.ws[0] += 1.0f;
+            L.a[0] += 1.0f * plane[r][lc];
         }
 #pragma unroll
         for (int o = 0; o < XS_NO; o++) {

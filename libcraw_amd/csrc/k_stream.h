@@ -81,7 +81,8 @@ KDEV void rs_finish(float (&acc)[3], const KFrame &f, int oy, bool out_cols, int
         }
         if ((oy & 7) == 7 && out_cols && (X0 & 7) == 0) {
             CRAWAwbSample smp;
-            const bool valid = craw_awb_sample(acc, 64.0f, f.out.awb_wb0, f.out.awb_wb2, &smp);
+            const bool valid = craw_awb_sample_log(acc, 64.0f, f.out.awb_wb0, f.out.awb_wb2,
+                                                   f.out.float_log, &smp);
 
             f.out.samples[(size_t)(oy / 8) * ((f.W + 7) / 8) + X0 / 8] =
                 valid ? make_uint2(smp.bin | (smp.weight << 14), smp.bright)

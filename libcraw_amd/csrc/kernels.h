@@ -117,8 +117,12 @@ struct KFrame {
 #define KG_FT_TH 64
 #define KG_AWB_CHUNK 8192
 #define KG_XS_TW 224
-#define KG_XS_TH 57
+#ifndef KG_XS_TH
+#define KG_XS_TH 48             /* 3 workgroups a CU in 53 KB of LDS */
+#endif
+#ifndef KG_XH_TH
 #define KG_XH_TH 30             /* the same kernel with a temporal history */
+#endif
 #define KG_RS_OUT 116           /* streamed RCD: output columns per strip */
 #define KG_RS_OUT_F 112         /* the same without the chroma stage */
 #define KG_RS_R   40            /* and rows */
@@ -144,7 +148,7 @@ enum KStage {
 
 /* Which denoise kernel's tiles: the exact tiled kernel of the temporal
  * filter alone (32x16), the fast forms' (28x64), the exact kernel with
- * pairwise distances for the spatial modes, without (224x57) and with
+ * pairwise distances for the spatial modes, without (224x48) and with
  * (224x30) a temporal history. */
 enum KDnGeom {
     KDN_GEOM_DIRECT = 0,
@@ -365,6 +369,7 @@ struct KHighlight {
     int W, H;
     float black[4], inv_range[4];
     float wb[3];
+    int float_math;              /* the precision form: float log/exp (kf_log, kf_exp) */
 };
 
 /* Level 0 and its pull to levels 1-4; the rest of the pull and every push;

@@ -330,6 +330,10 @@ int craw_frame_plan(CRAWContext *ctx, const CRAWStageConfig &cfg, const CRAWFram
     plan->f.col.flags = (fp->flags & CRAW_FRAME_FLAG_CLAMP_NEGATIVE) ? KCOLOR_CLAMP_NEGATIVE : 0u;
     if (out_space == CRAW_CS_ACES_AP1_LINEAR && !(fp->flags & CRAW_FRAME_FLAG_NO_GAMUT_COMPRESSION))
         plan->f.col.flags |= KCOLOR_GAMUT_COMPRESS;
+    /* the precision form: no double arithmetic on the device, the colour,
+     * chroma, histogram and highlight stages' transcendentals in float */
+    if (cfg.fp16)
+        plan->f.col.flags |= KCOLOR_FLOAT_MATH;
 
     plan->f.out.dst = dst->data;
     plan->f.out.pitch = dst->pitch;
@@ -340,6 +344,7 @@ int craw_frame_plan(CRAWContext *ctx, const CRAWStageConfig &cfg, const CRAWFram
         return CRAW_ERROR(EINVAL);
     plan->f.out.awb_wb0 = frame.wb[0];
     plan->f.out.awb_wb2 = frame.wb[2];
+    plan->f.out.float_log = cfg.fp16 ? 1 : 0;
 
     plan->f.W = (int)w;
     plan->f.H = (int)h;
@@ -382,6 +387,7 @@ int craw_frame_plan(CRAWContext *ctx, const CRAWStageConfig &cfg, const CRAWFram
         }
         for (int c = 0; c < 3; c++)
             k.wb[c] = frame.wb[c];
+        k.float_math = cfg.fp16 ? 1 : 0;
         plan->f.src = buf.hl_mosaic;
         plan->f.src_pitch = (int)w;
         plan->f.prep.half = 1;
