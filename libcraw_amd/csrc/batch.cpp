@@ -215,13 +215,14 @@ static int batch_sizes(CRAWContext *ctx, const CRAWBatchParams *params, CRAWStag
     for (int st = 0; st < KSTAGE_NB; st++)
         sz->tiles_cap += stage_tiles_bound(st, params, (int)params->debayer,
                                            fast ? KDN_GEOM_FAST : KDN_GEOM_DIRECT);
-    /* exact batches may also hold frames with the pairwise-distance tiles */
-    if (!fast) {
-        sz->tiles_cap += stage_tiles_bound(KSTAGE_DENOISE, params, (int)params->debayer,
-                                           KDN_GEOM_SYM);
+    /* frames with the pairwise-distance tiles (the exact form's spatial
+     * modes, with and without history, and the precision form's spatial
+     * ones) */
+    sz->tiles_cap += stage_tiles_bound(KSTAGE_DENOISE, params, (int)params->debayer,
+                                       KDN_GEOM_SYM);
+    if (!fast)
         sz->tiles_cap += stage_tiles_bound(KSTAGE_DENOISE, params, (int)params->debayer,
                                            KDN_GEOM_SYMH);
-    }
     sz->tiles_cap += NB_LISTS * 64;
     return 0;
 }

@@ -41,6 +41,16 @@
  * and forms its own. Each division by a variance is A times the lane's
  * double reciprocal of it (one a row), rounded once to float: the
  * correctly rounded float quotient the reference computes (xs_quot).
+ *
+ * The precision form (F16, without history) walks the same band with the
+ * same pairs: each position's difference for an offset is formed in FP32
+ * (two stabilised samples, both large, their difference small) and packed
+ * with the next offset's into a half2; squares, the patch sums (across
+ * columns through the same DPP shifts, then down three rows of those
+ * column sums) and the weight's argument are packed FP16, the weight the
+ * hardware exp2, and the sums FP32. A pair's weight is still formed once,
+ * for both of its samples: half the distances and half the exps of
+ * evaluating every sample's 48 neighbours.
  */
 #include "k_common.h"
 #include "kernels.h"
@@ -54,8 +64,12 @@
 #define XS_LW    (XS_TW + 2 * XS_AX)   /* 238 */
 #define XS_NO    24                    /* offsets after the centre */
 
+#define XS_NH    (XS_NO / 2)           /* packed pairs of offsets */
+
 #define DPP_WAVE_SHR1 0x138
 #define DPP_WAVE_SHL1 0x130
+
+typedef _Float16 xh2 __attribute__((ext_vector_type(2)));
 
 /* Band rows and the LDS it stages: without history the plane; with it the
  * plane and the variance, which share the LDS with the staged frames the
@@ -117,6 +131,18 @@ KDEV float xs_shl(float v)
                                                                  DPP_WAVE_SHL1, 0xf, 0xf, true));
 }
 
+KDEV xh2 xs_shr16(xh2 v)
+{
+    return __builtin_bit_cast(xh2, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v),
+                                                               DPP_WAVE_SHR1, 0xf, 0xf, true));
+}
+
+KDEV xh2 xs_shl16(xh2 v)
+{
+    return __builtin_bit_cast(xh2, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v),
+                                                               DPP_WAVE_SHL1, 0xf, 0xf, true));
+}
+
 /* lib/ref/ref_denoise.c:36-58 */
 KDEV float xs_gat(const KDenoise &p, float u)
 {
@@ -159,9 +185,10 @@ struct XsLane {
     double rv[4];               /* with history: 1 / v of rows r .. r + 3 */
 };
 
-template <int MODE>
+template <int MODE, bool F16 = false>
 struct XsBand {
     static constexpr bool HIST = XsG<MODE>::HIST;
+    static_assert(!(F16 && HIST), "the precision form's pairwise kernel has no history mode");
     SmXs<MODE> &sm;
     const KFrame &f;
     int lane, lc, x, y0, px, py;
@@ -243,6 +270,17 @@ struct XsBand {
             }
         }
 
+        terms(w, A, r, L);
+    }
+
+    /* Adds weight row r's terms (w, and with history the arguments A) to
+     * the four output rows it feeds, completes output row r and moves the
+     * partial sums up a row. */
+    __attribute__((always_inline)) KDEV void terms(const float (&w)[XS_NO],
+                                                   const float (&A)[XS_NO], int r, XsLane &L)
+    {
+        auto &plane = sm.m.plane;
+
         /* Row r is neighbour row ndy of output row r - ndy. Term (ndx, ndy)
          * of an output at column X is the weight of the pair (X + ndx, X)
          * in X's sum, formed from what lane X + ndx holds for o = (-ndx,
@@ -264,7 +302,7 @@ struct XsBand {
                 const float q = plane[r][lc + ndx];
 
                 L.ws[k] += wv;
-                L.a[k] += wv * q;
+                L.a[k] = F16 ? __builtin_fmaf(wv, q, L.a[k]) : L.a[k] + wv * q;
             }
         }
         /* output row r: the centre (distance 0: the argument is at most 0,
@@ -281,7 +319,7 @@ struct XsBand {
             const float q = plane[r + xs_dy(o)][lc + xs_dx(o)];
 
             L.ws[0] += w[o];
-            L.a[0] += w[o] * q;
+            L.a[0] = F16 ? __builtin_fmaf(w[o], q, L.a[0]) : L.a[0] + w[o] * q;
         }
 
         if (r >= XS_AY) {
@@ -316,6 +354,54 @@ struct XsBand {
                 L.rv[k] = L.rv[k + 1];
         }
     }
+
+    /* The horizontal patch sums of LDS row r's squared differences for
+     * every offset, packed by pairs of offsets (precision form). */
+    __attribute__((always_inline)) KDEV void hrow16(int r, xh2 (&h)[XS_NH])
+    {
+        auto &plane = sm.m.plane;
+        const float cen = plane[r][lc];
+        xh2 sq[XS_NH];
+
+#pragma unroll
+        for (int j = 0; j < XS_NH; j++) {
+            const float e0 = cen - plane[r + xs_dy(2 * j)][lc + xs_dx(2 * j)];
+            const float e1 = cen - plane[r + xs_dy(2 * j + 1)][lc + xs_dx(2 * j + 1)];
+            const xh2 e = __builtin_bit_cast(xh2, __builtin_amdgcn_cvt_pkrtz(e0, e1));
+
+            sq[j] = e * e;
+        }
+#pragma unroll
+        for (int j = 0; j < XS_NH; j++)
+            h[j] = (xs_shr16(sq[j]) + sq[j]) + xs_shl16(sq[j]);
+    }
+
+    /* Weight row r of the precision form: the column sums of row r + 1
+     * into hc, the patch distances from those of rows r - 1 (ha), r (hb)
+     * and r + 1, the argument d2 / 9 / h^2 - 2 clamped at 0 (times log2 e)
+     * in FP16, the weight exp2 of it in FP32. */
+    __attribute__((always_inline)) KDEV void step16(const xh2 (&ha)[XS_NH],
+                                                    const xh2 (&hb)[XS_NH], xh2 (&hc)[XS_NH],
+                                                    int r, XsLane &L)
+    {
+        const float l2e = 1.44269504088896341f;
+        const xh2 kk = { (_Float16)(-(1.0f / 9.0f) * inv_h2 * l2e),
+                         (_Float16)(-(1.0f / 9.0f) * inv_h2 * l2e) };
+        const xh2 c2 = { (_Float16)(2.0f * l2e), (_Float16)(2.0f * l2e) };
+        const xh2 zero = { (_Float16)0.0f, (_Float16)0.0f };
+        float w[XS_NO];
+
+        hrow16(r + 1, hc);
+#pragma unroll
+        for (int j = 0; j < XS_NH; j++) {
+            xh2 d2 = (ha[j] + hb[j]) + hc[j];
+
+            d2 = __builtin_elementwise_min(__builtin_elementwise_fma(d2, kk, c2), zero);
+            w[2 * j] = __builtin_amdgcn_exp2f((float)d2.x);
+            w[2 * j + 1] = __builtin_amdgcn_exp2f((float)d2.y);
+        }
+        terms(w, w, r, L);
+    }
 };
 
 /* The variance's state encoding: its square root in eight bits
@@ -333,7 +419,7 @@ KDEV float xs_var_decode(uint8_t c)
     return q * q;
 }
 
-template <int MODE>
+template <int MODE, bool F16>
 KDEV void dev_denoise_sym(SmXs<MODE> &sm, const KFrame &f, int x0, int y0, int pl)
 {
     using G = XsG<MODE>;
@@ -453,11 +539,31 @@ KDEV void dev_denoise_sym(SmXs<MODE> &sm, const KFrame &f, int x0, int y0, int p
     /* LDS column of this lane; plane column x0 - 4 + 56 wave + lane */
     const int lc = XS_AX - 4 + XS_OUTW * wave + lane;
     const int x = x0 - 4 + XS_OUTW * wave + lane;
-    XsBand<MODE> band = { sm, f, lane, lc, x, y0, px, py,
-                          lane >= 4 && lane < 4 + XS_OUTW && x < w2,
-                          black, range, inv_range, p.inv_h2 };
-    float s0[XS_NO], s1[XS_NO], s2[XS_NO];
+    XsBand<MODE, F16> band = { sm, f, lane, lc, x, y0, px, py,
+                               lane >= 4 && lane < 4 + XS_OUTW && x < w2,
+                               black, range, inv_range, p.inv_h2 };
     XsLane L;
+
+    if constexpr (F16) {
+        xh2 h0[XS_NH], h1[XS_NH], h2[XS_NH];
+
+        /* the column sums of the two rows above the first weight row */
+        band.hrow16(0, h0);
+        band.hrow16(1, h1);
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            L.a[k] = L.ws[k] = 0.0f;
+            L.rv[k] = 0.0;
+        }
+#pragma unroll 1
+        for (int r = 1; r <= TH + 3; r += 3) {
+            band.step16(h0, h1, h2, r, L);
+            band.step16(h1, h2, h0, r + 1, L);
+            band.step16(h2, h0, h1, r + 2, L);
+        }
+        return;
+    }
+    float s0[XS_NO], s1[XS_NO], s2[XS_NO];
 
     /* squared differences of the two rows above the first weight row
      * (LDS row 1, plane row y0 - 3) */
@@ -491,14 +597,15 @@ KDEV void dev_denoise_sym(SmXs<MODE> &sm, const KFrame &f, int x0, int y0, int p
     }
 }
 
-template <int MODE>
+template <int MODE, bool F16>
 __global__ __launch_bounds__(XS_NT) void k_denoise_sym(KFrame f)
 {
     __shared__ SmXs<MODE> sm;
-    dev_denoise_sym<MODE>(sm, f, blockIdx.x * XS_TW, blockIdx.y * XsG<MODE>::TH, blockIdx.z);
+    dev_denoise_sym<MODE, F16>(sm, f, blockIdx.x * XS_TW, blockIdx.y * XsG<MODE>::TH,
+                               blockIdx.z);
 }
 
-template <int MODE>
+template <int MODE, bool F16>
 __global__ __launch_bounds__(XS_NT) void kb_denoise_sym(const KFrame *__restrict__ frames,
                                                         const uint32_t *__restrict__ tiles,
                                                         uint32_t ntiles)
@@ -513,7 +620,7 @@ __global__ __launch_bounds__(XS_NT) void kb_denoise_sym(const KFrame *__restrict
         const int i = KTILE_IDX(d), pl = i / (ntx * nty), r = i % (ntx * nty);
 
         __syncthreads();
-        dev_denoise_sym<MODE>(sm, f, (r % ntx) * XS_TW, (r / ntx) * TH, pl);
+        dev_denoise_sym<MODE, F16>(sm, f, (r % ntx) * XS_TW, (r / ntx) * TH, pl);
     }
 }
 
@@ -527,7 +634,7 @@ hipError_t craw_launch_denoise_sym(int mode, const KFrame &f, hipStream_t s)
     case M: {                                                                              \
         const dim3 grid((f.dn.w2 + XS_TW - 1) / XS_TW, (f.dn.h2 + XsG<M>::TH - 1) / XsG<M>::TH, \
                         4);                                                                \
-        KLAUNCH(k_denoise_sym<M>, grid, dim3(XS_NT), 0, s, f);                             \
+        KLAUNCH((k_denoise_sym<M, false>), grid, dim3(XS_NT), 0, s, f);                    \
         break;                                                                             \
     }
     XS_MODES(X)
@@ -546,10 +653,50 @@ hipError_t craw_launch_batch_denoise_sym(int mode, const KFrame *frames, const u
     switch (mode) {
 #define X(M)                                                                               \
     case M:                                                                                \
-        KLAUNCH(kb_denoise_sym<M>, dim3(grid), dim3(XS_NT), 0, s, frames, tiles, ntiles);  \
+        KLAUNCH((kb_denoise_sym<M, false>), dim3(grid), dim3(XS_NT), 0, s, frames, tiles,   \
+                ntiles);                                                                   \
         break;
     XS_MODES(X)
 #undef X
+    default:
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+/* The precision form's spatial modes (no history): packed-FP16 distances,
+ * each pair's weight formed once. */
+hipError_t craw_launch_denoise_sym16(int mode, const KFrame &f, hipStream_t s)
+{
+    const dim3 grid((f.dn.w2 + XS_TW - 1) / XS_TW, (f.dn.h2 + KG_XS_TH - 1) / KG_XS_TH, 4);
+
+    switch (mode) {
+    case KDN_SPATIAL:
+        KLAUNCH((k_denoise_sym<KDN_SPATIAL, true>), grid, dim3(XS_NT), 0, s, f);
+        break;
+    case KDN_SPATIAL | KDN_TEMPORAL:
+        KLAUNCH((k_denoise_sym<KDN_SPATIAL | KDN_TEMPORAL, true>), grid, dim3(XS_NT), 0, s, f);
+        break;
+    default:
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t craw_launch_batch_denoise_sym16(int mode, const KFrame *frames, const uint32_t *tiles,
+                                           uint32_t ntiles, int grid, hipStream_t s)
+{
+    if (!ntiles)
+        return hipSuccess;
+    switch (mode) {
+    case KDN_SPATIAL:
+        KLAUNCH((kb_denoise_sym<KDN_SPATIAL, true>), dim3(grid), dim3(XS_NT), 0, s, frames, tiles,
+                ntiles);
+        break;
+    case KDN_SPATIAL | KDN_TEMPORAL:
+        KLAUNCH((kb_denoise_sym<KDN_SPATIAL | KDN_TEMPORAL, true>), dim3(grid), dim3(XS_NT), 0, s,
+                frames, tiles, ntiles);
+        break;
     default:
         return hipErrorInvalidValue;
     }

@@ -159,6 +159,10 @@ enum KDnGeom {
 
 static inline int craw_dn_geom(int form, int mode)
 {
+    /* the precision form's spatial modes share the exact form's pairwise
+     * band kernel, in packed FP16 */
+    if (form == KFORM_FP16 && (mode & KDN_SPATIAL) && !(mode & KDN_HISTORY))
+        return KDN_GEOM_SYM;
     if (form != KFORM_EXACT)
         return KDN_GEOM_FAST;
     if (!(mode & KDN_SPATIAL))
@@ -397,6 +401,9 @@ hipError_t craw_launch_denoise_fast(int mode, bool fp16, const KFrame &f, hipStr
 
 /* The exact form with pairwise weights (modes without a history). */
 hipError_t craw_launch_denoise_sym(int mode, const KFrame &f, hipStream_t s);
+hipError_t craw_launch_denoise_sym16(int mode, const KFrame &f, hipStream_t s);
+hipError_t craw_launch_batch_denoise_sym16(int mode, const KFrame *frames, const uint32_t *tiles,
+                                           uint32_t ntiles, int grid, hipStream_t s);
 hipError_t craw_launch_batch_denoise_sym(int mode, const KFrame *frames, const uint32_t *tiles,
                                          uint32_t ntiles, int grid, hipStream_t s);
 

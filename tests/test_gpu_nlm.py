@@ -158,3 +158,19 @@ def test_temporal_forms(gpu_available, form):
     else:
         for i, (o, lvl) in enumerate(fp64_levels(frames, 1.0, 0.8, a, b)):
             check_bound(g[i], o, lvl, "%s frame %d" % (form, i))
+
+
+@pytest.mark.parametrize("size,strength", [((912, 272), 1.0), ((1000, 300), 0.6), ((458, 140), 1.0),
+                                           ((40, 520), 1.0), ((5760, 3240), 1.0)])
+def test_fp16_pairwise_bands_within_bound_of_fp64(gpu_available, size, strength):
+    """The precision form's spatial modes run on the pairwise band kernel
+    (224x48 bands, packed-FP16 distances): several bands, partial last
+    bands across and down, a plane narrower than a band's reach, and the
+    benchmarked 5760x3240 frame, each within the bound."""
+    a, b = 2e-4, 4e-6
+    w, h = size
+    oracle.f64.set_threads(16)
+    frames = [scene.make_frame(w, h, seed=29, a=a, b=b)]
+    g = gpu_sequence("fp16", frames, strength, 0.0, a, b)[0]
+    o, lvl = fp64_levels(frames, strength, 0.0, a, b)[0]
+    check_bound(g, o, lvl, "fp16 %dx%d h %g" % (w, h, strength))
