@@ -384,7 +384,7 @@ static hipError_t launch_batch_fast(int mode, const KFrame *frames, const uint32
 
 hipError_t craw_launch_denoise_fast(int mode, bool fp16, const KFrame &f, hipStream_t s)
 {
-    if (fp16 && craw_dn_geom(KFORM_FP16, mode) == KDN_GEOM_SYM)
+    if (fp16 && craw_dn_geom(KFORM_FP16, mode) != KDN_GEOM_FAST)
         return craw_launch_denoise_sym16(mode, f, s);
     return fp16 ? launch_fast<true>(mode, f, s) : launch_fast<false>(mode, f, s);
 }
@@ -400,7 +400,7 @@ hipError_t craw_launch_batch_denoise(int mode, int form, const KFrame *frames,
         return hipSuccess;
     if (form == KFORM_EXACT)
         return craw_launch_batch_denoise_exact(mode, frames, tiles, ntiles, grid, s);
-    if (form == KFORM_FP16 && craw_dn_geom(KFORM_FP16, mode) == KDN_GEOM_SYM)
+    if (form == KFORM_FP16 && craw_dn_geom(KFORM_FP16, mode) != KDN_GEOM_FAST)
         return craw_launch_batch_denoise_sym16(mode, frames, tiles, ntiles, grid, s);
     return form == KFORM_FP16 ? launch_batch_fast<true>(mode, frames, tiles, ntiles, grid, s)
                               : launch_batch_fast<false>(mode, frames, tiles, ntiles, grid, s);

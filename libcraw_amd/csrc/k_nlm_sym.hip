@@ -42,7 +42,7 @@
  * double reciprocal of it (one a row), rounded once to float: the
  * correctly rounded float quotient the reference computes (xs_quot).
  *
- * The precision form (F16, without history) walks the same band with the
+ * The precision form (F16) walks the same band with the
  * same pairs: each position's difference for an offset is formed in FP32
  * (two stabilised samples, both large, their difference small) and packed
  * with the next offset's into a half2; squares, the patch sums (across
@@ -50,7 +50,10 @@
  * column sums) and the weight's argument are packed FP16, the weight the
  * hardware exp2, and the sums FP32. A pair's weight is still formed once,
  * for both of its samples: half the distances and half the exps of
- * evaluating every sample's 48 neighbours.
+ * evaluating every sample's 48 neighbours (with history, half the
+ * distances: the argument is shared, each sample divides it by its own
+ * variance). The temporal average's weight then uses the hardware exp
+ * too: the form has no double arithmetic.
  */
 #include "k_common.h"
 #include "kernels.h"
@@ -183,12 +186,12 @@ KDEV float xs_quot(float a, double r)
 struct XsLane {
     float a[4], ws[4];          /* acc and wsum of output rows r .. r + 3 */
     double rv[4];               /* with history: 1 / v of rows r .. r + 3 */
+    float iv[4];                /* the same in float (precision form) */
 };
 
 template <int MODE, bool F16 = false>
 struct XsBand {
     static constexpr bool HIST = XsG<MODE>::HIST;
-    static_assert(!(F16 && HIST), "the precision form's pairwise kernel has no history mode");
     SmXs<MODE> &sm;
     const KFrame &f;
     int lane, lc, x, y0, px, py;
@@ -298,7 +301,12 @@ struct XsBand {
                 const float hv = ndx == 0 ? h :
                     __builtin_bit_cast(float, __builtin_amdgcn_ds_bpermute(
                                                   (lane + ndx) << 2, __builtin_bit_cast(int, h)));
-                const float wv = HIST ? craw_dm_expf_negmax0(xs_quot(hv, L.rv[k])) : hv;
+                /* with history the lane's own weight from the pair's
+                 * argument: exp(-A / v) (in the precision form -A log2 e
+                 * is what travels, and the weight is exp2 of it / v) */
+                const float wv = !HIST ? hv :
+                    (F16 ? __builtin_amdgcn_exp2f(hv * L.iv[k])
+                         : craw_dm_expf_negmax0(xs_quot(hv, L.rv[k])));
                 const float q = plane[r][lc + ndx];
 
                 L.ws[k] += wv;
@@ -350,8 +358,10 @@ struct XsBand {
         for (int k = 0; k < 3; k++) {
             L.a[k] = L.a[k + 1];
             L.ws[k] = L.ws[k + 1];
-            if constexpr (HIST)
+            if constexpr (HIST) {
                 L.rv[k] = L.rv[k + 1];
+                L.iv[k] = L.iv[k + 1];
+            }
         }
     }
 
@@ -389,18 +399,38 @@ struct XsBand {
                          (_Float16)(-(1.0f / 9.0f) * inv_h2 * l2e) };
         const xh2 c2 = { (_Float16)(2.0f * l2e), (_Float16)(2.0f * l2e) };
         const xh2 zero = { (_Float16)0.0f, (_Float16)0.0f };
-        float w[XS_NO];
+        float w[XS_NO], A[XS_NO];
 
+        if constexpr (HIST)
+            L.iv[3] = 1.0f / sm.m.var[r + 3][lc];
         hrow16(r + 1, hc);
 #pragma unroll
         for (int j = 0; j < XS_NH; j++) {
             xh2 d2 = (ha[j] + hb[j]) + hc[j];
 
-            d2 = __builtin_elementwise_min(__builtin_elementwise_fma(d2, kk, c2), zero);
-            w[2 * j] = __builtin_amdgcn_exp2f((float)d2.x);
-            w[2 * j + 1] = __builtin_amdgcn_exp2f((float)d2.y);
+            if constexpr (HIST) {
+                /* the argument with the two samples' variances in FP32:
+                 * t = min(d2 kd + (v_p + v_q) log2 e, 0) = -A log2 e, the
+                 * weight exp2(t / v_p) here and exp2(t / v_q) in q's lane */
+                const float kd = -(1.0f / 9.0f) * inv_h2 * l2e;
+                const float vp = sm.m.var[r][lc];
+#pragma unroll
+                for (int h = 0; h < 2; h++) {
+                    const int o = 2 * j + h;
+                    const float vq = sm.m.var[r + xs_dy(o)][lc + xs_dx(o)];
+                    const float t = __builtin_fminf(
+                        __builtin_fmaf((float)(h ? d2.y : d2.x), kd, (vp + vq) * l2e), 0.0f);
+
+                    A[o] = t;
+                    w[o] = __builtin_amdgcn_exp2f(t * L.iv[0]);
+                }
+            } else {
+                d2 = __builtin_elementwise_min(__builtin_elementwise_fma(d2, kk, c2), zero);
+                w[2 * j] = __builtin_amdgcn_exp2f((float)d2.x);
+                w[2 * j + 1] = __builtin_amdgcn_exp2f((float)d2.y);
+            }
         }
-        terms(w, w, r, L);
+        terms(w, A, r, L);
     }
 };
 
@@ -490,7 +520,11 @@ KDEV void dev_denoise_sym(SmXs<MODE> &sm, const KFrame &f, int x0, int y0, int p
                         d2 += e * e;
                     }
                 }
-                a = p.alpha_max * craw_dm_expf_negmax0(d2 * (1.0f / 9.0f) * p.inv_h2 - 2.0f);
+                if constexpr (F16)
+                    a = p.alpha_max * kf_exp(-__builtin_fmaxf(d2 * (1.0f / 9.0f) * p.inv_h2 -
+                                                              2.0f, 0.0f));
+                else
+                    a = p.alpha_max * craw_dm_expf_negmax0(d2 * (1.0f / 9.0f) * p.inv_h2 - 2.0f);
                 t = sm.f.cur[cy][cx] * (1.0f - a) + sm.f.prev[cy][cx] * a;
                 v = (1.0f - a) * (1.0f - a) +
                     a * a * xs_var_decode(st.prev_var[pl * psize + (size_t)my * w2 + mx]);
@@ -554,6 +588,12 @@ KDEV void dev_denoise_sym(SmXs<MODE> &sm, const KFrame &f, int x0, int y0, int p
         for (int k = 0; k < 4; k++) {
             L.a[k] = L.ws[k] = 0.0f;
             L.rv[k] = 0.0;
+            L.iv[k] = 0.0f;
+        }
+        if constexpr (HIST) {
+#pragma unroll
+            for (int k = 0; k < 3; k++)
+                L.iv[k] = 1.0f / sm.m.var[1 + k][lc];
         }
 #pragma unroll 1
         for (int r = 1; r <= TH + 3; r += 3) {
@@ -668,15 +708,16 @@ hipError_t craw_launch_batch_denoise_sym(int mode, const KFrame *frames, const u
  * each pair's weight formed once. */
 hipError_t craw_launch_denoise_sym16(int mode, const KFrame &f, hipStream_t s)
 {
-    const dim3 grid((f.dn.w2 + XS_TW - 1) / XS_TW, (f.dn.h2 + KG_XS_TH - 1) / KG_XS_TH, 4);
-
     switch (mode) {
-    case KDN_SPATIAL:
-        KLAUNCH((k_denoise_sym<KDN_SPATIAL, true>), grid, dim3(XS_NT), 0, s, f);
-        break;
-    case KDN_SPATIAL | KDN_TEMPORAL:
-        KLAUNCH((k_denoise_sym<KDN_SPATIAL | KDN_TEMPORAL, true>), grid, dim3(XS_NT), 0, s, f);
-        break;
+#define X(M)                                                                               \
+    case M: {                                                                              \
+        const dim3 grid((f.dn.w2 + XS_TW - 1) / XS_TW, (f.dn.h2 + XsG<M>::TH - 1) / XsG<M>::TH, \
+                        4);                                                                \
+        KLAUNCH((k_denoise_sym<M, true>), grid, dim3(XS_NT), 0, s, f);                     \
+        break;                                                                             \
+    }
+    XS_MODES(X)
+#undef X
     default:
         return hipErrorInvalidValue;
     }
@@ -689,14 +730,13 @@ hipError_t craw_launch_batch_denoise_sym16(int mode, const KFrame *frames, const
     if (!ntiles)
         return hipSuccess;
     switch (mode) {
-    case KDN_SPATIAL:
-        KLAUNCH((kb_denoise_sym<KDN_SPATIAL, true>), dim3(grid), dim3(XS_NT), 0, s, frames, tiles,
-                ntiles);
+#define X(M)                                                                               \
+    case M:                                                                                \
+        KLAUNCH((kb_denoise_sym<M, true>), dim3(grid), dim3(XS_NT), 0, s, frames, tiles,    \
+                ntiles);                                                                   \
         break;
-    case KDN_SPATIAL | KDN_TEMPORAL:
-        KLAUNCH((kb_denoise_sym<KDN_SPATIAL | KDN_TEMPORAL, true>), dim3(grid), dim3(XS_NT), 0, s,
-                frames, tiles, ntiles);
-        break;
+    XS_MODES(X)
+#undef X
     default:
         return hipErrorInvalidValue;
     }

@@ -160,9 +160,9 @@ enum KDnGeom {
 static inline int craw_dn_geom(int form, int mode)
 {
     /* the precision form's spatial modes share the exact form's pairwise
-     * band kernel, in packed FP16 */
-    if (form == KFORM_FP16 && (mode & KDN_SPATIAL) && !(mode & KDN_HISTORY))
-        return KDN_GEOM_SYM;
+     * band kernel (with and without history), in packed FP16 */
+    if (form == KFORM_FP16 && (mode & KDN_SPATIAL))
+        return (mode & KDN_HISTORY) ? KDN_GEOM_SYMH : KDN_GEOM_SYM;
     if (form != KFORM_EXACT)
         return KDN_GEOM_FAST;
     if (!(mode & KDN_SPATIAL))
