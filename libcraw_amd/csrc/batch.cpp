@@ -72,6 +72,12 @@ struct CRAWBatch {
     std::vector<uint32_t> graph_key;
     hipGraph_t graph;
     hipGraphExec_t exec;
+    /* the graph's chroma-up launches, which take a frame's constants as
+     * arguments: each node, its captured launch shape and the frame it
+     * develops; their arguments are set again before every replay */
+    std::vector<hipGraphNode_t> cu_node;
+    std::vector<hipKernelNodeParams> cu_params;
+    std::vector<uint32_t> cu_frame;
 
     CRAWProfiler prof;
 };
@@ -450,10 +456,10 @@ static int grid_for(CRAWBatch *b, int kind, size_t ntiles)
 }
 
 /* The stage launches of a planned recording, in order: with `part` 1 those
- * a graph may hold (every launch reads the recording's tables, which are
- * uploaded again before each replay), with 2 the rest (the chroma-up
- * launches, which take a frame's constants as arguments, and the
- * histogram after them), with 0 all. */
+ * that read only the recording's tables (uploaded again before each
+ * replay), with 2 the rest (the chroma-up launches, which take a frame's
+ * constants as arguments, and the histogram after them), with 0 all (what
+ * a graph holds). */
 static hipError_t batch_launch(CRAWBatch *b, const BatchPlan &bp, const size_t *list_off,
                                hipStream_t s, CRAWProfRun *pr, int part = 0)
 {
@@ -568,6 +574,57 @@ rest:
     return e;
 }
 
+/* The chroma-up launches of a captured recording: every kernel node of
+ * that kernel, matched to the frame whose constants (its one argument,
+ * copied at capture) it took. */
+static hipError_t graph_chroma_nodes(hipGraph_t g, const BatchPlan &bp,
+                                     std::vector<hipGraphNode_t> &node,
+                                     std::vector<hipKernelNodeParams> &params,
+                                     std::vector<uint32_t> &frame)
+{
+    const void *cu = craw_chroma_up_kernel();
+    std::vector<hipGraphNode_t> nodes;
+    std::vector<bool> taken(bp.plans.size(), false);
+    size_t n = 0, want = 0;
+    hipError_t e;
+
+    for (const CRAWFramePlan &pl : bp.plans)
+        want += pl.chroma ? 1 : 0;
+    if ((e = hipGraphGetNodes(g, NULL, &n)) != hipSuccess)
+        return e;
+    nodes.resize(n);
+    if (n && (e = hipGraphGetNodes(g, nodes.data(), &n)) != hipSuccess)
+        return e;
+    for (hipGraphNode_t nd : nodes) {
+        hipGraphNodeType t;
+        hipKernelNodeParams kp;
+
+        if ((e = hipGraphNodeGetType(nd, &t)) != hipSuccess)
+            return e;
+        if (t != hipGraphNodeTypeKernel)
+            continue;
+        if ((e = hipGraphKernelNodeGetParams(nd, &kp)) != hipSuccess)
+            return e;
+        if (kp.func != cu)
+            continue;
+        if (!kp.kernelParams || !kp.kernelParams[0])
+            return hipErrorInvalidValue;
+        const KFrame *a = (const KFrame *)kp.kernelParams[0];
+        size_t i = 0;
+        while (i < bp.plans.size() &&
+               (taken[i] || !bp.plans[i].chroma ||
+                memcmp(a, &bp.plans[i].f, sizeof(KFrame)) != 0))
+            i++;
+        if (i == bp.plans.size())
+            return hipErrorInvalidValue;
+        taken[i] = true;
+        node.push_back(nd);
+        params.push_back(kp);
+        frame.push_back((uint32_t)i);
+    }
+    return node.size() == want ? hipSuccess : hipErrorInvalidValue;
+}
+
 static int batch_record(CRAWBatch *b, hipStream_t s, const CRAWBatchFrame *frames, uint32_t nb,
                         bool graphs)
 {
@@ -620,13 +677,19 @@ static int batch_record(CRAWBatch *b, hipStream_t s, const CRAWBatchFrame *frame
             hipGraph_t g = NULL;
             hipGraphExec_t x = NULL;
 
+            std::vector<hipGraphNode_t> cu_node;
+            std::vector<hipKernelNodeParams> cu_params;
+            std::vector<uint32_t> cu_frame;
+
             e = hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed);
             if (e == hipSuccess) {
-                hipError_t le = batch_launch(b, bp, list_off, s, NULL, 1);
+                hipError_t le = batch_launch(b, bp, list_off, s, NULL, 0);
                 e = hipStreamEndCapture(s, &g);
                 if (le != hipSuccess)
                     e = le;
             }
+            if (e == hipSuccess)
+                e = graph_chroma_nodes(g, bp, cu_node, cu_params, cu_frame);
             if (e == hipSuccess)
                 e = hipGraphInstantiate(&x, g, NULL, NULL, 0);
             if (e != hipSuccess) {
@@ -643,19 +706,32 @@ static int batch_record(CRAWBatch *b, hipStream_t s, const CRAWBatchFrame *frame
             b->graph = g;
             b->exec = x;
             b->graph_key = bp.key;
+            b->cu_node.swap(cu_node);
+            b->cu_params.swap(cu_params);
+            b->cu_frame.swap(cu_frame);
+        }
+        /* this recording's constants into the chroma-up launches */
+        for (size_t k = 0; k < b->cu_node.size() && e == hipSuccess; k++) {
+            hipKernelNodeParams kp = b->cu_params[k];
+            KFrame arg = bp.plans[b->cu_frame[k]].f;
+            void *args[1] = { &arg };
+
+            kp.kernelParams = args;
+            kp.extra = NULL;
+            e = hipGraphExecKernelNodeSetParams(b->exec, b->cu_node[k], &kp);
+        }
+        if (e != hipSuccess) {
+            craw_log(b->ctx, CRAW_LOG_ERROR, "Graph update failed: %s\n", hipGetErrorString(e));
+            return CRAW_ERROR(EIO);
         }
         if (b->prof.on) {
             CRAWProfRun pr = craw_profiler_start(&b->prof, s);
             craw_profiler_begin(pr, "batch_graph", false);
             e = hipGraphLaunch(b->exec, s);
             craw_profiler_end(pr);
-            if (e == hipSuccess)
-                e = batch_launch(b, bp, list_off, s, &pr, 2);
             craw_profiler_finish(pr);
         } else {
             e = hipGraphLaunch(b->exec, s);
-            if (e == hipSuccess)
-                e = batch_launch(b, bp, list_off, s, NULL, 2);
         }
     } else if (b->prof.on) {
         CRAWProfRun pr = craw_profiler_start(&b->prof, s);

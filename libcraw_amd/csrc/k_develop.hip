@@ -845,6 +845,11 @@ hipError_t craw_launch_chroma_blur(const KFrame &f, hipStream_t s)
     return hipGetLastError();
 }
 
+const void *craw_chroma_up_kernel(void)
+{
+    return (const void *)k_chroma_up_finish;
+}
+
 hipError_t craw_launch_chroma_up_finish(const KFrame &f, hipStream_t s)
 {
     dim3 grid((f.W + CU_TW - 1) / CU_TW, (f.H + CU_TH - 1) / CU_TH);

@@ -349,6 +349,8 @@ hipError_t craw_launch_hist_export(uint32_t *hist, uint32_t *host, size_t n, hip
 hipError_t craw_launch_debayer(bool to_inter, const KFrame &f, hipStream_t s);
 hipError_t craw_launch_chroma_blur(const KFrame &f, hipStream_t s);
 hipError_t craw_launch_chroma_up_finish(const KFrame &f, hipStream_t s);
+/* The chroma-up kernel's host handle, as a graph's kernel nodes name it. */
+const void *craw_chroma_up_kernel(void);
 /* RCD over the interior of craw_rcd_split(), into the chroma stage or to
  * the output (the tiled launcher does the ring). */
 hipError_t craw_launch_rcd_stream(bool to_inter, const KFrame &f, hipStream_t s);
