@@ -75,15 +75,31 @@ CRAW_API int craw_batch_get_awb_histogram(CRAWBatch *batch, uint32_t index,
 CRAW_API int craw_batch_get_stage_times(CRAWBatch *batch, CRAWStageTime *out, uint32_t max);
 
 /* Picks the launch shape of each stage kind (one workgroup per tile, or a
- * persistent 1 to 16 workgroups per CU) by timing each candidate on
- * `frames` (developing them
- * `iterations` times per candidate on `stream`, synchronously), and keeps
- * the fastest for later recordings. Returns the chosen configuration's
- * time per recording in microseconds. Outputs are the same whichever
- * candidate runs. */
+ * persistent 1 to 16 workgroups per CU) and of the denoise stage (its
+ * CRAW_BATCH_DENOISE_* options) by timing each candidate on `frames`
+ * (developing them `iterations` times per candidate on `stream`,
+ * synchronously), and keeps the fastest for later recordings. Returns the
+ * chosen configuration's time per recording in microseconds. Outputs are
+ * the same whichever candidate runs. */
 CRAW_API int craw_batch_autotune(CRAWBatch *batch, CRAWStream stream,
                                  const CRAWBatchFrame *frames, uint32_t nb,
                                  uint32_t iterations);
+
+/* Denoise launch options: the pairwise band kernel (exact and precision
+ * forms, without history) in 57-row bands instead of 48; the fast form's
+ * 7 search rows unrolled. */
+#define CRAW_BATCH_DENOISE_TALL   (1 << 0)
+#define CRAW_BATCH_DENOISE_UNROLL (1 << 1)
+
+/* Launch shapes of later recordings, set directly (as the autotuner
+ * does): persistent workgroups per CU for the denoise, debayer, chroma
+ * blur and AWB histogram stages (0: one workgroup per tile; at most 64),
+ * and the CRAW_BATCH_DENOISE_* options. CRAW_ERROR(EINVAL) for a value out
+ * of range. craw_batch_get_launch_shape() reads the current ones. */
+CRAW_API int craw_batch_set_launch_shape(CRAWBatch *batch, const int wg_per_cu[4],
+                                         int denoise_options);
+CRAW_API int craw_batch_get_launch_shape(CRAWBatch *batch, int wg_per_cu[4],
+                                         int *denoise_options);
 
 /* Temporal state of one clip stream: two sets of (level, variance) for a
  * width x height mosaic, 5 bytes a sample each, on the context's device. */

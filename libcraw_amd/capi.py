@@ -42,6 +42,9 @@ CRAW_PIPELINE_FLAG_PROFILE = 1 << 9
 CRAW_PIPELINE_FLAG_NLM_FP16 = 1 << 16
 CRAW_PIPELINE_FLAG_NLM_FAST = 1 << 17
 
+CRAW_BATCH_DENOISE_TALL = 1 << 0
+CRAW_BATCH_DENOISE_UNROLL = 1 << 1
+
 CRAW_FRAME_FLAG_CLAMP_NEGATIVE = 1 << 0
 CRAW_FRAME_FLAG_NO_GAMUT_COMPRESSION = 1 << 2
 CRAW_FRAME_FLAG_HIGHLIGHT_RECONSTRUCTION = 1 << 3
@@ -283,6 +286,8 @@ SIGNATURES = [
     ("craw_batch_get_awb_histogram", c_i32, [vp, c_u32, P(CRAWAwbHistogram)]),
     ("craw_batch_get_stage_times", c_i32, [vp, P(CRAWStageTime), c_u32]),
     ("craw_batch_autotune", c_i32, [vp, vp, P(CRAWBatchFrame), c_u32, c_u32]),
+    ("craw_batch_set_launch_shape", c_i32, [vp, P(c_i32), c_i32]),
+    ("craw_batch_get_launch_shape", c_i32, [vp, P(c_i32), P(c_i32)]),
     ("craw_temporal_state_create", c_i32, [vp, c_u32, c_u32, P(vp)]),
     ("craw_temporal_state_free", None, [P(vp)]),
     ("craw_temporal_state_reset", None, [vp]),

@@ -67,6 +67,9 @@ struct CRAWBatch {
     /* persistent workgroups per CU for each list kind (0 denoise, 1
      * debayer, 2 chroma blur, 3 AWB histogram); 0: one workgroup per tile */
     int wg_per_cu[4];
+    /* the denoise launch shape (KDN_SHAPE_*): band height of the pairwise
+     * kernel, search-row unroll of the fast one */
+    int dn_shape;
 
     /* graph of the last captured shape */
     std::vector<uint32_t> graph_key;
@@ -266,6 +269,7 @@ CRAW_API int craw_batch_create(CRAWContext *ctx, const CRAWBatchParams *params, 
     b->form = b->cfg.fp16 ? KFORM_FP16 : (b->cfg.fast ? KFORM_FAST : KFORM_EXACT);
     for (int i = 0; i < 4; i++)
         b->wg_per_cu[i] = 0;
+    b->dn_shape = 0;
     if (hipSetDevice(ctx->device) != hipSuccess) {
         delete b;
         return CRAW_ERROR(ENODEV);
@@ -387,7 +391,7 @@ static int batch_plan(CRAWBatch *b, const CRAWBatchFrame *frames, uint32_t nb, B
                 if (dn_modes[m] == pl.dn_mode)
                     li = L_DN0 + m;
             n = craw_stage_tiles(KSTAGE_DENOISE, W, H, pl.f.method,
-                                 craw_dn_geom(b->form, pl.dn_mode), &ntx);
+                                 craw_dn_geom(b->form, pl.dn_mode, b->dn_shape), &ntx);
             for (int t = 0; t < n; t++)
                 bp.lists[li].push_back(KTILE(i, t));
         }
@@ -432,6 +436,7 @@ static int batch_plan(CRAWBatch *b, const CRAWBatchFrame *frames, uint32_t nb, B
     }
     for (int k = 0; k < 4; k++)
         bp.key.push_back((uint32_t)b->wg_per_cu[k]);
+    bp.key.push_back((uint32_t)b->dn_shape);
     return 0;
 }
 
@@ -481,8 +486,9 @@ static hipError_t batch_launch(CRAWBatch *b, const BatchPlan &bp, const size_t *
             continue;
         PB(b->form == KFORM_FP16 ? "denoise_nlm_fp16" : (b->form == KFORM_FAST ?
                                                             "denoise_nlm_fast" : "denoise_nlm"));
-        e = craw_launch_batch_denoise(dn_modes[m], b->form, b->frames, b->tiles + list_off[L_DN0 + m],
-                                      (uint32_t)n, grid_for(b, 0, n), s);
+        e = craw_launch_batch_denoise(dn_modes[m], b->form, b->dn_shape, b->frames,
+                                      b->tiles + list_off[L_DN0 + m], (uint32_t)n,
+                                      grid_for(b, 0, n), s);
         PE();
     }
     /* highlight reconstruction and false colour: plain launches a frame
@@ -790,26 +796,65 @@ CRAW_API int craw_batch_get_stage_times(CRAWBatch *b, CRAWStageTime *out, uint32
     return craw_profiler_times(b->ctx, &b->prof, 0, out, max);
 }
 
+static_assert(CRAW_BATCH_DENOISE_TALL == KDN_SHAPE_TALL &&
+              CRAW_BATCH_DENOISE_UNROLL == KDN_SHAPE_UNROLL, "the public options are the shapes");
+
+CRAW_API int craw_batch_set_launch_shape(CRAWBatch *b, const int wg_per_cu[4],
+                                         int denoise_options)
+{
+    if (!b || !wg_per_cu ||
+        (denoise_options & ~(CRAW_BATCH_DENOISE_TALL | CRAW_BATCH_DENOISE_UNROLL)))
+        return CRAW_ERROR(EINVAL);
+    for (int k = 0; k < 4; k++)
+        if (wg_per_cu[k] < 0 || wg_per_cu[k] > 64)
+            return CRAW_ERROR(EINVAL);
+    for (int k = 0; k < 4; k++)
+        b->wg_per_cu[k] = wg_per_cu[k];
+    b->dn_shape = denoise_options;
+    return 0;
+}
+
+CRAW_API int craw_batch_get_launch_shape(CRAWBatch *b, int wg_per_cu[4], int *denoise_options)
+{
+    if (!b || !wg_per_cu || !denoise_options)
+        return CRAW_ERROR(EINVAL);
+    for (int k = 0; k < 4; k++)
+        wg_per_cu[k] = b->wg_per_cu[k];
+    *denoise_options = b->dn_shape;
+    return 0;
+}
+
 /* Times every stage of `iterations` recordings for each candidate launch
- * shape (one workgroup per tile, or 1 to 16 persistent workgroups per CU)
- * and keeps, per stage kind, the fastest. The temporal states of the
- * frames are left as they were. */
+ * shape and keeps, per stage kind, the fastest: one workgroup per tile or
+ * 1 to 16 persistent workgroups per CU for every kind, and for the denoise
+ * stage also its band height (pairwise kernel) and search-row unroll (fast
+ * kernel), where the batch's form has that kernel. The temporal states of
+ * the frames are left as they were. */
 CRAW_API int craw_batch_autotune(CRAWBatch *b, CRAWStream stream, const CRAWBatchFrame *frames,
                                  uint32_t nb, uint32_t iterations)
 {
     static const int cand[] = { 0, 1, 2, 4, 8, 16 };
     const int ncand = (int)(sizeof(cand) / sizeof(cand[0]));
     double best[4] = { 1e30, 1e30, 1e30, 1e30 }, total = 0.0;
-    int pick[4] = { 0, 0, 0, 0 };
+    int pick[4] = { 0, 0, 0, 0 }, pick_shape = 0;
     hipStream_t s = (hipStream_t)stream;
     CRAWProfiler saved;
     std::vector<int> prev(nb), cur(nb);
+    std::vector<int> shapes;
     int ret = 0;
 
     if (!b || !frames || !nb)
         return CRAW_ERROR(EINVAL);
     if (!iterations)
         iterations = 3;
+    /* the shapes that change a launch of this form */
+    shapes.push_back(0);
+    if (b->form != KFORM_FAST)
+        shapes.push_back(KDN_SHAPE_TALL);
+    if (b->form != KFORM_EXACT)
+        shapes.push_back(KDN_SHAPE_UNROLL);
+    if (b->form == KFORM_FP16)
+        shapes.push_back(KDN_SHAPE_TALL | KDN_SHAPE_UNROLL);
     for (uint32_t i = 0; i < nb; i++) {
         prev[i] = frames[i].state ? frames[i].state->prev : -1;
         cur[i] = frames[i].state ? frames[i].state->cur : -1;
@@ -820,11 +865,13 @@ CRAW_API int craw_batch_autotune(CRAWBatch *b, CRAWStream stream, const CRAWBatc
         b->prof = saved;
         return ret;
     }
+    for (size_t sh = 0; sh < shapes.size() && ret >= 0; sh++)
     for (int c = 0; c < ncand && ret >= 0; c++) {
         double acc[4] = { 0, 0, 0, 0 };
 
         for (int k = 0; k < 4; k++)
             b->wg_per_cu[k] = cand[c];
+        b->dn_shape = shapes[sh];
         for (uint32_t it = 0; it <= iterations && ret >= 0; it++) {
             CRAWStageTime st[CRAW_MAX_STAGES];
             int n;
@@ -855,10 +902,14 @@ CRAW_API int craw_batch_autotune(CRAWBatch *b, CRAWStream stream, const CRAWBatc
                     acc[kind] += st[j].ms;
             }
         }
+        /* the other kinds' launches do not depend on the denoise shape:
+         * they are timed at shape 0 only */
         for (int k = 0; k < 4; k++)
-            if (acc[k] > 0.0 && acc[k] < best[k]) {
+            if ((k == 0 || !sh) && acc[k] > 0.0 && acc[k] < best[k]) {
                 best[k] = acc[k];
                 pick[k] = cand[c];
+                if (!k)
+                    pick_shape = shapes[sh];
             }
     }
     craw_profiler_free(&b->prof);
@@ -868,15 +919,20 @@ CRAW_API int craw_batch_autotune(CRAWBatch *b, CRAWStream stream, const CRAWBatc
             frames[i].state->prev = prev[i];
             frames[i].state->cur = cur[i];
         }
-    if (ret < 0)
+    if (ret < 0) {
+        for (int k = 0; k < 4; k++)
+            b->wg_per_cu[k] = 0;
+        b->dn_shape = 0;
         return ret;
+    }
     for (int k = 0; k < 4; k++) {
         b->wg_per_cu[k] = pick[k];
         if (best[k] < 1e30)
             total += best[k] / iterations;
     }
-    craw_log(b->ctx, CRAW_LOG_VERBOSE, "Autotune: workgroups per CU %d/%d/%d/%d\n", pick[0],
-             pick[1], pick[2], pick[3]);
+    b->dn_shape = pick_shape;
+    craw_log(b->ctx, CRAW_LOG_VERBOSE, "Autotune: workgroups per CU %d/%d/%d/%d, denoise shape "
+             "%d\n", pick[0], pick[1], pick[2], pick[3], pick_shape);
     return (int)(total * 1000.0);
 }
 

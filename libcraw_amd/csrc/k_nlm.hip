@@ -244,13 +244,15 @@ hipError_t craw_launch_denoise(int mode, const KFrame &f, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t craw_launch_batch_denoise_exact(int mode, const KFrame *frames, const uint32_t *tiles,
-                                           uint32_t ntiles, int grid, hipStream_t s)
+hipError_t craw_launch_batch_denoise_exact(int mode, int shape, const KFrame *frames,
+                                           const uint32_t *tiles, uint32_t ntiles, int grid,
+                                           hipStream_t s)
 {
     if (!ntiles)
         return hipSuccess;
     if (craw_dn_geom(KFORM_EXACT, mode) != KDN_GEOM_DIRECT)
-        return craw_launch_batch_denoise_sym(mode, frames, tiles, ntiles, grid, s);
+        return craw_launch_batch_denoise_sym(mode, frames, tiles, ntiles, grid,
+                                             (shape & KDN_SHAPE_TALL) != 0, s);
     switch (mode) {
 #define X(M)                                                                               \
     case M:                                                                                \

@@ -85,7 +85,7 @@ struct SmFt {
     float stp[HISTORY ? CH * CW : 1];
 };
 
-template <int MODE, bool FP16>
+template <int MODE, bool FP16, int UNROLL>
 KDEV void dev_denoise_fast(SmFt<MODE> &sm, const KFrame &f, int x0, int y0, int pl)
 {
     constexpr bool SPATIAL = MODE & KDN_SPATIAL;
@@ -206,7 +206,8 @@ KDEV void dev_denoise_fast(SmFt<MODE> &sm, const KFrame &f, int x0, int y0, int 
             }
         }
 
-#pragma unroll 1
+        /* the search rows: a loop, or (UNROLL 7) unrolled */
+#pragma unroll UNROLL
         for (int dy = -3; dy <= 3; dy++) {
 #pragma unroll
             for (int dx = -3; dx <= 3; dx++) {
@@ -319,10 +320,10 @@ template <int MODE, bool FP16>
 __global__ __launch_bounds__(FT_NT) void k_denoise_fast(KFrame f)
 {
     __shared__ SmFt<MODE> sm;
-    dev_denoise_fast<MODE, FP16>(sm, f, blockIdx.x * FT_TW, blockIdx.y * FT_TH, blockIdx.z);
+    dev_denoise_fast<MODE, FP16, 1>(sm, f, blockIdx.x * FT_TW, blockIdx.y * FT_TH, blockIdx.z);
 }
 
-template <int MODE, bool FP16>
+template <int MODE, bool FP16, int UNROLL>
 __global__ __launch_bounds__(FT_NT) void kb_denoise_fast(const KFrame *__restrict__ frames,
                                                          const uint32_t *__restrict__ tiles,
                                                          uint32_t ntiles)
@@ -336,7 +337,7 @@ __global__ __launch_bounds__(FT_NT) void kb_denoise_fast(const KFrame *__restric
         const int i = KTILE_IDX(d), pl = i / (ntx * nty), r = i % (ntx * nty);
 
         __syncthreads();
-        dev_denoise_fast<MODE, FP16>(sm, f, (r % ntx) * FT_TW, (r / ntx) * FT_TH, pl);
+        dev_denoise_fast<MODE, FP16, UNROLL>(sm, f, (r % ntx) * FT_TW, (r / ntx) * FT_TH, pl);
     }
 }
 
@@ -365,14 +366,19 @@ static hipError_t launch_fast(int mode, const KFrame &f, hipStream_t s)
 }
 
 template <bool FP16>
-static hipError_t launch_batch_fast(int mode, const KFrame *frames, const uint32_t *tiles,
-                                    uint32_t ntiles, int grid, hipStream_t s)
+static hipError_t launch_batch_fast(int mode, bool unroll, const KFrame *frames,
+                                    const uint32_t *tiles, uint32_t ntiles, int grid,
+                                    hipStream_t s)
 {
     switch (mode) {
 #define X(M)                                                                               \
     case M:                                                                                \
-        KLAUNCH((kb_denoise_fast<M, FP16>), dim3(grid), dim3(FT_NT), 0, s, frames, \
-                           tiles, ntiles);                                                 \
+        if (unroll)                                                                        \
+            KLAUNCH((kb_denoise_fast<M, FP16, 7>), dim3(grid), dim3(FT_NT), 0, s, frames,  \
+                    tiles, ntiles);                                                        \
+        else                                                                               \
+            KLAUNCH((kb_denoise_fast<M, FP16, 1>), dim3(grid), dim3(FT_NT), 0, s, frames,  \
+                    tiles, ntiles);                                                        \
         break;
     FT_MODES(X)
 #undef X
@@ -389,19 +395,24 @@ hipError_t craw_launch_denoise_fast(int mode, bool fp16, const KFrame &f, hipStr
     return fp16 ? launch_fast<true>(mode, f, s) : launch_fast<false>(mode, f, s);
 }
 
-hipError_t craw_launch_batch_denoise_exact(int mode, const KFrame *frames, const uint32_t *tiles,
-                                           uint32_t ntiles, int grid, hipStream_t s);
+hipError_t craw_launch_batch_denoise_exact(int mode, int shape, const KFrame *frames,
+                                           const uint32_t *tiles, uint32_t ntiles, int grid,
+                                           hipStream_t s);
 
-hipError_t craw_launch_batch_denoise(int mode, int form, const KFrame *frames,
+hipError_t craw_launch_batch_denoise(int mode, int form, int shape, const KFrame *frames,
                                      const uint32_t *tiles, uint32_t ntiles, int grid,
                                      hipStream_t s)
 {
+    const bool unroll = (shape & KDN_SHAPE_UNROLL) != 0;
+
     if (!ntiles)
         return hipSuccess;
     if (form == KFORM_EXACT)
-        return craw_launch_batch_denoise_exact(mode, frames, tiles, ntiles, grid, s);
+        return craw_launch_batch_denoise_exact(mode, shape, frames, tiles, ntiles, grid, s);
     if (form == KFORM_FP16 && craw_dn_geom(KFORM_FP16, mode) != KDN_GEOM_FAST)
-        return craw_launch_batch_denoise_sym16(mode, frames, tiles, ntiles, grid, s);
-    return form == KFORM_FP16 ? launch_batch_fast<true>(mode, frames, tiles, ntiles, grid, s)
-                              : launch_batch_fast<false>(mode, frames, tiles, ntiles, grid, s);
+        return craw_launch_batch_denoise_sym16(mode, frames, tiles, ntiles, grid,
+                                               (shape & KDN_SHAPE_TALL) != 0, s);
+    return form == KFORM_FP16
+        ? launch_batch_fast<true>(mode, unroll, frames, tiles, ntiles, grid, s)
+        : launch_batch_fast<false>(mode, unroll, frames, tiles, ntiles, grid, s);
 }

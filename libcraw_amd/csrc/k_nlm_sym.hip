@@ -77,38 +77,41 @@ typedef _Float16 xh2 __attribute__((ext_vector_type(2)));
 /* Band rows and the LDS it stages: without history the plane; with it the
  * plane and the variance, which share the LDS with the staged frames the
  * temporal average is formed from. */
-template <int MODE>
+template <int MODE, bool TALL = false>
 struct XsG {
     static constexpr bool HIST = MODE & KDN_HISTORY;
-    static constexpr int TH = HIST ? KG_XH_TH : KG_XS_TH;
+    /* TALL: the batched form's taller band without history, which its
+     * autotuner may pick (fewer band rows of halo, fewer workgroups) */
+    static constexpr int TH = HIST ? KG_XH_TH : (TALL ? KG_XS_TH_TALL : KG_XS_TH);
     static constexpr int LH = TH + 2 * XS_AY;
 };
 static_assert(XS_LW <= XS_NT, "a thread per staged column");
 
-template <int MODE, bool HIST = XsG<MODE>::HIST>
+template <int MODE, bool TALL = false, bool HIST = XsG<MODE, TALL>::HIST>
 union SmXs;
 
-template <int MODE>
-union SmXs<MODE, false> {
+template <int MODE, bool TALL>
+union SmXs<MODE, TALL, false> {
     struct {
-        float plane[XsG<MODE>::LH][XS_LW];
+        float plane[XsG<MODE, TALL>::LH][XS_LW];
     } m;
 };
 
-template <int MODE>
-union SmXs<MODE, true> {
+template <int MODE, bool TALL>
+union SmXs<MODE, TALL, true> {
     struct {
-        float plane[XsG<MODE>::LH][XS_LW];
-        float var[XsG<MODE>::LH][XS_LW];
+        float plane[XsG<MODE, TALL>::LH][XS_LW];
+        float var[XsG<MODE, TALL>::LH][XS_LW];
     } m;
     struct {
-        float cur[XsG<MODE>::LH + 2][XS_LW + 2];
-        float prev[XsG<MODE>::LH + 2][XS_LW + 2];
+        float cur[XsG<MODE, TALL>::LH + 2][XS_LW + 2];
+        float prev[XsG<MODE, TALL>::LH + 2][XS_LW + 2];
     } f;
 };
 
 static_assert(XS_TW == KG_XS_TW, "tile geometry");
-static_assert((KG_XS_TH + 3) % 3 == 0 && (KG_XH_TH + 3) % 3 == 0,
+static_assert((KG_XS_TH + 3) % 3 == 0 && (KG_XS_TH_TALL + 3) % 3 == 0 &&
+              (KG_XH_TH + 3) % 3 == 0,
               "the row loop is unrolled by three");
 
 /* Offset o (0..23) after the centre, in raster order: dy 0, dx 1..3; then
@@ -189,10 +192,10 @@ struct XsLane {
     float iv[4];                /* the same in float (precision form) */
 };
 
-template <int MODE, bool F16 = false>
+template <int MODE, bool F16 = false, bool TALL = false>
 struct XsBand {
-    static constexpr bool HIST = XsG<MODE>::HIST;
-    SmXs<MODE> &sm;
+    static constexpr bool HIST = XsG<MODE, TALL>::HIST;
+    SmXs<MODE, TALL> &sm;
     const KFrame &f;
     int lane, lc, x, y0, px, py;
     bool out_lane;
@@ -449,10 +452,10 @@ KDEV float xs_var_decode(uint8_t c)
     return q * q;
 }
 
-template <int MODE, bool F16>
-KDEV void dev_denoise_sym(SmXs<MODE> &sm, const KFrame &f, int x0, int y0, int pl)
+template <int MODE, bool F16, bool TALL>
+KDEV void dev_denoise_sym(SmXs<MODE, TALL> &sm, const KFrame &f, int x0, int y0, int pl)
 {
-    using G = XsG<MODE>;
+    using G = XsG<MODE, TALL>;
     constexpr bool TEMPORAL = MODE & KDN_TEMPORAL;
     constexpr bool HIST = G::HIST;
     constexpr int TH = G::TH, LH = G::LH;
@@ -573,7 +576,7 @@ KDEV void dev_denoise_sym(SmXs<MODE> &sm, const KFrame &f, int x0, int y0, int p
     /* LDS column of this lane; plane column x0 - 4 + 56 wave + lane */
     const int lc = XS_AX - 4 + XS_OUTW * wave + lane;
     const int x = x0 - 4 + XS_OUTW * wave + lane;
-    XsBand<MODE, F16> band = { sm, f, lane, lc, x, y0, px, py,
+    XsBand<MODE, F16, TALL> band = { sm, f, lane, lc, x, y0, px, py,
                                lane >= 4 && lane < 4 + XS_OUTW && x < w2,
                                black, range, inv_range, p.inv_h2 };
     XsLane L;
@@ -641,17 +644,17 @@ template <int MODE, bool F16>
 __global__ __launch_bounds__(XS_NT) void k_denoise_sym(KFrame f)
 {
     __shared__ SmXs<MODE> sm;
-    dev_denoise_sym<MODE, F16>(sm, f, blockIdx.x * XS_TW, blockIdx.y * XsG<MODE>::TH,
-                               blockIdx.z);
+    dev_denoise_sym<MODE, F16, false>(sm, f, blockIdx.x * XS_TW, blockIdx.y * XsG<MODE>::TH,
+                                      blockIdx.z);
 }
 
-template <int MODE, bool F16>
+template <int MODE, bool F16, bool TALL>
 __global__ __launch_bounds__(XS_NT) void kb_denoise_sym(const KFrame *__restrict__ frames,
                                                         const uint32_t *__restrict__ tiles,
                                                         uint32_t ntiles)
 {
-    constexpr int TH = XsG<MODE>::TH;
-    __shared__ SmXs<MODE> sm;
+    constexpr int TH = XsG<MODE, TALL>::TH;
+    __shared__ SmXs<MODE, TALL> sm;
 
     for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const uint32_t d = tiles[t];
@@ -660,7 +663,7 @@ __global__ __launch_bounds__(XS_NT) void kb_denoise_sym(const KFrame *__restrict
         const int i = KTILE_IDX(d), pl = i / (ntx * nty), r = i % (ntx * nty);
 
         __syncthreads();
-        dev_denoise_sym<MODE, F16>(sm, f, (r % ntx) * XS_TW, (r / ntx) * TH, pl);
+        dev_denoise_sym<MODE, F16, TALL>(sm, f, (r % ntx) * XS_TW, (r / ntx) * TH, pl);
     }
 }
 
@@ -686,15 +689,19 @@ hipError_t craw_launch_denoise_sym(int mode, const KFrame &f, hipStream_t s)
 }
 
 hipError_t craw_launch_batch_denoise_sym(int mode, const KFrame *frames, const uint32_t *tiles,
-                                         uint32_t ntiles, int grid, hipStream_t s)
+                                         uint32_t ntiles, int grid, bool tall, hipStream_t s)
 {
     if (!ntiles)
         return hipSuccess;
     switch (mode) {
 #define X(M)                                                                               \
     case M:                                                                                \
-        KLAUNCH((kb_denoise_sym<M, false>), dim3(grid), dim3(XS_NT), 0, s, frames, tiles,   \
-                ntiles);                                                                   \
+        if (tall && !((M) & KDN_HISTORY))                                                  \
+            KLAUNCH((kb_denoise_sym<M, false, true>), dim3(grid), dim3(XS_NT), 0, s, frames, \
+                    tiles, ntiles);                                                        \
+        else                                                                               \
+            KLAUNCH((kb_denoise_sym<M, false, false>), dim3(grid), dim3(XS_NT), 0, s, frames, \
+                    tiles, ntiles);                                                        \
         break;
     XS_MODES(X)
 #undef X
@@ -725,15 +732,19 @@ hipError_t craw_launch_denoise_sym16(int mode, const KFrame &f, hipStream_t s)
 }
 
 hipError_t craw_launch_batch_denoise_sym16(int mode, const KFrame *frames, const uint32_t *tiles,
-                                           uint32_t ntiles, int grid, hipStream_t s)
+                                           uint32_t ntiles, int grid, bool tall, hipStream_t s)
 {
     if (!ntiles)
         return hipSuccess;
     switch (mode) {
 #define X(M)                                                                               \
     case M:                                                                                \
-        KLAUNCH((kb_denoise_sym<M, true>), dim3(grid), dim3(XS_NT), 0, s, frames, tiles,    \
-                ntiles);                                                                   \
+        if (tall && !((M) & KDN_HISTORY))                                                  \
+            KLAUNCH((kb_denoise_sym<M, true, true>), dim3(grid), dim3(XS_NT), 0, s, frames, \
+                    tiles, ntiles);                                                        \
+        else                                                                               \
+            KLAUNCH((kb_denoise_sym<M, true, false>), dim3(grid), dim3(XS_NT), 0, s, frames, \
+                    tiles, ntiles);                                                        \
         break;
     XS_MODES(X)
 #undef X

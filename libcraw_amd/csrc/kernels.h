@@ -120,6 +120,7 @@ struct KFrame {
 #ifndef KG_XS_TH
 #define KG_XS_TH 48             /* 3 workgroups a CU in 53 KB of LDS */
 #endif
+#define KG_XS_TH_TALL 57        /* the taller band a batch's autotuner may pick */
 #ifndef KG_XH_TH
 #define KG_XH_TH 30             /* the same kernel with a temporal history */
 #endif
@@ -155,19 +156,28 @@ enum KDnGeom {
     KDN_GEOM_FAST,
     KDN_GEOM_SYM,
     KDN_GEOM_SYMH,
+    KDN_GEOM_SYM_TALL,          /* KDN_GEOM_SYM in taller bands (224x57) */
 };
 
-static inline int craw_dn_geom(int form, int mode)
+/* A batch's denoise launch shape, picked by its autotuner: the pairwise
+ * kernel's taller band without history, and the fast form's search rows
+ * unrolled (1 or all 7). */
+#define KDN_SHAPE_TALL   1
+#define KDN_SHAPE_UNROLL 2
+
+static inline int craw_dn_geom(int form, int mode, int shape = 0)
 {
+    const int sym = (shape & KDN_SHAPE_TALL) ? KDN_GEOM_SYM_TALL : KDN_GEOM_SYM;
+
     /* the precision form's spatial modes share the exact form's pairwise
      * band kernel (with and without history), in packed FP16 */
     if (form == KFORM_FP16 && (mode & KDN_SPATIAL))
-        return (mode & KDN_HISTORY) ? KDN_GEOM_SYMH : KDN_GEOM_SYM;
+        return (mode & KDN_HISTORY) ? KDN_GEOM_SYMH : sym;
     if (form != KFORM_EXACT)
         return KDN_GEOM_FAST;
     if (!(mode & KDN_SPATIAL))
         return KDN_GEOM_DIRECT;
-    return (mode & KDN_HISTORY) ? KDN_GEOM_SYMH : KDN_GEOM_SYM;
+    return (mode & KDN_HISTORY) ? KDN_GEOM_SYMH : sym;
 }
 
 /* Tile geometry of `stage`: tw x th samples of a grid `div` times coarser
@@ -192,6 +202,10 @@ static inline void craw_stage_geom(int stage, int method, int dn_geom, int *tw, 
         case KDN_GEOM_SYMH:
             *tw = KG_XS_TW;
             *th = KG_XH_TH;
+            break;
+        case KDN_GEOM_SYM_TALL:
+            *tw = KG_XS_TW;
+            *th = KG_XS_TH_TALL;
             break;
         default:
             *tw = KG_DN_TW;
@@ -405,15 +419,15 @@ hipError_t craw_launch_denoise_fast(int mode, bool fp16, const KFrame &f, hipStr
 hipError_t craw_launch_denoise_sym(int mode, const KFrame &f, hipStream_t s);
 hipError_t craw_launch_denoise_sym16(int mode, const KFrame &f, hipStream_t s);
 hipError_t craw_launch_batch_denoise_sym16(int mode, const KFrame *frames, const uint32_t *tiles,
-                                           uint32_t ntiles, int grid, hipStream_t s);
+                                           uint32_t ntiles, int grid, bool tall, hipStream_t s);
 hipError_t craw_launch_batch_denoise_sym(int mode, const KFrame *frames, const uint32_t *tiles,
-                                         uint32_t ntiles, int grid, hipStream_t s);
+                                         uint32_t ntiles, int grid, bool tall, hipStream_t s);
 
 /* Batched form: `frames` is a device table, `tiles` a device work-list of
  * `ntiles` KTILE entries for the stage, `grid` the number of persistent
  * workgroups. All frames share the method, the denoise mode and whether
  * the chroma stage runs. */
-hipError_t craw_launch_batch_denoise(int mode, int form, const KFrame *frames,
+hipError_t craw_launch_batch_denoise(int mode, int form, int shape, const KFrame *frames,
                                      const uint32_t *tiles, uint32_t ntiles, int grid,
                                      hipStream_t s);
 hipError_t craw_launch_batch_debayer(int method, bool to_inter, const KFrame *frames,

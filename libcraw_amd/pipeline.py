@@ -199,6 +199,18 @@ class Batch:
                                                  self._array(frames), len(frames), iterations),
                    "craw_batch_autotune")
 
+    def set_launch_shape(self, wg_per_cu, denoise_options=0):
+        arr = (ctypes.c_int32 * 4)(*wg_per_cu)
+        _ok(self._lib.craw_batch_set_launch_shape(self.handle, arr, denoise_options),
+            "craw_batch_set_launch_shape")
+
+    def launch_shape(self):
+        arr = (ctypes.c_int32 * 4)()
+        opt = ctypes.c_int32()
+        _ok(self._lib.craw_batch_get_launch_shape(self.handle, arr, ctypes.byref(opt)),
+            "craw_batch_get_launch_shape")
+        return list(arr), opt.value
+
     def histogram(self, index):
         h = capi.CRAWAwbHistogram()
         _ok(self._lib.craw_batch_get_awb_histogram(self.handle, index, ctypes.byref(h)),

@@ -14,8 +14,8 @@ import pytest
 
 import oracle
 from libcraw_amd import capi, hip, scene
-from libcraw_amd.pipeline import (Batch, TemporalState, batch_frame, batch_params,
-                                  frame_params, image)
+from libcraw_amd.pipeline import (Batch, CrawError, TemporalState, batch_frame,
+                                  batch_params, frame_params, image)
 
 from gpu_util import Runner, assert_bit_exact, context, half_to_float
 
@@ -278,22 +278,64 @@ def test_state_untouched_by_a_batch_without_the_pass(gpu_available):
             assert_bit_exact(fr.output(0), run.output(), "step %d" % t)
 
 
-def test_autotune_keeps_outputs(gpu_available):
+FORMS = {"exact": 0, "fp16": capi.CRAW_PIPELINE_FLAG_NLM_FP16,
+         "fast": capi.CRAW_PIPELINE_FLAG_NLM_FAST}
+DN_STAGE = {"exact": "denoise_nlm", "fp16": "denoise_nlm_fp16", "fast": "denoise_nlm_fast"}
+
+
+@pytest.mark.parametrize("form", sorted(FORMS))
+def test_autotune_keeps_outputs(gpu_available, form):
+    flags = FULL | FORMS[form]
     prof = scene.make_profile()
     fr = Frames(SIZES)
     mos = mosaics(SIZES, 500)
     fr.upload(mos)
     fps = [params_for(prof, i) for i in range(len(SIZES))]
-    b = new_batch(SIZES, FULL | capi.CRAW_PIPELINE_FLAG_PROFILE)
+    b = new_batch(SIZES, flags | capi.CRAW_PIPELINE_FLAG_PROFILE)
     stream = hip.Stream()
     frames = [batch_frame(*fr.images(i), fps[i]) for i in range(len(SIZES))]
     us = b.autotune(stream, frames, iterations=2)
     assert us > 0
+    wg, opt = b.launch_shape()
+    print("%s: autotuned workgroups per CU %s, denoise options %d" % (form, wg, opt))
     assert b.record(stream, frames) == 0
     stream.synchronize()
     names = [n for n, _, _ in b.stage_times()]
-    assert "rcd" in names and "denoise_nlm" in names
-    check_against_plain(b, fr, fps, mos, FULL, capi.CRAW_DEBAYER_RCD, "autotuned")
+    assert "rcd" in names and DN_STAGE[form] in names
+    check_against_plain(b, fr, fps, mos, flags, capi.CRAW_DEBAYER_RCD, "autotuned")
+
+
+@pytest.mark.parametrize("form", sorted(FORMS))
+def test_every_launch_shape_equals_plain(gpu_available, form):
+    """Each denoise option (taller pairwise bands, unrolled fast search)
+    with one-workgroup-per-tile and persistent grids: the same bits as the
+    plain pipeline, which has neither option."""
+    flags = FULL | FORMS[form]
+    prof = scene.make_profile()
+    fr = Frames(SIZES)
+    mos = mosaics(SIZES, 520)
+    fr.upload(mos)
+    fps = [params_for(prof, i) for i in range(len(SIZES))]
+    refs = [plain(size, mos[i], fps[i], flags) for i, size in enumerate(SIZES)]
+    b = new_batch(SIZES, flags)
+    stream = hip.Stream()
+    frames = [batch_frame(*fr.images(i), fps[i]) for i in range(len(SIZES))]
+    tall, unroll = capi.CRAW_BATCH_DENOISE_TALL, capi.CRAW_BATCH_DENOISE_UNROLL
+    for opt in (0, tall, unroll, tall | unroll):
+        for wg in ((0, 0, 0, 0), (1, 2, 4, 8), (3, 1, 1, 1)):
+            b.set_launch_shape(wg, opt)
+            assert b.launch_shape() == (list(wg), opt)
+            assert b.record(stream, frames) == 0
+            stream.synchronize()
+            for i, size in enumerate(SIZES):
+                what = "%s options %d grid %s frame %d %s" % (form, opt, wg, i, size)
+                assert_bit_exact(fr.output(i), refs[i][0], what)
+                got = b.histogram(i)
+                for a, r in zip(got, refs[i][1]):
+                    assert np.array_equal(a, r), what + ": histogram differs"
+    for bad in (((0, 0, 0, 65), 0), ((-1, 0, 0, 0), 0), ((0, 0, 0, 0), 4)):
+        with pytest.raises(CrawError):
+            b.set_launch_shape(*bad)
 
 
 def test_limits_and_errors(gpu_available):
