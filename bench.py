@@ -9,13 +9,17 @@ on one stream. The input frame is resident in HBM before timing starts.
 That is `value`.
 
 Prints one JSON line. `value` and `ms_per_step` come from hipEvents on the
-launch stream around K recordings of a pipeline without profiling. The
-same K frames are then recorded through a pipeline created with
-CRAW_PIPELINE_FLAG_PROFILE, whose per-stage hipEvents ride on the stages'
-kernel dispatches (hipExtLaunchKernel) on the same stream: `roofline` is
+launch stream around K recordings of a pipeline without profiling, after
+W untimed ones. Before them, K frames are recorded through a pipeline
+created with CRAW_PIPELINE_FLAG_PROFILE and 200 frames are each recorded
+and waited for (`latency_ms`); that work also brings the GPU's clocks to
+their sustained level before the timed region, whatever W is. The
+profiled pipeline's per-stage hipEvents ride on the stages' kernel
+dispatches (hipExtLaunchKernel) on the same stream: `roofline` is
 the dominant kernel's algorithmic bytes per launch over its average launch
 time from those events, `kernels` the same for every kernel (each against the roof it is nearer to:
-HBM bytes, or FP32 FLOP for the NLM counted on its definition), and
+HBM bytes, or FP32 FLOP for the NLM and the chroma blur counted on their
+definitions), and
 `ms_per_step_profiled` that run's frame time (the dispatch timestamps add
 a few microseconds per kernel). `roofline.traffic` is the dominant
 kernel's HBM bytes a launch (read + write) from the committed rocprofv3
@@ -77,11 +81,18 @@ ALGO_BYTES_PER_PX = {
 # arguments and exps at 5 each (scale, subtract, clamp, negate, exp), 49 x 3
 # accumulations (weight sum, product, sum) and the final division.
 NLM_FLOP_PER_SAMPLE = 441 * 2 + 49 * 8 + 49 * 5 + 49 * 3 + 1
+# The chroma stage's 9x9 joint bilateral on the quarter grid (one cell a
+# 4x4 block of pixels), per tap: three differences from the centre scaled
+# (6), their squares subtracted from the spatial term (6), the exp (5), two
+# products added to the sums and the weight sum (5).
+CHROMA_BLUR_FLOP_PER_PX = 81 * (6 + 6 + 5 + 5) / 16.0
 # Per stage: algorithmic FLOP a pixel where the stage is bound by arithmetic
-# rather than by HBM (the NLM with or without its packed-FP16 form).
+# rather than by HBM (the NLM with or without its packed-FP16 form, the
+# chroma blur).
 ALGO_FLOP_PER_PX = {
     "denoise_nlm": NLM_FLOP_PER_SAMPLE,
     "denoise_nlm_fp16": NLM_FLOP_PER_SAMPLE,
+    "chroma_blur": CHROMA_BLUR_FLOP_PER_PX,
     "denoise_nlm_fast": NLM_FLOP_PER_SAMPLE,
 }
 
@@ -100,6 +111,9 @@ def parse():
     # for hardware-counter runs: copies of a known size, to scale the
     # counters' bytes by what they read for a plain copy
     ap.add_argument("--calibrate-copy-mib", type=int, default=0)
+    # for hardware-counter runs of the precision form: the plain frame with
+    # the packed-FP16 NLM (the line then names that form and its dtype)
+    ap.add_argument("--nlm-form", choices=("exact", "fp16"), default="exact")
     return ap.parse_args()
 
 
@@ -520,7 +534,8 @@ def main():
                                              out_space=capi.CRAW_CS_ACES_AP1_LINEAR,
                                              out_format=fmt, flags=flags | extra))
 
-    pipe = make(0)
+    form_flag = capi.CRAW_PIPELINE_FLAG_NLM_FP16 if args.nlm_form == "fp16" else 0
+    pipe = make(form_flag)
     src, dst = hip.DeviceBuffer(W * H * 2), hip.DeviceBuffer(W * H * 16)
     src.upload(bayer)
     stream = hip.Stream()
@@ -548,20 +563,17 @@ def main():
         hip.synchronize()
         return e0.elapsed_ms(e1), t_enq * 1e3, time.perf_counter() - t0
 
-    # The timed region: the product as a caller runs it (no profiling).
-    gpu_ms, enq_ms, wall = timed(pipe, args.steps, args.warmup)
-    print("bench: timed region done", file=sys.stderr, flush=True)
-    ms_per_step = gpu_ms / args.steps
-    mpix = W * H / 1e6
-    value = mpix * args.steps / (gpu_ms / 1e3)
-
-    # The same K frames again through a pipeline created with
-    # CRAW_PIPELINE_FLAG_PROFILE: the library's per-stage hipEvents, bound
-    # to the stage's kernel dispatches on the launch stream, give each
-    # kernel's average launch duration (the timestamps cost the dispatches
-    # a few microseconds each, so the throughput above is measured
-    # without them).
-    pipe_prof = make(capi.CRAW_PIPELINE_FLAG_PROFILE)
+    # Before the timed region: the per-stage profile and the latency
+    # frames, which also hold the GPU busy long enough for its clocks to
+    # reach their sustained level (they ramp over the first tens of
+    # milliseconds of work), whatever the warm-up count.
+    #
+    # K frames through a pipeline created with CRAW_PIPELINE_FLAG_PROFILE:
+    # the library's per-stage hipEvents, bound to the stage's kernel
+    # dispatches on the launch stream, give each kernel's average launch
+    # duration (the timestamps cost the dispatches a few microseconds each,
+    # so the throughput is measured without them).
+    pipe_prof = make(form_flag | capi.CRAW_PIPELINE_FLAG_PROFILE)
     gpu_ms_prof, _, _ = timed(pipe_prof, args.steps, min(args.warmup, 20))
     stages = {}
     for back in range(min(args.steps, capi_slots())):
@@ -588,15 +600,26 @@ def main():
         lat.append(a.elapsed_ms(b))
     lat = np.array(lat)
 
+    # The timed region: the product as a caller runs it (no profiling):
+    # W warm-up recordings, then exactly K between the events.
+    gpu_ms, enq_ms, wall = timed(pipe, args.steps, args.warmup)
+    print("bench: timed region done", file=sys.stderr, flush=True)
+    ms_per_step = gpu_ms / args.steps
+    mpix = W * H / 1e6
+    value = mpix * args.steps / (gpu_ms / 1e3)
+
     out = {
         "metric": "developed Mpix/s, 5.8K full pipeline",
         "value": round(value, 2), "unit": "Mpix/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": "plain form, one 5760x3240 RGGB 12-bit frame per step: "
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": "f16" if form_flag else "f32", "data": "synthetic",
+        "config": {"workload": "plain form%s, one 5760x3240 RGGB 12-bit frame per step: "
                                "VST+NLM 7x7/3x3 strength 1.0, RCD, chroma denoise 0.5, AWB "
-                               "histogram, ACES AP1 + RGC, RGBA32F",
-                   "width": W, "height": H, "noise": list(NOISE), "form": "plain"},
+                               "histogram, ACES AP1 + RGC, RGBA32F"
+                               % (" with the packed-FP16 NLM" if form_flag else ""),
+                   "width": W, "height": H, "noise": list(NOISE),
+                   "form": "plain_fp16" if form_flag else "plain"},
         "wall_s": round(wall, 3),
         "host_record_ms": round(enq_ms / args.steps, 4),
         "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()},
@@ -693,7 +716,7 @@ def host_cpu():
 
 # The kernel behind each stage of the plain 5.8K pipeline, as rocprofv3 names it.
 STAGE_KERNEL = {
-    "denoise_nlm": "void k_denoise_sym<1>(KFrame)",
+    "denoise_nlm": "void k_denoise_sym<1, false>(KFrame)",
     "chroma_up_color": "k_chroma_up_finish(KFrame)",
     "chroma_blur": "k_chroma_blur(KFrame)",
 }

@@ -640,8 +640,17 @@ KDEV void dev_denoise_sym(SmXs<MODE, TALL> &sm, const KFrame &f, int x0, int y0,
     }
 }
 
+/* Waves a SIMD the kernels are compiled for (the register budget: 512 /
+ * waves VGPRs a lane; the LDS of a 48-row band allows three). The exact
+ * form at three spills 128 bytes a lane and runs 0.679 ms at 5760x3240
+ * against 0.603 at two; the precision form fits three without a spill. */
+#ifndef KG_XS_WAVES
+#define KG_XS_WAVES 2
+#endif
+
 template <int MODE, bool F16>
-__global__ __launch_bounds__(XS_NT) void k_denoise_sym(KFrame f)
+__global__ __launch_bounds__(XS_NT) __attribute__((amdgpu_waves_per_eu(
+    (MODE & KDN_HISTORY) ? 2 : (F16 ? 3 : KG_XS_WAVES)))) void k_denoise_sym(KFrame f)
 {
     __shared__ SmXs<MODE> sm;
     dev_denoise_sym<MODE, F16, false>(sm, f, blockIdx.x * XS_TW, blockIdx.y * XsG<MODE>::TH,
