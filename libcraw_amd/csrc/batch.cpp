@@ -110,6 +110,8 @@ static size_t stage_tiles_bound(int stage, const CRAWBatchParams *p, int method,
     hm = (double)p->max_height / div + 1.0;
     area = (double)p->max_pixels / ((double)div * div) + F * (wm + hm);
     n = planes * (area / ((double)tw * th) + F * (wm / tw + hm / th + 1.0));
+    if (stage == KSTAGE_DENOISE)
+        n += 15.0 * F;          /* each frame's entries rounded to 16 */
     return (size_t)n + 64;
 }
 

@@ -197,7 +197,12 @@ template <int MODE>
 __global__ __launch_bounds__(DN_NT) void k_denoise(KFrame f)
 {
     __shared__ SmDn<MODE> sm;
-    dev_denoise<MODE>(sm, f, blockIdx.x * DN_TW, blockIdx.y * DN_TH, blockIdx.z);
+    const int ntx = (f.dn.w2 + DN_TW - 1) / DN_TW, nty = (f.dn.h2 + DN_TH - 1) / DN_TH;
+    int t, pl;
+
+    if (!craw_dn_pair_tile(blockIdx.x, ntx * nty, &t, &pl))
+        return;
+    dev_denoise<MODE>(sm, f, (t % ntx) * DN_TW, (t / ntx) * DN_TH, pl);
 }
 
 /* Persistent form over a work-list of (frame, plane, tile) entries. */
@@ -212,8 +217,10 @@ __global__ __launch_bounds__(DN_NT) void kb_denoise(const KFrame *__restrict__ f
         const uint32_t d = tiles[t];
         const KFrame &f = frames[KTILE_FRAME(d)];
         const int ntx = (f.dn.w2 + DN_TW - 1) / DN_TW, nty = (f.dn.h2 + DN_TH - 1) / DN_TH;
-        const int i = KTILE_IDX(d), pl = i / (ntx * nty), r = i % (ntx * nty);
+        int r, pl;
 
+        if (!craw_dn_pair_tile(KTILE_IDX(d), ntx * nty, &r, &pl))
+            continue;
         __syncthreads();
         dev_denoise<MODE>(sm, f, (r % ntx) * DN_TW, (r / ntx) * DN_TH, pl);
     }
@@ -226,7 +233,8 @@ static_assert(DN_TW == KG_DN_TW && DN_TH == KG_DN_TH, "tile geometry");
 
 hipError_t craw_launch_denoise(int mode, const KFrame &f, hipStream_t s)
 {
-    dim3 grid((f.dn.w2 + DN_TW - 1) / DN_TW, (f.dn.h2 + DN_TH - 1) / DN_TH, 4);
+    dim3 grid(craw_dn_pair_count(((f.dn.w2 + DN_TW - 1) / DN_TW) *
+                                 ((f.dn.h2 + DN_TH - 1) / DN_TH)));
 
     if (craw_dn_geom(KFORM_EXACT, mode) != KDN_GEOM_DIRECT)
         return craw_launch_denoise_sym(mode, f, s);

@@ -320,7 +320,12 @@ template <int MODE, bool FP16>
 __global__ __launch_bounds__(FT_NT) void k_denoise_fast(KFrame f)
 {
     __shared__ SmFt<MODE> sm;
-    dev_denoise_fast<MODE, FP16, 1>(sm, f, blockIdx.x * FT_TW, blockIdx.y * FT_TH, blockIdx.z);
+    const int ntx = (f.dn.w2 + FT_TW - 1) / FT_TW, nty = (f.dn.h2 + FT_TH - 1) / FT_TH;
+    int t, pl;
+
+    if (!craw_dn_pair_tile(blockIdx.x, ntx * nty, &t, &pl))
+        return;
+    dev_denoise_fast<MODE, FP16, 1>(sm, f, (t % ntx) * FT_TW, (t / ntx) * FT_TH, pl);
 }
 
 template <int MODE, bool FP16, int UNROLL>
@@ -334,8 +339,10 @@ __global__ __launch_bounds__(FT_NT) void kb_denoise_fast(const KFrame *__restric
         const uint32_t d = tiles[t];
         const KFrame &f = frames[KTILE_FRAME(d)];
         const int ntx = (f.dn.w2 + FT_TW - 1) / FT_TW, nty = (f.dn.h2 + FT_TH - 1) / FT_TH;
-        const int i = KTILE_IDX(d), pl = i / (ntx * nty), r = i % (ntx * nty);
+        int r, pl;
 
+        if (!craw_dn_pair_tile(KTILE_IDX(d), ntx * nty, &r, &pl))
+            continue;
         __syncthreads();
         dev_denoise_fast<MODE, FP16, UNROLL>(sm, f, (r % ntx) * FT_TW, (r / ntx) * FT_TH, pl);
     }
@@ -350,7 +357,8 @@ static_assert(FT_TW == KG_FT_TW && FT_TH == KG_FT_TH, "tile geometry");
 template <bool FP16>
 static hipError_t launch_fast(int mode, const KFrame &f, hipStream_t s)
 {
-    dim3 grid((f.dn.w2 + FT_TW - 1) / FT_TW, (f.dn.h2 + FT_TH - 1) / FT_TH, 4);
+    dim3 grid(craw_dn_pair_count(((f.dn.w2 + FT_TW - 1) / FT_TW) *
+                                 ((f.dn.h2 + FT_TH - 1) / FT_TH)));
 
     switch (mode) {
 #define X(M)                                                                               \

@@ -652,9 +652,14 @@ template <int MODE, bool F16>
 __global__ __launch_bounds__(XS_NT) __attribute__((amdgpu_waves_per_eu(
     (MODE & KDN_HISTORY) ? 2 : (F16 ? 3 : KG_XS_WAVES)))) void k_denoise_sym(KFrame f)
 {
+    constexpr int TH = XsG<MODE>::TH;
     __shared__ SmXs<MODE> sm;
-    dev_denoise_sym<MODE, F16, false>(sm, f, blockIdx.x * XS_TW, blockIdx.y * XsG<MODE>::TH,
-                                      blockIdx.z);
+    const int ntx = (f.dn.w2 + XS_TW - 1) / XS_TW, nty = (f.dn.h2 + TH - 1) / TH;
+    int t, pl;
+
+    if (!craw_dn_pair_tile(blockIdx.x, ntx * nty, &t, &pl))
+        return;
+    dev_denoise_sym<MODE, F16, false>(sm, f, (t % ntx) * XS_TW, (t / ntx) * TH, pl);
 }
 
 template <int MODE, bool F16, bool TALL>
@@ -669,8 +674,10 @@ __global__ __launch_bounds__(XS_NT) void kb_denoise_sym(const KFrame *__restrict
         const uint32_t d = tiles[t];
         const KFrame &f = frames[KTILE_FRAME(d)];
         const int ntx = (f.dn.w2 + XS_TW - 1) / XS_TW, nty = (f.dn.h2 + TH - 1) / TH;
-        const int i = KTILE_IDX(d), pl = i / (ntx * nty), r = i % (ntx * nty);
+        int r, pl;
 
+        if (!craw_dn_pair_tile(KTILE_IDX(d), ntx * nty, &r, &pl))
+            continue;
         __syncthreads();
         dev_denoise_sym<MODE, F16, TALL>(sm, f, (r % ntx) * XS_TW, (r / ntx) * TH, pl);
     }
@@ -684,8 +691,8 @@ hipError_t craw_launch_denoise_sym(int mode, const KFrame &f, hipStream_t s)
     switch (mode) {
 #define X(M)                                                                               \
     case M: {                                                                              \
-        const dim3 grid((f.dn.w2 + XS_TW - 1) / XS_TW, (f.dn.h2 + XsG<M>::TH - 1) / XsG<M>::TH, \
-                        4);                                                                \
+        const dim3 grid(craw_dn_pair_count(((f.dn.w2 + XS_TW - 1) / XS_TW) *                \
+                                           ((f.dn.h2 + XsG<M>::TH - 1) / XsG<M>::TH)));    \
         KLAUNCH((k_denoise_sym<M, false>), grid, dim3(XS_NT), 0, s, f);                    \
         break;                                                                             \
     }
@@ -727,8 +734,8 @@ hipError_t craw_launch_denoise_sym16(int mode, const KFrame &f, hipStream_t s)
     switch (mode) {
 #define X(M)                                                                               \
     case M: {                                                                              \
-        const dim3 grid((f.dn.w2 + XS_TW - 1) / XS_TW, (f.dn.h2 + XsG<M>::TH - 1) / XsG<M>::TH, \
-                        4);                                                                \
+        const dim3 grid(craw_dn_pair_count(((f.dn.w2 + XS_TW - 1) / XS_TW) *                \
+                                           ((f.dn.h2 + XsG<M>::TH - 1) / XsG<M>::TH)));    \
         KLAUNCH((k_denoise_sym<M, true>), grid, dim3(XS_NT), 0, s, f);                     \
         break;                                                                             \
     }

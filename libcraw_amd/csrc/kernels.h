@@ -299,8 +299,37 @@ static inline void craw_ring_tile(const KRcdSplit &s, int i, int *tx, int *ty)
     }
 }
 
+/* The denoise stage's workgroups in plane-pair order. Planes 0 and 1 (and
+ * 2 and 3) are the two columns of the same mosaic rows, so every cache
+ * line a workgroup of one reads or writes holds its partner's samples too.
+ * Entry i = 16 g + 8 m + k (k < 8) is plane 2 py + m of tile t, where
+ * 2 t + py = 8 g + k: the two planes of a pair are entries i and i + 8,
+ * which the dispatcher sends to the same XCD (workgroups go to the eight
+ * XCDs in turn) one after the other, so the second finds the lines in
+ * that XCD's L2. A frame of nt tiles a plane has 16 ceil(nt / 4) entries;
+ * those past 2 nt pairs are empty. */
+#ifdef __HIPCC__
+__host__ __device__
+#endif
+static inline int craw_dn_pair_count(int nt)
+{
+    return 16 * ((nt + 3) / 4);
+}
+
+#ifdef __HIPCC__
+__host__ __device__
+#endif
+static inline bool craw_dn_pair_tile(int i, int nt, int *t, int *pl)
+{
+    const int u = 8 * (i >> 4) + (i & 7);
+
+    *t = u >> 1;
+    *pl = ((u & 1) << 1) | ((i >> 3) & 1);
+    return u < 2 * nt;
+}
+
 /* Number of tiles of `stage` for a frame of W x H (even W and H; the
- * quarter grid rounds up). */
+ * quarter grid rounds up; the denoise stage's in plane-pair order). */
 static inline int craw_stage_tiles(int stage, int W, int H, int method, int dn_geom, int *ntx)
 {
     int tw, th, div, planes, w, h;
@@ -326,6 +355,8 @@ static inline int craw_stage_tiles(int stage, int W, int H, int method, int dn_g
     w = (W + div - 1) / div;
     h = (H + div - 1) / div;
     *ntx = (w + tw - 1) / tw;
+    if (stage == KSTAGE_DENOISE)
+        return craw_dn_pair_count(*ntx * ((h + th - 1) / th));
     return planes * *ntx * ((h + th - 1) / th);
 }
 
