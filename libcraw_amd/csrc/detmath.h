@@ -172,9 +172,14 @@ CRAW_DM_FN float craw_dm_expf(float x)
  * that cannot fire there. */
 CRAW_DM_FN float craw_dm_expf_nonpos(float x)
 {
+#if defined(__HIP_DEVICE_COMPILE__)
+    /* the clamp in float (one max; -200 is a float), the same value */
+    return (float)craw_dm_exp_d((double)__builtin_fmaxf(x, -200.0f));
+#else
     const double xd = (double)x;
 
     return (float)craw_dm_exp_d(xd < -200.0 ? -200.0 : xd);
+#endif
 }
 
 /* exp(-max(a, 0)), the weight of the denoise filters, for any float a:

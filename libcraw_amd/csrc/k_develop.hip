@@ -439,7 +439,8 @@ KDEV void dev_chroma_blur(SmCb &sm, const KFrame &f, int qx0, int qy0)
     inv_range = 1.0f / (0.35f * level);
     inv_chroma = 1.0f / (0.35f * level);
 
-#pragma unroll 1
+    /* all 81 taps unrolled: each tap's spatial term is a constant */
+#pragma unroll
     for (int dy = -CB_R; dy <= CB_R; dy++) {
 #pragma unroll
         for (int dx = -CB_R; dx <= CB_R; dx++) {
