@@ -89,6 +89,12 @@ struct CRAWBatch {
  * w x h grid samples has at most (w/tw + 1)(h/th + 1) tiles per plane, and
  * w <= W/div + 1, h <= H/div + 1; summed over at most max_frames frames of
  * at most max_pixels pixels together. */
+/* Whether the frame's plan launches the chroma-up + colour kernel. */
+static bool has_up(const CRAWFramePlan &pl)
+{
+    return pl.chroma || pl.split_color;
+}
+
 static size_t stage_tiles_bound(int stage, const CRAWBatchParams *p, int method, int dn_geom)
 {
     int tw, th, div, planes;
@@ -204,7 +210,7 @@ static int batch_sizes(CRAWContext *ctx, const CRAWBatchParams *params, CRAWStag
         sz->down = 3 * qpx * sizeof(float);
         sz->blur = 2 * qpx * sizeof(float);
     }
-    if (cfg->has_chroma || cfg->has_fc)
+    if (craw_needs_inter_rgb(*cfg))
         sz->inter_rgb = 3 * px * sizeof(float);
     if (cfg->has_chroma && cfg->has_fc)
         sz->inter_rgb2 = 3 * px * sizeof(float);
@@ -384,7 +390,7 @@ static int batch_plan(CRAWBatch *b, const CRAWBatchFrame *frames, uint32_t nb, B
     for (uint32_t i = 0; i < nb; i++) {
         const CRAWFramePlan &pl = bp.plans[i];
         /* the debayer writes camera RGB planes for a later stage */
-        const bool inter = pl.chroma || pl.fc_passes;
+        const bool inter = pl.chroma || pl.fc_passes || pl.split_color;
         int W = pl.f.W, H = pl.f.H, ntx, n;
 
         if (pl.dn_mode) {
@@ -415,6 +421,8 @@ static int batch_plan(CRAWBatch *b, const CRAWBatchFrame *frames, uint32_t nb, B
             n = craw_stage_tiles(KSTAGE_CHROMA_BLUR, W, H, pl.f.method, 0, &ntx);
             for (int t = 0; t < n; t++)
                 bp.lists[L_CB].push_back(KTILE(i, t));
+        }
+        if (has_up(pl)) {
             n = craw_stage_tiles(KSTAGE_CHROMA_UP, W, H, pl.f.method, 0, &ntx);
             for (int t = 0; t < n; t++)
                 bp.lists[L_CU].push_back(KTILE(i, t));
@@ -427,7 +435,8 @@ static int batch_plan(CRAWBatch *b, const CRAWBatchFrame *frames, uint32_t nb, B
         bp.key.push_back((uint32_t)W);
         bp.key.push_back((uint32_t)H);
         bp.key.push_back((uint32_t)pl.dn_mode | (pl.chroma ? 0x100u : 0u) |
-                         (pl.hl ? 0x200u : 0u) | ((uint32_t)pl.fc_passes << 12));
+                         (pl.hl ? 0x200u : 0u) | (pl.split_color ? 0x400u : 0u) |
+                         ((uint32_t)pl.fc_passes << 12));
     }
     {
         size_t total = 0;
@@ -526,10 +535,18 @@ static hipError_t batch_launch(CRAWBatch *b, const BatchPlan &bp, const size_t *
     }
     for (int l = L_RS; l <= L_RS_F && e == hipSuccess; l++) {
         size_t n = bp.lists[l].size();
+        bool gamut = false;
+
         if (!n)
             continue;
+        /* the fused colour tail with the compression, if a frame there has it */
+        for (uint32_t i = 0; i < nb && l == L_RS_F; i++) {
+            const CRAWFramePlan &pl = bp.plans[i];
+            gamut = gamut || (pl.f.method == 2 && !(pl.chroma || pl.fc_passes || pl.split_color) &&
+                              (pl.f.col.flags & KCOLOR_GAMUT_COMPRESS));
+        }
         PB("rcd");
-        e = craw_launch_batch_rcd_stream(l == L_RS, b->frames, b->tiles + list_off[l],
+        e = craw_launch_batch_rcd_stream(l == L_RS, gamut, b->frames, b->tiles + list_off[l],
                                          (uint32_t)n, grid_for(b, 1, n), s);
         PE();
     }
@@ -562,11 +579,13 @@ rest:
     if (e == hipSuccess && bp.lists[L_CU].size()) {
         /* one plain launch a frame, its constants in the kernel's
          * arguments: the persistent form of this stage, reading them from
-         * memory, ran at 0.26 ms a 5.8K frame against 0.16. A graph never
-         * holds these launches: their arguments change with the frames */
+         * memory, ran at 0.26 ms a 5.8K frame against 0.16. In a graph
+         * these launches' arguments are set again before each replay (the
+         * frames' constants change); the frames whose RCD colour stage
+         * runs from its planes take the same kernel */
         PB("chroma_up_color");
         for (uint32_t i = 0; i < nb && e == hipSuccess; i++)
-            if (bp.plans[i].chroma)
+            if (has_up(bp.plans[i]))
                 e = craw_launch_chroma_up_finish(bp.plans[i].f, s);
         PE();
     }
@@ -597,7 +616,7 @@ static hipError_t graph_chroma_nodes(hipGraph_t g, const BatchPlan &bp,
     hipError_t e;
 
     for (const CRAWFramePlan &pl : bp.plans)
-        want += pl.chroma ? 1 : 0;
+        want += has_up(pl) ? 1 : 0;
     if ((e = hipGraphGetNodes(g, NULL, &n)) != hipSuccess)
         return e;
     nodes.resize(n);
@@ -620,7 +639,7 @@ static hipError_t graph_chroma_nodes(hipGraph_t g, const BatchPlan &bp,
         const KFrame *a = (const KFrame *)kp.kernelParams[0];
         size_t i = 0;
         while (i < bp.plans.size() &&
-               (taken[i] || !bp.plans[i].chroma ||
+               (taken[i] || !has_up(bp.plans[i]) ||
                 memcmp(a, &bp.plans[i].f, sizeof(KFrame)) != 0))
             i++;
         if (i == bp.plans.size())

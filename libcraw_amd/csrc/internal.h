@@ -51,10 +51,22 @@ struct CRAWStageConfig {
     bool awb, has_denoise, has_temporal, has_chroma, has_fc, has_hl, fp16, fast;
 };
 
+/* Whether a pipeline or batch of this configuration holds the camera RGB
+ * planes: for the chroma or false-colour stage, and for the RCD debayer
+ * to ACES AP1, whose colour stage with the gamut compression then runs
+ * from the planes (streamed RCD holds too many registers to run the
+ * compression's double-precision pows at speed). */
+static inline bool craw_needs_inter_rgb(const CRAWStageConfig &cfg)
+{
+    return cfg.has_chroma || cfg.has_fc ||
+           (cfg.debayer == CRAW_DEBAYER_RCD && cfg.out_space == CRAW_CS_ACES_AP1_LINEAR);
+}
+
 /* The device buffers one frame's recording uses. */
 struct CRAWFrameBuffers {
     uint16_t *clean;            /* W x H, when denoising */
-    float *inter_rgb;           /* 3 x W x H, with chroma denoise or false colour */
+    float *inter_rgb;           /* 3 x W x H, with chroma denoise or false colour, or an
+                                 * RCD debayer to ACES AP1 (craw_split_color_buffers) */
     float *inter_rgb2;          /* the debayer's planes for false colour before chroma */
     uint16_t *hl_mosaic;        /* W x H R16F, with highlight reconstruction */
     float4 *hl_levels;          /* the colour field's levels, one after another */
@@ -70,6 +82,10 @@ struct CRAWFramePlan {
     KFrame f;                   /* what the kernels read */
     int dn_mode;                /* KDN_* bits, 0: no denoise */
     bool chroma;
+    /* the RCD debayer writes its planes and the colour stage runs after it
+     * from them (the chroma-up kernel with f.inter.up = 0): with the gamut
+     * compression and no chroma or false-colour stage */
+    bool split_color;
     int fc_passes;              /* false colour passes, 0: none */
     bool hl;                    /* highlight reconstruction before the debayer */
     KHighlight hl_k;

@@ -115,6 +115,12 @@ struct KColor {
 KDEV void kgamut(const KColor &c, float &r, float &g, float &b)
 {
     const float ach = r > (g > b ? g : b) ? r : (g > b ? g : b);
+    /* the constants read once, before the loop: indexed by the loop's
+     * counter inside it they would be a scalar load and a wait a pass */
+    const float t0 = c.thr[0], t1 = c.thr[1], t2 = c.thr[2];
+    const float s0 = c.scl[0], s1 = c.scl[1], s2 = c.scl[2];
+    const float pwr = c.pwr, inv_pwr = c.inv_pwr;
+    const bool float_math = (c.flags & KCOLOR_FLOAT_MATH) != 0;
     float mag, d0, d1, d2;
 
     if (ach == 0.0f)
@@ -126,18 +132,19 @@ KDEV void kgamut(const KColor &c, float &r, float &g, float &b)
 #pragma unroll 1
     for (int i = 0; i < 3; i++) {
         const float d = i == 0 ? d0 : (i == 1 ? d1 : d2);
-        const float thr = c.thr[i], scl = c.scl[i];
+        const float thr = i == 0 ? t0 : (i == 1 ? t1 : t2);
+        const float scl = i == 0 ? s0 : (i == 1 ? s1 : s2);
         float nd, p, v;
 
         if (d < thr)
             continue;
         nd = (d - thr) / scl;
-        if (c.flags & KCOLOR_FLOAT_MATH) {
-            p = kf_pow(nd, c.pwr);
-            v = ach - (thr + scl * nd / kf_pow(1.0f + p, c.inv_pwr)) * mag;
+        if (float_math) {
+            p = kf_pow(nd, pwr);
+            v = ach - (thr + scl * nd / kf_pow(1.0f + p, inv_pwr)) * mag;
         } else {
-            p = craw_dm_powf(nd, c.pwr);
-            v = ach - (thr + scl * nd / craw_dm_powf(1.0f + p, c.inv_pwr)) * mag;
+            p = craw_dm_powf(nd, pwr);
+            v = ach - (thr + scl * nd / craw_dm_powf(1.0f + p, inv_pwr)) * mag;
         }
         if (i == 0)
             r = v;
@@ -148,6 +155,9 @@ KDEV void kgamut(const KColor &c, float &r, float &g, float &b)
     }
 }
 
+/* GAMUT false: a caller that never sees the gamut compression (its code
+ * left out). */
+template <bool GAMUT = true>
 KDEV void kcolor(const KColor &c, float cr, float cg, float cb, float o[3])
 {
     float x[3];
@@ -158,7 +168,7 @@ KDEV void kcolor(const KColor &c, float cr, float cg, float cb, float o[3])
 #pragma unroll
     for (int r = 0; r < 3; r++)
         o[r] = c.out[r][0] * x[0] + c.out[r][1] * x[1] + c.out[r][2] * x[2];
-    if (c.flags & KCOLOR_GAMUT_COMPRESS)
+    if (GAMUT && (c.flags & KCOLOR_GAMUT_COMPRESS))
         kgamut(c, o[0], o[1], o[2]);
     if (c.flags & KCOLOR_CLAMP_NEGATIVE) {
 #pragma unroll

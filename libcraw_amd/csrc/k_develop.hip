@@ -469,7 +469,8 @@ KDEV void dev_chroma_blur(SmCb &sm, const KFrame &f, int qx0, int qy0)
 
 /* lib/ref/ref_chroma.c:119-155: the difference of the filtered and the
  * unfiltered band, bilinearly up to every pixel and blended by the
- * strength, into R and B; then the histogram and the colour stage. */
+ * strength, into R and B; then the histogram and the colour stage. With
+ * f.inter.up 0, the histogram and the colour stage of the planes alone. */
 /* The quarter-grid cells a 64x16 tile's bilinear taps can reach: one
  * before and one after, clamped to the grid as the reference clamps. */
 #define CU_QW (CU_TW / 4 + 2)
@@ -490,63 +491,78 @@ KDEV void dev_chroma_up_finish(SmUp &sm, const KFrame &f, int x0, int y0)
     const size_t plane = (size_t)W * H, qplane = (size_t)it.wq * it.hq;
     const int qbx = x0 / 4 - 1, qby = y0 / 4 - 1;
 
-    /* cell (j, i) of the stage holds grid cell clamp(qbx + i, qby + j) */
-    for (int i = tid; i < 4 * CU_QH * CU_QW; i += CU_NT) {
-        const int c = i / (CU_QH * CU_QW), r = i % (CU_QH * CU_QW);
-        const int lj = r / CU_QW, li = r % CU_QW;
-        int gx = qbx + li, gy = qby + lj;
-        const KGLOBAL(float) *srcp = kglobal(c < 2 ? it.down : it.blur) + (c & 1) * qplane;
-
-        gx = gx > 0 ? (gx < it.wq - 1 ? gx : it.wq - 1) : 0;
-        gy = gy > 0 ? (gy < it.hq - 1 ? gy : it.hq - 1) : 0;
-        sm.q[c][lj][li] = srcp[(size_t)gy * it.wq + gx];
-    }
-    __syncthreads();
-
+    if (!it.up) {
+        /* the colour stage alone from the planes */
 #pragma unroll
-    for (int k = 0; k < CU_TW * CU_TH / CU_NT; k++) {
-        int p = tid + k * CU_NT, px = p % CU_TW, py = p / CU_TW;
-        int x = x0 + px, y = y0 + py;
-        float fy, fly, ty, fx, flx, tx, w00, w10, w01, w11, r, g, b, d[2];
-        int ly0, ly1, lx0, lx1;
+        for (int k = 0; k < CU_TW * CU_TH / CU_NT; k++) {
+            const int p = tid + k * CU_NT, px = p % CU_TW, py = p / CU_TW;
+            const int x = x0 + px, y = y0 + py;
 
-        if (x >= W || y >= H)
-            continue;
-        r = kglobal(it.rgb)[(size_t)y * W + x];
-        g = kglobal(it.rgb)[plane + (size_t)y * W + x];
-        b = kglobal(it.rgb)[2 * plane + (size_t)y * W + x];
-
-        /* lib/ref/ref_chroma.c:128-140: the taps' weights and (clamped)
-         * cells; the clamp is in the staged cells */
-        fy = ((float)y + 0.5f) * 0.25f - 0.5f;
-        fly = floorf(fy);
-        ty = fy - fly;
-        ly0 = (int)fly - qby;
-        ly1 = ly0 + 1;
-        fx = ((float)x + 0.5f) * 0.25f - 0.5f;
-        flx = floorf(fx);
-        tx = fx - flx;
-        lx0 = (int)flx - qbx;
-        lx1 = lx0 + 1;
-        w00 = (1.0f - tx) * (1.0f - ty);
-        w10 = tx * (1.0f - ty);
-        w01 = (1.0f - tx) * ty;
-        w11 = tx * ty;
-
-        d[0] = r - g;
-        d[1] = b - g;
-#pragma unroll
-        for (int c = 0; c < 2; c++) {
-            const float (&dn)[CU_QH][CU_QW] = sm.q[c], (&bl)[CU_QH][CU_QW] = sm.q[2 + c];
-            float lo = w00 * dn[ly0][lx0] + w10 * dn[ly0][lx1] + w01 * dn[ly1][lx0] +
-                       w11 * dn[ly1][lx1];
-            float lf = w00 * bl[ly0][lx0] + w10 * bl[ly0][lx1] + w01 * bl[ly1][lx0] +
-                       w11 * bl[ly1][lx1];
-            d[c] += strength * (lf - lo);
+            if (x >= W || y >= H)
+                continue;
+            rgbs[0][py][px] = kglobal(it.rgb)[(size_t)y * W + x];
+            rgbs[1][py][px] = kglobal(it.rgb)[plane + (size_t)y * W + x];
+            rgbs[2][py][px] = kglobal(it.rgb)[2 * plane + (size_t)y * W + x];
         }
-        rgbs[0][py][px] = g + d[0];
-        rgbs[1][py][px] = g;
-        rgbs[2][py][px] = g + d[1];
+    } else {
+        /* cell (j, i) of the stage holds grid cell clamp(qbx + i, qby + j) */
+        for (int i = tid; i < 4 * CU_QH * CU_QW; i += CU_NT) {
+            const int c = i / (CU_QH * CU_QW), r = i % (CU_QH * CU_QW);
+            const int lj = r / CU_QW, li = r % CU_QW;
+            int gx = qbx + li, gy = qby + lj;
+            const KGLOBAL(float) *srcp = kglobal(c < 2 ? it.down : it.blur) + (c & 1) * qplane;
+
+            gx = gx > 0 ? (gx < it.wq - 1 ? gx : it.wq - 1) : 0;
+            gy = gy > 0 ? (gy < it.hq - 1 ? gy : it.hq - 1) : 0;
+            sm.q[c][lj][li] = srcp[(size_t)gy * it.wq + gx];
+        }
+        __syncthreads();
+
+#pragma unroll
+        for (int k = 0; k < CU_TW * CU_TH / CU_NT; k++) {
+            int p = tid + k * CU_NT, px = p % CU_TW, py = p / CU_TW;
+            int x = x0 + px, y = y0 + py;
+            float fy, fly, ty, fx, flx, tx, w00, w10, w01, w11, r, g, b, d[2];
+            int ly0, ly1, lx0, lx1;
+
+            if (x >= W || y >= H)
+                continue;
+            r = kglobal(it.rgb)[(size_t)y * W + x];
+            g = kglobal(it.rgb)[plane + (size_t)y * W + x];
+            b = kglobal(it.rgb)[2 * plane + (size_t)y * W + x];
+
+            /* lib/ref/ref_chroma.c:128-140: the taps' weights and (clamped)
+             * cells; the clamp is in the staged cells */
+            fy = ((float)y + 0.5f) * 0.25f - 0.5f;
+            fly = floorf(fy);
+            ty = fy - fly;
+            ly0 = (int)fly - qby;
+            ly1 = ly0 + 1;
+            fx = ((float)x + 0.5f) * 0.25f - 0.5f;
+            flx = floorf(fx);
+            tx = fx - flx;
+            lx0 = (int)flx - qbx;
+            lx1 = lx0 + 1;
+            w00 = (1.0f - tx) * (1.0f - ty);
+            w10 = tx * (1.0f - ty);
+            w01 = (1.0f - tx) * ty;
+            w11 = tx * ty;
+
+            d[0] = r - g;
+            d[1] = b - g;
+#pragma unroll
+            for (int c = 0; c < 2; c++) {
+                const float (&dn)[CU_QH][CU_QW] = sm.q[c], (&bl)[CU_QH][CU_QW] = sm.q[2 + c];
+                float lo = w00 * dn[ly0][lx0] + w10 * dn[ly0][lx1] + w01 * dn[ly1][lx0] +
+                           w11 * dn[ly1][lx1];
+                float lf = w00 * bl[ly0][lx0] + w10 * bl[ly0][lx1] + w01 * bl[ly1][lx0] +
+                           w11 * bl[ly1][lx1];
+                d[c] += strength * (lf - lo);
+            }
+            rgbs[0][py][px] = g + d[0];
+            rgbs[1][py][px] = g;
+            rgbs[2][py][px] = g + d[1];
+        }
     }
     __syncthreads();
 

@@ -126,7 +126,7 @@ KDEV float rs_colour(const RsState &s)
 
 /* One step: P is the parity of the step (the newest V row is odd for
  * P = 0, the strip starting on an even row). */
-template <bool INTER, int P>
+template <bool INTER, bool GAMUT, int P>
 KDEV void rs_step(RsState &s, const KFrame &f, int vt, int ld0, bool out_row, bool out_cols,
                   int X0)
 {
@@ -210,12 +210,17 @@ KDEV void rs_step(RsState &s, const KFrame &f, int vt, int ld0, bool out_row, bo
         if (INTER)
             rs_store_inter(s.acc, f, oy, out_cols, X0, r, g, b);
         else
-            rs_finish(s.acc, f, oy, out_cols, X0, r, g, b);
+            rs_finish<GAMUT>(s.acc, f, oy, out_cols, X0, r, g, b);
     }
 }
 
-/* One strip: output columns [xs, xe), rows [ys, ye) of the interior. */
-template <bool INTER>
+/* One strip: output columns [xs, xe), rows [ys, ye) of the interior.
+ * GAMUT (to the output only): the colour stage may have the gamut
+ * compression. Its code holds registers the streamed kernel needs
+ * elsewhere, and the planner sends frames with the compression through
+ * the planes instead wherever it has them, so the kernel comes both
+ * ways. */
+template <bool INTER, bool GAMUT>
 KDEV void dev_rcd_stream(const KFrame &f, int strip)
 {
     constexpr int OUTW = INTER ? KG_RS_OUT : KG_RS_OUT_F;
@@ -259,27 +264,31 @@ KDEV void dev_rcd_stream(const KFrame &f, int strip)
         /* ys is even and nsteps is even: steps alternate parity 0, 1 */
 #pragma unroll 1
         for (int t = 0; t < nsteps; t += 2) {
-            rs_step<INTER, 0>(s, f, ys - 5 + t, ld0, t >= 10, out_cols, X0);
-            rs_step<INTER, 1>(s, f, ys - 4 + t, ld0, t + 1 >= 10, out_cols, X0);
+            rs_step<INTER, GAMUT, 0>(s, f, ys - 5 + t, ld0, t >= 10, out_cols, X0);
+            rs_step<INTER, GAMUT, 1>(s, f, ys - 4 + t, ld0, t + 1 >= 10, out_cols, X0);
         }
     }
 }
 
-template <bool INTER>
-__global__ __launch_bounds__(RS_NT) void k_rcd_stream(KFrame f)
+/* Four waves a SIMD for the fused colour tail without the compression
+ * (128 VGPRs; the compiler's own choice is 130, three waves). */
+template <bool INTER, bool GAMUT>
+__global__ __launch_bounds__(RS_NT) __attribute__((amdgpu_waves_per_eu(GAMUT ? 1 : 4)))
+void k_rcd_stream(KFrame f)
 {
-    dev_rcd_stream<INTER>(f, blockIdx.x * RS_NW + (threadIdx.x >> 6));
+    dev_rcd_stream<INTER, GAMUT>(f, blockIdx.x * RS_NW + (threadIdx.x >> 6));
 }
 
 /* Persistent form: work-list entries are groups of RS_NW strips. */
-template <bool INTER>
+template <bool INTER, bool GAMUT>
 __global__ __launch_bounds__(RS_NT) void kb_rcd_stream(const KFrame *__restrict__ frames,
                                                        const uint32_t *__restrict__ tiles,
                                                        uint32_t ntiles)
 {
     for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const uint32_t d = tiles[t];
-        dev_rcd_stream<INTER>(frames[KTILE_FRAME(d)], KTILE_IDX(d) * RS_NW + (threadIdx.x >> 6));
+        dev_rcd_stream<INTER, GAMUT>(frames[KTILE_FRAME(d)],
+                                     KTILE_IDX(d) * RS_NW + (threadIdx.x >> 6));
     }
 }
 
@@ -291,23 +300,28 @@ hipError_t craw_launch_rcd_stream(bool to_inter, const KFrame &f, hipStream_t s)
     if (!n)
         return hipSuccess;
     if (to_inter)
-        KLAUNCH(k_rcd_stream<true>, dim3(n), dim3(RS_NT), 0, s, f);
+        KLAUNCH((k_rcd_stream<true, false>), dim3(n), dim3(RS_NT), 0, s, f);
+    else if (f.col.flags & KCOLOR_GAMUT_COMPRESS)
+        KLAUNCH((k_rcd_stream<false, true>), dim3(n), dim3(RS_NT), 0, s, f);
     else
-        KLAUNCH(k_rcd_stream<false>, dim3(n), dim3(RS_NT), 0, s, f);
+        KLAUNCH((k_rcd_stream<false, false>), dim3(n), dim3(RS_NT), 0, s, f);
     return hipGetLastError();
 }
 
-hipError_t craw_launch_batch_rcd_stream(bool to_inter, const KFrame *frames,
+hipError_t craw_launch_batch_rcd_stream(bool to_inter, bool gamut, const KFrame *frames,
                                         const uint32_t *tiles, uint32_t ntiles, int grid,
                                         hipStream_t s)
 {
     if (!ntiles)
         return hipSuccess;
     if (to_inter)
-        KLAUNCH(kb_rcd_stream<true>, dim3(grid), dim3(RS_NT), 0, s, frames, tiles,
-                           ntiles);
+        KLAUNCH((kb_rcd_stream<true, false>), dim3(grid), dim3(RS_NT), 0, s, frames, tiles,
+                ntiles);
+    else if (gamut)
+        KLAUNCH((kb_rcd_stream<false, true>), dim3(grid), dim3(RS_NT), 0, s, frames, tiles,
+                ntiles);
     else
-        KLAUNCH(kb_rcd_stream<false>, dim3(grid), dim3(RS_NT), 0, s, frames, tiles,
-                           ntiles);
+        KLAUNCH((kb_rcd_stream<false, false>), dim3(grid), dim3(RS_NT), 0, s, frames, tiles,
+                ntiles);
     return hipGetLastError();
 }

@@ -56,6 +56,7 @@ template <int C> KDEV float rs_r2(const RsRow &r) { return rs_shl(r.c[C]); }
  * columns and then its three right neighbours'; lib/awb.c:249-256), its
  * sample on the block's last row, and the colour stage into the
  * destination (lib/ref/ref_color.c:10-76). */
+template <bool GAMUT>
 KDEV void rs_finish(float (&acc)[3], const KFrame &f, int oy, bool out_cols, int X0,
                     const float r[2], const float g[2], const float b[2])
 {
@@ -94,7 +95,7 @@ KDEV void rs_finish(float (&acc)[3], const KFrame &f, int oy, bool out_cols, int
         for (int c = 0; c < 2; c++) {
             float o[3];
 
-            kcolor(f.col, r[c], g[c], b[c], o);
+            kcolor<GAMUT>(f.col, r[c], g[c], b[c], o);
             kstore_rgba(f.out, X0 + c, oy, o);
         }
     }

@@ -49,6 +49,11 @@ struct KInter {
     int wq, hq;
     float *rgb_out;
     int means;
+    /* the chroma-up + colour kernel: 1 the chroma stage's correction of
+     * the planes, 0 the colour stage alone from them (the RCD debayer
+     * without the chroma stage, when its gamut compression would otherwise
+     * run inside the streamed RCD) */
+    int up;
 };
 
 /* Constants of the raw-domain denoise (lib/pipeline.c:1414-1467 and
@@ -399,7 +404,7 @@ const void *craw_chroma_up_kernel(void);
 /* RCD over the interior of craw_rcd_split(), into the chroma stage or to
  * the output (the tiled launcher does the ring). */
 hipError_t craw_launch_rcd_stream(bool to_inter, const KFrame &f, hipStream_t s);
-hipError_t craw_launch_batch_rcd_stream(bool to_inter, const KFrame *frames,
+hipError_t craw_launch_batch_rcd_stream(bool to_inter, bool gamut, const KFrame *frames,
                                         const uint32_t *tiles, uint32_t ntiles, int grid,
                                         hipStream_t s);
 

@@ -10,9 +10,11 @@
  * A recording plans the frame on the host (craw_frame_plan(), which fails
  * without side effects), then enqueues on the caller's stream: histogram
  * clear, denoise, debayer (fused with the colour stage and the AWB block
- * samples when there is no chroma denoise), the chroma bilateral and the
- * chroma/samples/colour pass, the histogram of the block samples, and the
- * histogram's copy into the next pinned readback slot.
+ * samples when there is no chroma denoise, except RCD with the gamut
+ * compression, whose colour stage runs after it from its planes), the
+ * chroma bilateral and the chroma/samples/colour pass, the histogram of
+ * the block samples, and the histogram's copy into the next pinned
+ * readback slot.
  */
 #include <stdlib.h>
 #include <string.h>
@@ -152,7 +154,7 @@ CRAW_API int craw_pipeline_create(CRAWContext *ctx, const CRAWPipelineParams *pa
             (ret = dmalloc((void **)&p->buf.inter_rgb2, 3 * W * H * sizeof(float))) < 0)
             goto fail;
     }
-    if ((p->cfg.has_chroma || p->cfg.has_fc) &&
+    if (craw_needs_inter_rgb(p->cfg) &&
         (ret = dmalloc((void **)&p->buf.inter_rgb, 3 * W * H * sizeof(float))) < 0)
         goto fail;
     if (p->cfg.has_hl) {
@@ -257,7 +259,12 @@ static int pipeline_enqueue(CRAWPipeline *p, const CRAWFramePlan *plan, hipStrea
         static const char *names[3] = { "debayer_bilinear", "debayer_mhc", "rcd" };
 
         craw_profiler_begin(pr, names[f.method]);
-        e = craw_launch_debayer(plan->chroma || plan->fc_passes, f, stream);
+        e = craw_launch_debayer(plan->chroma || plan->fc_passes || plan->split_color, f, stream);
+        craw_profiler_end(pr);
+    }
+    if (e == hipSuccess && plan->split_color) {
+        craw_profiler_begin(pr, "color");
+        e = craw_launch_chroma_up_finish(f, stream);
         craw_profiler_end(pr);
     }
     if (e == hipSuccess && plan->fc_passes) {

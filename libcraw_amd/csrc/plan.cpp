@@ -288,6 +288,7 @@ int craw_frame_plan(CRAWContext *ctx, const CRAWStageConfig &cfg, const CRAWFram
     plan->f.inter.hq = (int)((h + 3) / 4);
     plan->f.inter.rgb_out = buf.inter_rgb;
     plan->f.inter.means = 1;
+    plan->f.inter.up = 1;
 
     /* lib/pipeline.c:1820-1836: zero passes leave the frame alone, with or
      * without the stage; more than the stage has are refused */
@@ -334,6 +335,17 @@ int craw_frame_plan(CRAWContext *ctx, const CRAWStageConfig &cfg, const CRAWFram
      * chroma, histogram and highlight stages' transcendentals in float */
     if (cfg.fp16)
         plan->f.col.flags |= KCOLOR_FLOAT_MATH;
+
+    /* RCD to the output with the gamut compression: the planes, then the
+     * colour stage from them (the same values: the planes hold the floats
+     * the fused tail would colour, and the block sums keep their order) */
+    plan->split_color = cfg.debayer == CRAW_DEBAYER_RCD && !plan->chroma && !plan->fc_passes &&
+                        (plan->f.col.flags & KCOLOR_GAMUT_COMPRESS) && buf.inter_rgb;
+    if (plan->split_color) {
+        plan->f.inter.rgb_out = buf.inter_rgb;
+        plan->f.inter.means = 0;
+        plan->f.inter.up = 0;
+    }
 
     plan->f.out.dst = dst->data;
     plan->f.out.pitch = dst->pitch;

@@ -54,9 +54,10 @@ class Frames:
         return half_to_float(self.dst[i].download(np.uint16, (h, w, 4)))
 
 
-def plain(size, mosaic, fp, flags, debayer=capi.CRAW_DEBAYER_RCD, half=False):
+def plain(size, mosaic, fp, flags, debayer=capi.CRAW_DEBAYER_RCD, half=False,
+          space=capi.CRAW_CS_XYZ_D65):
     w, h = size
-    run = Runner(w, h, debayer=debayer, flags=flags, half=half)
+    run = Runner(w, h, debayer=debayer, flags=flags, half=half, out_space=space)
     assert run.run(mosaic, fp) == 0
     hist = run.pipe.histogram() if flags & capi.CRAW_PIPELINE_FLAG_AWB_HISTOGRAM else None
     return run.output(), hist
@@ -75,17 +76,19 @@ def params_for(prof, i):
 
 
 def new_batch(sizes, flags, debayer=capi.CRAW_DEBAYER_RCD, graphs=False, fmt=F32,
-              max_frames=None):
+              max_frames=None, space=capi.CRAW_CS_XYZ_D65):
     mw = max(w for w, _ in sizes)
     mh = max(h for _, h in sizes)
     bp = batch_params(max_frames or len(sizes), mw, mh, sum(w * h for w, h in sizes),
-                      debayer=debayer, out_format=fmt, flags=flags, use_graphs=graphs)
+                      debayer=debayer, out_space=space, out_format=fmt, flags=flags,
+                      use_graphs=graphs)
     return Batch(context(), bp)
 
 
-def check_against_plain(batch, frames, fps, mos, flags, debayer, what, half=False):
+def check_against_plain(batch, frames, fps, mos, flags, debayer, what, half=False,
+                        space=capi.CRAW_CS_XYZ_D65):
     for i, size in enumerate(frames.sizes):
-        ref, hist = plain(size, mos[i], fps[i], flags, debayer=debayer, half=half)
+        ref, hist = plain(size, mos[i], fps[i], flags, debayer=debayer, half=half, space=space)
         assert_bit_exact(frames.output(i), ref, "%s frame %d %s" % (what, i, size))
         if hist is not None:
             got = batch.histogram(i)
@@ -336,6 +339,36 @@ def test_every_launch_shape_equals_plain(gpu_available, form):
     for bad in (((0, 0, 0, 65), 0), ((-1, 0, 0, 0), 0), ((0, 0, 0, 0), 4)):
         with pytest.raises(CrawError):
             b.set_launch_shape(*bad)
+
+
+@pytest.mark.parametrize("graphs", [False, True])
+def test_rcd_colour_from_planes_equals_plain(gpu_available, graphs):
+    """RCD to ACES AP1: frames without chroma (strength 0) take the colour
+    stage from the debayer's planes, beside frames with chroma and frames
+    whose gamut compression is vetoed (fused colour) in one recording;
+    each equal to the plain form, over recordings whose mix changes."""
+    flags = FULL
+    prof = scene.make_profile()
+    fr = Frames(SIZES)
+    mos = mosaics(SIZES, 700)
+    fr.upload(mos)
+    b = new_batch(SIZES, flags, graphs=graphs, space=capi.CRAW_CS_ACES_AP1_LINEAR)
+    stream = hip.Stream()
+    for rec in range(3):
+        fps = []
+        for i in range(len(SIZES)):
+            kind = (i + rec) % 3
+            fps.append(frame_params(prof, denoise_strength=1.0,
+                                    denoise_chroma_strength=0.6 if kind == 0 else 0.0,
+                                    flags=capi.CRAW_FRAME_FLAG_NO_GAMUT_COMPRESSION
+                                    if kind == 2 else 0,
+                                    noise_override=(2e-4, 4e-6)))
+        assert b.record(stream, [batch_frame(*fr.images(i), fps[i])
+                                 for i in range(len(SIZES))]) == 0
+        stream.synchronize()
+        check_against_plain(b, fr, fps, mos, flags, capi.CRAW_DEBAYER_RCD,
+                            "recording %d graphs %d" % (rec, graphs),
+                            space=capi.CRAW_CS_ACES_AP1_LINEAR)
 
 
 def test_limits_and_errors(gpu_available):

@@ -123,6 +123,27 @@ def test_rcd_interior_and_ring(gpu_available, size, chroma):
            "rcd %dx%d chroma %d" % (w, h, chroma), hist=True)
 
 
+@pytest.mark.parametrize("size", [(520, 336), (64, 32), (5760, 3240)])
+def test_rcd_colour_from_planes(gpu_available, size):
+    """RCD to ACES AP1 without the chroma stage: the debayer writes its
+    planes and the colour stage with the gamut compression runs from them
+    (stage "color"); with the compression vetoed by the frame the colour
+    stage stays fused into the debayer. Both bit-exact to the oracle, and
+    a frame of each kind on one pipeline in turn."""
+    w, h = size
+    run = Runner(w, h, debayer=capi.CRAW_DEBAYER_RCD, out_space=capi.CRAW_CS_ACES_AP1_LINEAR,
+                 flags=capi.CRAW_PIPELINE_FLAG_AWB_HISTOGRAM | capi.CRAW_PIPELINE_FLAG_PROFILE)
+    prof = scene.make_profile()
+    bayer = scene.make_frame(w, h, seed=71)
+    oracle.f32d.set_threads(16)
+    for flags, stage in ((0, True), (capi.CRAW_FRAME_FLAG_NO_GAMUT_COMPRESSION, False), (0, True)):
+        _check(run, bayer, frame_params(prof, flags=flags, exposure_stops=0.25),
+               "rcd %dx%d frame flags %d" % (w, h, flags), hist=True)
+        run.stream.synchronize()
+        names = [n for n, _, _ in run.pipe.stage_times()]
+        assert ("color" in names) == stage, names
+
+
 @pytest.mark.parametrize("method", ["bilinear", "mhc"])
 @pytest.mark.parametrize("chroma", [True, False])
 @pytest.mark.parametrize("size", [(250, 98), (520, 336), (610, 298), (96, 40), (194, 50),
