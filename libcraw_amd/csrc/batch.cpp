@@ -239,6 +239,8 @@ static int batch_sizes(CRAWContext *ctx, const CRAWBatchParams *params, CRAWStag
                                        KDN_GEOM_SYM);
     sz->tiles_cap += stage_tiles_bound(KSTAGE_DENOISE, params, (int)params->debayer,
                                        KDN_GEOM_SYMH);
+    sz->tiles_cap += stage_tiles_bound(KSTAGE_DENOISE, params, (int)params->debayer,
+                                       KDN_GEOM_SYMH16);
     sz->tiles_cap += NB_LISTS * 64;
     return 0;
 }

@@ -77,41 +77,46 @@ typedef _Float16 xh2 __attribute__((ext_vector_type(2)));
 /* Band rows and the LDS it stages: without history the plane; with it the
  * plane and the variance, which share the LDS with the staged frames the
  * temporal average is formed from. */
-template <int MODE, bool TALL = false>
+template <int MODE, bool TALL = false, bool F16 = false>
 struct XsG {
     static constexpr bool HIST = MODE & KDN_HISTORY;
     /* TALL: the batched form's taller band without history, which its
-     * autotuner may pick (fewer band rows of halo, fewer workgroups) */
-    static constexpr int TH = HIST ? KG_XH_TH : (TALL ? KG_XS_TH_TALL : KG_XS_TH);
+     * autotuner may pick (fewer band rows of halo, fewer workgroups). With
+     * history the precision form's band is shorter than the exact form's:
+     * its LDS then lets three workgroups share a CU (its registers allow
+     * three waves a SIMD; the exact form's allow two) */
+    static constexpr int TH = HIST ? (F16 ? KG_XH_TH16 : KG_XH_TH)
+                                   : (TALL ? KG_XS_TH_TALL : KG_XS_TH);
     static constexpr int LH = TH + 2 * XS_AY;
 };
 static_assert(XS_LW <= XS_NT, "a thread per staged column");
 
-template <int MODE, bool TALL = false, bool HIST = XsG<MODE, TALL>::HIST>
+template <int MODE, bool TALL = false, bool F16 = false,
+          bool HIST = XsG<MODE, TALL, F16>::HIST>
 union SmXs;
 
-template <int MODE, bool TALL>
-union SmXs<MODE, TALL, false> {
+template <int MODE, bool TALL, bool F16>
+union SmXs<MODE, TALL, F16, false> {
     struct {
-        float plane[XsG<MODE, TALL>::LH][XS_LW];
+        float plane[XsG<MODE, TALL, F16>::LH][XS_LW];
     } m;
 };
 
-template <int MODE, bool TALL>
-union SmXs<MODE, TALL, true> {
+template <int MODE, bool TALL, bool F16>
+union SmXs<MODE, TALL, F16, true> {
     struct {
-        float plane[XsG<MODE, TALL>::LH][XS_LW];
-        float var[XsG<MODE, TALL>::LH][XS_LW];
+        float plane[XsG<MODE, TALL, F16>::LH][XS_LW];
+        float var[XsG<MODE, TALL, F16>::LH][XS_LW];
     } m;
     struct {
-        float cur[XsG<MODE, TALL>::LH + 2][XS_LW + 2];
-        float prev[XsG<MODE, TALL>::LH + 2][XS_LW + 2];
+        float cur[XsG<MODE, TALL, F16>::LH + 2][XS_LW + 2];
+        float prev[XsG<MODE, TALL, F16>::LH + 2][XS_LW + 2];
     } f;
 };
 
 static_assert(XS_TW == KG_XS_TW, "tile geometry");
 static_assert((KG_XS_TH + 3) % 3 == 0 && (KG_XS_TH_TALL + 3) % 3 == 0 &&
-              (KG_XH_TH + 3) % 3 == 0,
+              (KG_XH_TH + 3) % 3 == 0 && (KG_XH_TH16 + 3) % 3 == 0,
               "the row loop is unrolled by three");
 
 /* Offset o (0..23) after the centre, in raster order: dy 0, dx 1..3; then
@@ -195,7 +200,7 @@ struct XsLane {
 template <int MODE, bool F16 = false, bool TALL = false>
 struct XsBand {
     static constexpr bool HIST = XsG<MODE, TALL>::HIST;
-    SmXs<MODE, TALL> &sm;
+    SmXs<MODE, TALL, F16> &sm;
     const KFrame &f;
     int lane, lc, x, y0, px, py;
     bool out_lane;
@@ -453,9 +458,9 @@ KDEV float xs_var_decode(uint8_t c)
 }
 
 template <int MODE, bool F16, bool TALL>
-KDEV void dev_denoise_sym(SmXs<MODE, TALL> &sm, const KFrame &f, int x0, int y0, int pl)
+KDEV void dev_denoise_sym(SmXs<MODE, TALL, F16> &sm, const KFrame &f, int x0, int y0, int pl)
 {
-    using G = XsG<MODE, TALL>;
+    using G = XsG<MODE, TALL, F16>;
     constexpr bool TEMPORAL = MODE & KDN_TEMPORAL;
     constexpr bool HIST = G::HIST;
     constexpr int TH = G::TH, LH = G::LH;
@@ -647,13 +652,17 @@ KDEV void dev_denoise_sym(SmXs<MODE, TALL> &sm, const KFrame &f, int x0, int y0,
 #ifndef KG_XS_WAVES
 #define KG_XS_WAVES 2
 #endif
+#ifndef KG_XH16_WAVES
+#define KG_XH16_WAVES 3
+#endif
 
 template <int MODE, bool F16>
 __global__ __launch_bounds__(XS_NT) __attribute__((amdgpu_waves_per_eu(
-    (MODE & KDN_HISTORY) ? 2 : (F16 ? 3 : KG_XS_WAVES)))) void k_denoise_sym(KFrame f)
+    F16 ? ((MODE & KDN_HISTORY) ? KG_XH16_WAVES : 3)
+        : ((MODE & KDN_HISTORY) ? 2 : KG_XS_WAVES)))) void k_denoise_sym(KFrame f)
 {
-    constexpr int TH = XsG<MODE>::TH;
-    __shared__ SmXs<MODE> sm;
+    constexpr int TH = XsG<MODE, false, F16>::TH;
+    __shared__ SmXs<MODE, false, F16> sm;
     const int ntx = (f.dn.w2 + XS_TW - 1) / XS_TW, nty = (f.dn.h2 + TH - 1) / TH;
     int t, pl;
 
@@ -663,12 +672,13 @@ __global__ __launch_bounds__(XS_NT) __attribute__((amdgpu_waves_per_eu(
 }
 
 template <int MODE, bool F16, bool TALL>
-__global__ __launch_bounds__(XS_NT) void kb_denoise_sym(const KFrame *__restrict__ frames,
-                                                        const uint32_t *__restrict__ tiles,
-                                                        uint32_t ntiles)
+__global__ __launch_bounds__(XS_NT) __attribute__((amdgpu_waves_per_eu(
+    (F16 && !TALL) ? ((MODE & KDN_HISTORY) ? KG_XH16_WAVES : 3) : 1)))
+void kb_denoise_sym(const KFrame *__restrict__ frames, const uint32_t *__restrict__ tiles,
+                    uint32_t ntiles)
 {
-    constexpr int TH = XsG<MODE, TALL>::TH;
-    __shared__ SmXs<MODE, TALL> sm;
+    constexpr int TH = XsG<MODE, TALL, F16>::TH;
+    __shared__ SmXs<MODE, TALL, F16> sm;
 
     for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const uint32_t d = tiles[t];
@@ -735,7 +745,8 @@ hipError_t craw_launch_denoise_sym16(int mode, const KFrame &f, hipStream_t s)
 #define X(M)                                                                               \
     case M: {                                                                              \
         const dim3 grid(craw_dn_pair_count(((f.dn.w2 + XS_TW - 1) / XS_TW) *                \
-                                           ((f.dn.h2 + XsG<M>::TH - 1) / XsG<M>::TH)));    \
+                                           ((f.dn.h2 + XsG<M, false, true>::TH - 1) /      \
+                                            XsG<M, false, true>::TH)));                    \
         KLAUNCH((k_denoise_sym<M, true>), grid, dim3(XS_NT), 0, s, f);                     \
         break;                                                                             \
     }

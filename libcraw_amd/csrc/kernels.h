@@ -129,6 +129,9 @@ struct KFrame {
 #ifndef KG_XH_TH
 #define KG_XH_TH 30             /* the same kernel with a temporal history */
 #endif
+#ifndef KG_XH_TH16
+#define KG_XH_TH16 18           /* the precision form's: 3 workgroups a CU in 53 KB */
+#endif
 #define KG_RS_OUT 116           /* streamed RCD: output columns per strip */
 #define KG_RS_OUT_F 112         /* the same without the chroma stage */
 #define KG_RS_R   40            /* and rows */
@@ -161,6 +164,7 @@ enum KDnGeom {
     KDN_GEOM_FAST,
     KDN_GEOM_SYM,
     KDN_GEOM_SYMH,
+    KDN_GEOM_SYMH16,            /* KDN_GEOM_SYMH of the precision form (224x18) */
     KDN_GEOM_SYM_TALL,          /* KDN_GEOM_SYM in taller bands (224x57) */
 };
 
@@ -177,7 +181,7 @@ static inline int craw_dn_geom(int form, int mode, int shape = 0)
     /* the precision form's spatial modes share the exact form's pairwise
      * band kernel (with and without history), in packed FP16 */
     if (form == KFORM_FP16 && (mode & KDN_SPATIAL))
-        return (mode & KDN_HISTORY) ? KDN_GEOM_SYMH : sym;
+        return (mode & KDN_HISTORY) ? KDN_GEOM_SYMH16 : sym;
     if (form != KFORM_EXACT)
         return KDN_GEOM_FAST;
     if (!(mode & KDN_SPATIAL))
@@ -207,6 +211,10 @@ static inline void craw_stage_geom(int stage, int method, int dn_geom, int *tw, 
         case KDN_GEOM_SYMH:
             *tw = KG_XS_TW;
             *th = KG_XH_TH;
+            break;
+        case KDN_GEOM_SYMH16:
+            *tw = KG_XS_TW;
+            *th = KG_XH_TH16;
             break;
         case KDN_GEOM_SYM_TALL:
             *tw = KG_XS_TW;

@@ -160,6 +160,24 @@ def test_temporal_forms(gpu_available, form):
             check_bound(g[i], o, lvl, "%s frame %d" % (form, i))
 
 
+@pytest.mark.parametrize("size,nframes", [((1000, 300), 3), ((458, 140), 3), ((40, 520), 3),
+                                          ((5760, 3240), 2)])
+def test_fp16_history_bands_within_bound_of_fp64(gpu_available, size, nframes):
+    """The precision form with a temporal history runs on 224x18 bands (a
+    shorter band than the exact form's 30 rows, so that three workgroups
+    share a CU): many bands, partial last bands across and down, a plane
+    narrower than a band's reach, and the benchmarked 5760x3240 frame, each
+    frame of a moving clip within the bound."""
+    a, b = 4e-4, 1.6e-5
+    w, h = size
+    oracle.f64.set_threads(16)
+    sq = max(8, min(w, h, 400) // 4 * 2)
+    frames = [scene.clip_frame(w, h, i, square=sq, speed=4, a=a, b=b) for i in range(nframes)]
+    g = gpu_sequence("fp16", frames, 1.0, 0.8, a, b)
+    for i, (o, lvl) in enumerate(fp64_levels(frames, 1.0, 0.8, a, b)):
+        check_bound(g[i], o, lvl, "fp16 history %dx%d frame %d" % (w, h, i))
+
+
 @pytest.mark.parametrize("size,strength", [((912, 272), 1.0), ((1000, 300), 0.6), ((458, 140), 1.0),
                                            ((40, 520), 1.0), ((5760, 3240), 1.0)])
 def test_fp16_pairwise_bands_within_bound_of_fp64(gpu_available, size, strength):
