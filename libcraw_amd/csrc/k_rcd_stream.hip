@@ -125,8 +125,12 @@ KDEV float rs_colour(const RsState &s)
 }
 
 /* One step: P is the parity of the step (the newest V row is odd for
- * P = 0, the strip starting on an even row). */
-template <bool INTER, bool GAMUT, int P>
+ * P = 0, the strip starting on an even row). L: the stages that step
+ * computes, from its first ten steps (whose rows of the later stages no
+ * output reaches): 0 the V row, 1 and the low pass, 2 and the gradients,
+ * 3 and the weights and green, 4 and the colour estimate, 5 and the
+ * output. */
+template <bool INTER, bool GAMUT, int P, int L = 5>
 KDEV void rs_step(RsState &s, const KFrame &f, int vt, int ld0, bool out_row, bool out_cols,
                   int X0)
 {
@@ -142,7 +146,7 @@ KDEV void rs_step(RsState &s, const KFrame &f, int vt, int ld0, bool out_row, bo
     /* low pass, row vt - 1 */
     rs_shift(s.lp);
 #pragma unroll
-    for (int c = 0; c < 2; c++) {
+    for (int c = 0; c < 2 && L >= 1; c++) {
         const RsRow *v = s.v;
         const float w_ = c == 0 ? rs_shr(v[5].c[1]) : v[5].c[0];
         const float e_ = c == 0 ? v[5].c[1] : rs_shl(v[5].c[0]);
@@ -160,27 +164,35 @@ KDEV void rs_step(RsState &s, const KFrame &f, int vt, int ld0, bool out_row, bo
     rs_shift(s.g1);
     rs_shift(s.g2);
     rs_shift(s.g3);
-    rs_gradients<0>(s);
-    rs_gradients<1>(s);
+    if constexpr (L >= 2) {
+        rs_gradients<0>(s);
+        rs_gradients<1>(s);
+    }
 
     /* direction weights, row vt - 3 */
     rs_shift(s.wv);
     rs_shift(s.wp);
-    rs_weights<0>(s);
-    rs_weights<1>(s);
+    if constexpr (L >= 3) {
+        rs_weights<0>(s);
+        rs_weights<1>(s);
+    }
 
     /* green, row vt - 3: the red/blue site is column P there */
     rs_shift(s.G);
-    s.G[3].c[P] = rs_green<P>(s);
-    s.G[3].c[1 - P] = s.v[3].c[1 - P];
+    if constexpr (L >= 3) {
+        s.G[3].c[P] = rs_green<P>(s);
+        s.G[3].c[1 - P] = s.v[3].c[1 - P];
+    }
 
     /* colour estimate, row vt - 4: red/blue column 1 - P */
     rs_shift(s.C);
-    s.C[2].c[1 - P] = rs_colour<1 - P>(s);
-    s.C[2].c[P] = 0.0f;
+    if constexpr (L >= 4) {
+        s.C[2].c[1 - P] = rs_colour<1 - P>(s);
+        s.C[2].c[P] = 0.0f;
+    }
 
     /* output, row vt - 5: green column 1 - P, red/blue column P */
-    if (out_row) {
+    if (L >= 5 && out_row) {
         constexpr int CG = 1 - P, CN = P;
         const int oy = vt - 5;
         float r[2], g[2], b[2];
@@ -261,11 +273,26 @@ KDEV void dev_rcd_stream(const KFrame &f, int strip)
         s.wp[0].c[0] = s.wp[0].c[1] = s.wp[1].c[0] = s.wp[1].c[1] = 0.0f;
         s.acc[0] = s.acc[1] = s.acc[2] = 0.0f;
 
-        /* ys is even and nsteps is even: steps alternate parity 0, 1 */
+        /* ys is even and nsteps is even: steps alternate parity 0, 1. The
+         * first ten steps fill the stages: V rows from ys - 5, the low pass
+         * from row ys - 4 (step 2), the gradients from ys - 3 (step 4), the
+         * weights and green from ys - 2 (step 6), the colour estimate from
+         * ys - 1 (step 8): the first rows output row ys reaches through
+         * them. A stage's rows before those are left as they were. */
+        rs_step<INTER, GAMUT, 0, 0>(s, f, ys - 5, ld0, false, out_cols, X0);
+        rs_step<INTER, GAMUT, 1, 0>(s, f, ys - 4, ld0, false, out_cols, X0);
+        rs_step<INTER, GAMUT, 0, 1>(s, f, ys - 3, ld0, false, out_cols, X0);
+        rs_step<INTER, GAMUT, 1, 1>(s, f, ys - 2, ld0, false, out_cols, X0);
+        rs_step<INTER, GAMUT, 0, 2>(s, f, ys - 1, ld0, false, out_cols, X0);
+        rs_step<INTER, GAMUT, 1, 2>(s, f, ys, ld0, false, out_cols, X0);
+        rs_step<INTER, GAMUT, 0, 3>(s, f, ys + 1, ld0, false, out_cols, X0);
+        rs_step<INTER, GAMUT, 1, 3>(s, f, ys + 2, ld0, false, out_cols, X0);
+        rs_step<INTER, GAMUT, 0, 4>(s, f, ys + 3, ld0, false, out_cols, X0);
+        rs_step<INTER, GAMUT, 1, 4>(s, f, ys + 4, ld0, false, out_cols, X0);
 #pragma unroll 1
-        for (int t = 0; t < nsteps; t += 2) {
-            rs_step<INTER, GAMUT, 0>(s, f, ys - 5 + t, ld0, t >= 10, out_cols, X0);
-            rs_step<INTER, GAMUT, 1>(s, f, ys - 4 + t, ld0, t + 1 >= 10, out_cols, X0);
+        for (int t = 10; t < nsteps; t += 2) {
+            rs_step<INTER, GAMUT, 0>(s, f, ys - 5 + t, ld0, true, out_cols, X0);
+            rs_step<INTER, GAMUT, 1>(s, f, ys - 4 + t, ld0, true, out_cols, X0);
         }
     }
 }
