@@ -478,7 +478,7 @@ KDEV void dev_chroma_blur(SmCb &sm, const KFrame &f, int qx0, int qy0)
 
 struct SmUp {
     float rgbs[3][CU_TH][CU_TW];
-    float q[4][CU_QH][CU_QW];     /* down R-G, down B-G, blur R-G, blur B-G */
+    float4 q[CU_QH][CU_QW];       /* a cell's down R-G, down B-G, blur R-G, blur B-G */
 };
 
 KDEV void dev_chroma_up_finish(SmUp &sm, const KFrame &f, int x0, int y0)
@@ -505,16 +505,18 @@ KDEV void dev_chroma_up_finish(SmUp &sm, const KFrame &f, int x0, int y0)
             rgbs[2][py][px] = kglobal(it.rgb)[2 * plane + (size_t)y * W + x];
         }
     } else {
-        /* cell (j, i) of the stage holds grid cell clamp(qbx + i, qby + j) */
-        for (int i = tid; i < 4 * CU_QH * CU_QW; i += CU_NT) {
-            const int c = i / (CU_QH * CU_QW), r = i % (CU_QH * CU_QW);
-            const int lj = r / CU_QW, li = r % CU_QW;
+        /* cell (j, i) of the stage holds grid cell clamp(qbx + i, qby + j),
+         * its four planes side by side (one 16-byte read a tap) */
+        for (int i = tid; i < CU_QH * CU_QW; i += CU_NT) {
+            const int lj = i / CU_QW, li = i % CU_QW;
             int gx = qbx + li, gy = qby + lj;
-            const KGLOBAL(float) *srcp = kglobal(c < 2 ? it.down : it.blur) + (c & 1) * qplane;
+            size_t q;
 
             gx = gx > 0 ? (gx < it.wq - 1 ? gx : it.wq - 1) : 0;
             gy = gy > 0 ? (gy < it.hq - 1 ? gy : it.hq - 1) : 0;
-            sm.q[c][lj][li] = srcp[(size_t)gy * it.wq + gx];
+            q = (size_t)gy * it.wq + gx;
+            sm.q[lj][li] = make_float4(kglobal(it.down)[q], kglobal(it.down)[qplane + q],
+                                       kglobal(it.blur)[q], kglobal(it.blur)[qplane + q]);
         }
         __syncthreads();
 
@@ -550,14 +552,16 @@ KDEV void dev_chroma_up_finish(SmUp &sm, const KFrame &f, int x0, int y0)
 
             d[0] = r - g;
             d[1] = b - g;
-#pragma unroll
-            for (int c = 0; c < 2; c++) {
-                const float (&dn)[CU_QH][CU_QW] = sm.q[c], (&bl)[CU_QH][CU_QW] = sm.q[2 + c];
-                float lo = w00 * dn[ly0][lx0] + w10 * dn[ly0][lx1] + w01 * dn[ly1][lx0] +
-                           w11 * dn[ly1][lx1];
-                float lf = w00 * bl[ly0][lx0] + w10 * bl[ly0][lx1] + w01 * bl[ly1][lx0] +
-                           w11 * bl[ly1][lx1];
-                d[c] += strength * (lf - lo);
+            {
+                const float4 t00 = sm.q[ly0][lx0], t01 = sm.q[ly0][lx1];
+                const float4 t10 = sm.q[ly1][lx0], t11 = sm.q[ly1][lx1];
+                const float lo0 = w00 * t00.x + w10 * t01.x + w01 * t10.x + w11 * t11.x;
+                const float lf0 = w00 * t00.z + w10 * t01.z + w01 * t10.z + w11 * t11.z;
+                const float lo1 = w00 * t00.y + w10 * t01.y + w01 * t10.y + w11 * t11.y;
+                const float lf1 = w00 * t00.w + w10 * t01.w + w01 * t10.w + w11 * t11.w;
+
+                d[0] += strength * (lf0 - lo0);
+                d[1] += strength * (lf1 - lo1);
             }
             rgbs[0][py][px] = g + d[0];
             rgbs[1][py][px] = g;
