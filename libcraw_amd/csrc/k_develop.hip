@@ -397,7 +397,7 @@ KDEV void dev_debayer_rcd_tile(SmRcd &sm, const KFrame &f, int x0, int y0)
 #define CB_LH (CB_TY + 2 * CB_R)
 
 struct SmCb {
-    float q[3][CB_LH][CB_LW];
+    float4 q[CB_LH][CB_LW];     /* a cell's R-G, B-G, G (w unused): one read a tap */
 };
 
 template <bool FLOAT_MATH>
@@ -424,17 +424,20 @@ KDEV void dev_chroma_blur(SmCb &sm, const KFrame &f, int qx0, int qy0)
         sx = sx < 0 ? -sx : (sx >= wq ? 2 * wq - 2 - sx : sx);
         sy = sy < 0 ? -sy : (sy >= hq ? 2 * hq - 2 - sy : sy);
         si = (size_t)sy * wq + sx;
-        sm.q[0][sy0][sx0] = it.down[si];
-        sm.q[1][sy0][sx0] = it.down[qplane + si];
-        sm.q[2][sy0][sx0] = it.down[2 * qplane + si];
+        sm.q[sy0][sx0] = make_float4(it.down[si], it.down[qplane + si],
+                                     it.down[2 * qplane + si], 0.0f);
     }
     __syncthreads();
 
     if (qx >= wq || qy >= hq)
         return;
-    c0 = sm.q[0][ly + CB_R][lx + CB_R];
-    c1 = sm.q[1][ly + CB_R][lx + CB_R];
-    c2 = sm.q[2][ly + CB_R][lx + CB_R];
+    {
+        const float4 c = sm.q[ly + CB_R][lx + CB_R];
+
+        c0 = c.x;
+        c1 = c.y;
+        c2 = c.z;
+    }
     level = (c2 > 0.0f ? c2 : 0.0f) + 0.02f;
     inv_range = 1.0f / (0.35f * level);
     inv_chroma = 1.0f / (0.35f * level);
@@ -444,9 +447,8 @@ KDEV void dev_chroma_blur(SmCb &sm, const KFrame &f, int qx0, int qy0)
     for (int dy = -CB_R; dy <= CB_R; dy++) {
 #pragma unroll
         for (int dx = -CB_R; dx <= CB_R; dx++) {
-            const float s0 = sm.q[0][ly + CB_R + dy][lx + CB_R + dx];
-            const float s1 = sm.q[1][ly + CB_R + dy][lx + CB_R + dx];
-            const float s2 = sm.q[2][ly + CB_R + dy][lx + CB_R + dx];
+            const float4 t = sm.q[ly + CB_R + dy][lx + CB_R + dx];
+            const float s0 = t.x, s1 = t.y, s2 = t.z;
             const float dg = (s2 - c2) * inv_range;
             const float dd0 = (s0 - c0) * inv_chroma;
             const float dd1 = (s1 - c1) * inv_chroma;
