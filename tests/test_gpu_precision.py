@@ -62,8 +62,9 @@ def test_float_stages_against_fp64(gpu_available, size):
     assert moved <= max(1, total // 10000)
 
 
-def test_whole_form_against_fp64(gpu_available):
-    w, h = 1032, 776
+@pytest.mark.parametrize("size", [(1032, 776), (5760, 3240)])
+def test_whole_form_against_fp64(gpu_available, size):
+    w, h = size
     flags = (FP16 | capi.CRAW_PIPELINE_FLAG_DENOISE | capi.CRAW_PIPELINE_FLAG_CHROMA_DENOISE |
              capi.CRAW_PIPELINE_FLAG_AWB_HISTOGRAM)
     run = Runner(w, h, debayer=capi.CRAW_DEBAYER_RCD, out_space=capi.CRAW_CS_XYZ_D65,
