@@ -116,6 +116,68 @@ CRAW_DM_FN double craw_dm_exp_d(double x)
 #endif
 }
 
+/* exp(x) in double through a table of 2^(j/N), for -200 <= x <= 0: the
+ * kernels' form of craw_dm_exp_d, whose float rounding it equals (below).
+ *
+ * x = m ln2/N + r with m the integer nearest x N/ln2 (the 1.5 * 2^52 shift
+ * again: t's low word is m), |r| <= ln2/(2N) by the same two-part
+ * Cody-Waite reduction (m has at most 19 bits, so m times the 32-bit head
+ * is exact); exp(x) = 2^(m >> log2 N) * T[m mod N] * exp(r), with exp(r) - 1
+ * its Taylor polynomial, degree 4 for N = 256 (truncation 2^-54.6) and
+ * degree 3 for N = 1024 (2^-50.7). T is the table of detmath_tab.h, 2^(j/N)
+ * rounded to double; on the GPU a copy of it in LDS, indexed by the low bits
+ * of m. Over every float argument in [-200, 0] the float rounding of either
+ * result equals that of craw_dm_exp_d, i.e. the correctly rounded exp
+ * (checked exhaustively, both table sizes). Against craw_dm_exp_d's 13
+ * FMAs it spends 7 (N = 256) or 6 (N = 1024) double operations and an LDS
+ * read. */
+CRAW_DM_FN double craw_dm_exp_t256_d(double x, const double *tab)
+{
+    const double shift = 0x1.8p52;
+    double t = __builtin_fma(x, 0x1.71547652b82fep+8, shift);      /* 256 / ln2 */
+    double m = t - shift;
+    double r = __builtin_fma(-m, 0x1.62e42fee00000p-9, x);         /* ln2/256 head */
+    uint32_t lo;
+    double T, c;
+
+    r = __builtin_fma(-m, 0x1.a39ef35793c76p-41, r);               /* and tail */
+    lo = (uint32_t)craw_dm_bits(t);
+    T = tab[lo & 255u];
+    c = __builtin_fma(r, 0x1.5555555555555p-5, 0x1.5555555555555p-3);
+    c = __builtin_fma(c, r, 0.5);
+    c = __builtin_fma(c, r, 1.0);
+    c = c * r;
+    T = __builtin_fma(T, c, T);
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_ldexp(T, (int32_t)lo >> 8);
+#else
+    return T * craw_dm_from_bits((uint64_t)(uint32_t)(((int32_t)lo >> 8) + 1023) << 52);
+#endif
+}
+
+CRAW_DM_FN double craw_dm_exp_t1024_d(double x, const double *tab)
+{
+    const double shift = 0x1.8p52;
+    double t = __builtin_fma(x, 0x1.71547652b82fep+10, shift);     /* 1024 / ln2 */
+    double m = t - shift;
+    double r = __builtin_fma(-m, 0x1.62e42fee00000p-11, x);        /* ln2/1024 head */
+    uint32_t lo;
+    double T, c;
+
+    r = __builtin_fma(-m, 0x1.a39ef35793c76p-43, r);               /* and tail */
+    lo = (uint32_t)craw_dm_bits(t);
+    T = tab[lo & 1023u];
+    c = __builtin_fma(r, 0x1.5555555555555p-3, 0.5);
+    c = __builtin_fma(c, r, 1.0);
+    c = c * r;
+    T = __builtin_fma(T, c, T);
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_ldexp(T, (int32_t)lo >> 10);
+#else
+    return T * craw_dm_from_bits((uint64_t)(uint32_t)(((int32_t)lo >> 10) + 1023) << 52);
+#endif
+}
+
 /* log(x) in double, for finite x > 0 (double normal). */
 CRAW_DM_FN double craw_dm_log_d(double x)
 {
@@ -196,6 +258,42 @@ CRAW_DM_FN float craw_dm_expf_negmax0(float a)
 #endif
 
     return (float)craw_dm_exp_d(-(double)c);
+}
+
+/* craw_dm_expf_negmax0 and craw_dm_expf_nonpos through the tables: the
+ * same float for every argument (the clamps are theirs, and over the
+ * clamped range [-200, 0] the table forms round like craw_dm_exp_d). */
+CRAW_DM_FN float craw_dm_expf_negmax0_t256(float a, const double *tab)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    const float c = __builtin_fminf(__builtin_fmaxf(a, 0.0f), 200.0f);
+#else
+    const float c = a > 0.0f ? (a < 200.0f ? a : 200.0f) : 0.0f;
+#endif
+
+    return (float)craw_dm_exp_t256_d(-(double)c, tab);
+}
+
+CRAW_DM_FN float craw_dm_expf_negmax0_t1024(float a, const double *tab)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    const float c = __builtin_fminf(__builtin_fmaxf(a, 0.0f), 200.0f);
+#else
+    const float c = a > 0.0f ? (a < 200.0f ? a : 200.0f) : 0.0f;
+#endif
+
+    return (float)craw_dm_exp_t1024_d(-(double)c, tab);
+}
+
+CRAW_DM_FN float craw_dm_expf_nonpos_t1024(float x, const double *tab)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return (float)craw_dm_exp_t1024_d((double)__builtin_fmaxf(x, -200.0f), tab);
+#else
+    const double xd = (double)x;
+
+    return (float)craw_dm_exp_t1024_d(xd < -200.0 ? -200.0 : xd, tab);
+#endif
 }
 
 /* logf: NaN for NaN and negatives, -inf at zero, +inf at +inf. Float

@@ -57,6 +57,7 @@
  */
 #include "k_common.h"
 #include "kernels.h"
+#include "detmath_tab.h"
 
 #define XS_NW    4                     /* waves per workgroup */
 #define XS_NT    (64 * XS_NW)
@@ -88,6 +89,12 @@ struct XsG {
     static constexpr int TH = HIST ? (F16 ? KG_XH_TH16 : KG_XH_TH)
                                    : (TALL ? KG_XS_TH_TALL : KG_XS_TH);
     static constexpr int LH = TH + 2 * XS_AY;
+    /* Entries of the exp table the exact form stages in LDS (the precision
+     * form uses the hardware exp). With history none: its 48 exps a row
+     * with their table reads in flight took the kernel from 220 VGPRs to
+     * 256 and a spill. */
+    static constexpr int TABN = (F16 || HIST) ? 0 : 1024;
+    static constexpr int TABS = TABN ? TABN : 1;
 };
 static_assert(XS_LW <= XS_NT, "a thread per staged column");
 
@@ -169,10 +176,24 @@ KDEV float xs_igat(const KDenoise &p, float t)
     return (t - p.gat_t0) * p.gat_sc;
 }
 
-/* The weight the reference gives a pair at patch distance d2. */
-KDEV float xs_weight(float d2, float inv_h2)
+/* exp(-max(a, 0)) as the oracle's craw_dm_expf_negmax0 rounds it, through
+ * the LDS copy of the table of N entries. */
+template <int N>
+KDEV float xs_exp(float a, const double *tab)
 {
-    return craw_dm_expf_negmax0(d2 * (1.0f / 9.0f) * inv_h2 - 2.0f);
+    if constexpr (N == 1024)
+        return craw_dm_expf_negmax0_t1024(a, tab);
+    else if constexpr (N == 256)
+        return craw_dm_expf_negmax0_t256(a, tab);
+    else
+        return craw_dm_expf_negmax0(a);
+}
+
+/* The weight the reference gives a pair at patch distance d2. */
+template <int N>
+KDEV float xs_weight(float d2, float inv_h2, const double *tab)
+{
+    return xs_exp<N>(d2 * (1.0f / 9.0f) * inv_h2 - 2.0f, tab);
 }
 
 /* A / v rounded to float, for A >= 0 finite and v > 0, given r = 1 / v
@@ -205,6 +226,8 @@ struct XsBand {
     int lane, lc, x, y0, px, py;
     bool out_lane;
     float black, range, inv_range, inv_h2;
+    const double *tab;                  /* the exp table in LDS */
+    static constexpr int TABN = XsG<MODE, TALL, F16>::TABN;
 
     /* Weight row r (LDS row r): fills sc with the squared differences of
      * row r + 1, forms the pair weights of row r from the squared
@@ -275,9 +298,9 @@ struct XsBand {
                 const float arg = d2 * (1.0f / 9.0f) * inv_h2 - (vp + vq);
 
                 A[o] = __builtin_fminf(__builtin_fmaxf(arg, 0.0f), 3.40282347e38f);
-                w[o] = craw_dm_expf_negmax0(xs_quot(A[o], L.rv[0]));
+                w[o] = xs_exp<TABN>(xs_quot(A[o], L.rv[0]), tab);
             } else {
-                w[o] = xs_weight(d2, inv_h2);
+                w[o] = xs_weight<TABN>(d2, inv_h2, tab);
             }
         }
 
@@ -314,7 +337,7 @@ struct XsBand {
                  * is what travels, and the weight is exp2 of it / v) */
                 const float wv = !HIST ? hv :
                     (F16 ? __builtin_amdgcn_exp2f(hv * L.iv[k])
-                         : craw_dm_expf_negmax0(xs_quot(hv, L.rv[k])));
+                         : xs_exp<TABN>(xs_quot(hv, L.rv[k]), tab));
                 const float q = plane[r][lc + ndx];
 
                 L.ws[k] += wv;
@@ -458,7 +481,8 @@ KDEV float xs_var_decode(uint8_t c)
 }
 
 template <int MODE, bool F16, bool TALL>
-KDEV void dev_denoise_sym(SmXs<MODE, TALL, F16> &sm, const KFrame &f, int x0, int y0, int pl)
+KDEV void dev_denoise_sym(SmXs<MODE, TALL, F16> &sm, double *tab, const KFrame &f, int x0,
+                          int y0, int pl)
 {
     using G = XsG<MODE, TALL, F16>;
     constexpr bool TEMPORAL = MODE & KDN_TEMPORAL;
@@ -478,6 +502,17 @@ KDEV void dev_denoise_sym(SmXs<MODE, TALL, F16> &sm, const KFrame &f, int x0, in
     const size_t psize = (size_t)w2 * h2;
     const bool interior = x0 >= XS_AX && y0 >= XS_AY && x0 + XS_TW + XS_AX <= w2 &&
                           y0 + TH + XS_AY <= h2;
+
+    /* the exact form's exp table into LDS; the staging's barrier below
+     * comes before its first read */
+    if constexpr (G::TABN == 1024) {
+#pragma unroll
+        for (int i = 0; i < 1024 / XS_NT; i++)
+            tab[tid + i * XS_NT] = craw_dm_exp2_tab1024[tid + i * XS_NT];
+    } else if constexpr (G::TABN == 256) {
+        static_assert(XS_NT == 256, "a thread per entry");
+        tab[tid] = craw_dm_exp2_tab256[tid];
+    }
 
 #define RAW(X, Y) ((float)src[(size_t)(2 * (Y) + py) * p.src_pitch + 2 * (X) + px] *          \
                    (1.0f / 65535.0f))
@@ -532,7 +567,8 @@ KDEV void dev_denoise_sym(SmXs<MODE, TALL, F16> &sm, const KFrame &f, int x0, in
                     a = p.alpha_max * kf_exp(-__builtin_fmaxf(d2 * (1.0f / 9.0f) * p.inv_h2 -
                                                               2.0f, 0.0f));
                 else
-                    a = p.alpha_max * craw_dm_expf_negmax0(d2 * (1.0f / 9.0f) * p.inv_h2 - 2.0f);
+                    a = p.alpha_max * xs_exp<G::TABN>(d2 * (1.0f / 9.0f) * p.inv_h2 - 2.0f,
+                                                      tab);
                 t = sm.f.cur[cy][cx] * (1.0f - a) + sm.f.prev[cy][cx] * a;
                 v = (1.0f - a) * (1.0f - a) +
                     a * a * xs_var_decode(st.prev_var[pl * psize + (size_t)my * w2 + mx]);
@@ -583,7 +619,7 @@ KDEV void dev_denoise_sym(SmXs<MODE, TALL, F16> &sm, const KFrame &f, int x0, in
     const int x = x0 - 4 + XS_OUTW * wave + lane;
     XsBand<MODE, F16, TALL> band = { sm, f, lane, lc, x, y0, px, py,
                                lane >= 4 && lane < 4 + XS_OUTW && x < w2,
-                               black, range, inv_range, p.inv_h2 };
+                               black, range, inv_range, p.inv_h2, tab };
     XsLane L;
 
     if constexpr (F16) {
@@ -663,12 +699,13 @@ __global__ __launch_bounds__(XS_NT) __attribute__((amdgpu_waves_per_eu(
 {
     constexpr int TH = XsG<MODE, false, F16>::TH;
     __shared__ SmXs<MODE, false, F16> sm;
+    __shared__ double tab[XsG<MODE, false, F16>::TABS];
     const int ntx = (f.dn.w2 + XS_TW - 1) / XS_TW, nty = (f.dn.h2 + TH - 1) / TH;
     int t, pl;
 
     if (!craw_dn_pair_tile(blockIdx.x, ntx * nty, &t, &pl))
         return;
-    dev_denoise_sym<MODE, F16, false>(sm, f, (t % ntx) * XS_TW, (t / ntx) * TH, pl);
+    dev_denoise_sym<MODE, F16, false>(sm, tab, f, (t % ntx) * XS_TW, (t / ntx) * TH, pl);
 }
 
 template <int MODE, bool F16, bool TALL>
@@ -679,6 +716,7 @@ void kb_denoise_sym(const KFrame *__restrict__ frames, const uint32_t *__restric
 {
     constexpr int TH = XsG<MODE, TALL, F16>::TH;
     __shared__ SmXs<MODE, TALL, F16> sm;
+    __shared__ double tab[XsG<MODE, TALL, F16>::TABS];
 
     for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const uint32_t d = tiles[t];
@@ -689,7 +727,7 @@ void kb_denoise_sym(const KFrame *__restrict__ frames, const uint32_t *__restric
         if (!craw_dn_pair_tile(KTILE_IDX(d), ntx * nty, &r, &pl))
             continue;
         __syncthreads();
-        dev_denoise_sym<MODE, F16, TALL>(sm, f, (r % ntx) * XS_TW, (r / ntx) * TH, pl);
+        dev_denoise_sym<MODE, F16, TALL>(sm, tab, f, (r % ntx) * XS_TW, (r / ntx) * TH, pl);
     }
 }
 
