@@ -47,6 +47,13 @@ KDEV float ksel4(int s, float a0, float a1, float a2, float a3)
     return (s & 2) ? hi : lo;
 }
 
+KDEV double ksel4d(int s, double a0, double a1, double a2, double a3)
+{
+    double lo = (s & 1) ? a1 : a0;
+    double hi = (s & 1) ? a3 : a2;
+    return (s & 2) ? hi : lo;
+}
+
 /* Black/white normalisation, white balance and clip of one sample,
  * as lib/ref/ref_debayer.c:42-63 computes it: divide by the
  * range (white - black, formed on the host in float, exactly as the
@@ -58,16 +65,55 @@ struct KPrep {
     float black[4];
     float range[4];
     float wb[4];      /* per quad site: R, G, G, B gains */
+    double rrange[4]; /* 1 / range[s] in double, correctly rounded */
     int half;
+    int fdiv;         /* divide in float (the precision form: no double arithmetic) */
+    /* The frame's black levels, ranges and gains bound the debayer's
+     * quotients to operands kdiv_rn divides exactly (prep_kdiv_ok in
+     * plan.cpp). */
+    int kdiv;
 };
 
+/* n / d correctly rounded, for d in [2^-40, 2^40] and n +0 or in
+ * magnitude in [2^-80, 2^40], where neither operand nor the quotient is
+ * near the ends of the float range (a -0 dividend gives +0, the division
+ * -0): a reciprocal estimate refined once by
+ * Newton's iteration, then the quotient corrected twice by its exact
+ * residual (Markstein, IBM J. Res. Dev. 34(1), 1990). These are the steps
+ * of the compiler's correctly rounded float division without its scaling
+ * and fix-up instructions (v_div_scale, v_div_fmas, v_div_fixup), which
+ * leave such operands unchanged; eight instructions against eleven.
+ * Checked against the division on the GPU over operands across that range
+ * (tests/test_gpu_kdiv.py). */
+KDEV float kdiv_rn(float n, float d)
+{
+    float y = __builtin_amdgcn_rcpf(d);
+    const float e = __builtin_fmaf(-d, y, 1.0f);
+    float q, r;
+
+    y = __builtin_fmaf(e, y, y);
+    q = n * y;
+    r = __builtin_fmaf(-d, q, n);
+    q = __builtin_fmaf(r, y, q);
+    r = __builtin_fmaf(-d, q, n);
+    return __builtin_fmaf(r, y, q);
+}
+
+/* The division of x = raw / 65535 - black by the range is x times the
+ * range's reciprocal in double, rounded once to float: the product is
+ * within 2^-52 (relative) of x / range, and a quotient of two floats lies
+ * at least 2^-49 from every midpoint of the float grid, so its rounding is
+ * the correctly rounded float quotient, the float division's value (three
+ * instructions against eleven). The precision form divides in float. */
 KDEV float kprep(uint32_t raw, int s, const KPrep &p)
 {
     if (p.half)
         return __half2float(__ushort_as_half((unsigned short)raw));
     float r = (float)raw * (1.0f / 65535.0f);
-    float t = (r - ksel4(s, p.black[0], p.black[1], p.black[2], p.black[3])) /
-              ksel4(s, p.range[0], p.range[1], p.range[2], p.range[3]);
+    const float x = r - ksel4(s, p.black[0], p.black[1], p.black[2], p.black[3]);
+    float t = p.fdiv ? x / ksel4(s, p.range[0], p.range[1], p.range[2], p.range[3])
+                     : (float)((double)x * ksel4d(s, p.rrange[0], p.rrange[1], p.rrange[2],
+                                                  p.rrange[3]));
     t *= ksel4(s, p.wb[0], p.wb[1], p.wb[2], p.wb[3]);
     return t < 1.0f ? t : 1.0f;
 }

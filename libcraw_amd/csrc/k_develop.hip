@@ -22,6 +22,7 @@
 #include "k_common.h"
 #include "kernels.h"
 #include "k_finish.h"
+#include "detmath_tab.h"
 
 /* -------------------------------------------------- bilinear and MHC */
 
@@ -391,13 +392,16 @@ KDEV void dev_debayer_rcd_tile(SmRcd &sm, const KFrame &f, int x0, int y0)
 /* lib/ref/ref_chroma.c:86-117: 9x9 joint bilateral on the quarter grid,
  * guided by G, one reflection at the borders, summed row by row. The
  * tile's three quarter-grid planes are staged in LDS with the 4-sample
- * reach. */
+ * reach. The exact form's tap weights go through the table-driven exp
+ * (detmath.h craw_dm_expf_nonpos_t1024: the same float as the oracle's
+ * craw_dm_expf_nonpos for every argument), its table staged in LDS too. */
 #define CB_R 4
 #define CB_LW (CB_TX + 2 * CB_R)
 #define CB_LH (CB_TY + 2 * CB_R)
 
 struct SmCb {
     float4 q[CB_LH][CB_LW];     /* a cell's R-G, B-G, G (w unused): one read a tap */
+    double tab[1024];           /* 2^(j/1024), the exact form's exp table */
 };
 
 template <bool FLOAT_MATH>
@@ -412,6 +416,12 @@ KDEV void dev_chroma_blur(SmCb &sm, const KFrame &f, int qx0, int qy0)
     float c0, c1, c2, level, inv_range, inv_chroma;
     float acc0 = 0.0f, acc1 = 0.0f, wsum = 0.0f;
 
+    static_assert(1024 % (CB_TX * CB_TY) == 0, "table entries a thread");
+    if constexpr (!FLOAT_MATH) {
+#pragma unroll
+        for (int i = 0; i < 1024 / (CB_TX * CB_TY); i++)
+            sm.tab[tid + i * CB_TX * CB_TY] = craw_dm_exp2_tab1024[tid + i * CB_TX * CB_TY];
+    }
     /* positions up to the reach past the last valid sample, mirrored once
      * (wq, hq >= 5) */
     for (int i = tid; i < CB_LW * CB_LH; i += CB_TX * CB_TY) {
@@ -454,7 +464,7 @@ KDEV void dev_chroma_blur(SmCb &sm, const KFrame &f, int qx0, int qy0)
             const float dd1 = (s1 - c1) * inv_chroma;
             const float arg = -(float)(dx * dx + dy * dy) * 0.125f - dg * dg - dd0 * dd0 -
                               dd1 * dd1;
-            const float wgt = FLOAT_MATH ? kf_exp(arg) : craw_dm_expf_nonpos(arg);
+            const float wgt = FLOAT_MATH ? kf_exp(arg) : craw_dm_expf_nonpos_t1024(arg, sm.tab);
 
             acc0 += wgt * s0;
             acc1 += wgt * s1;

@@ -33,12 +33,19 @@
 
 static_assert(KG_RS_OUT == 116 && KG_RS_OUT_F == 112 && KG_RS_R % 8 == 0, "strip geometry");
 
+/* n / d, the float division's value: through kdiv_rn where the frame's
+ * constants bound the operands to its range (a uniform branch). */
+KDEV float rs_div(float n, float d, bool kdiv)
+{
+    return kdiv ? kdiv_rn(n, d) : n / d;
+}
+
 /* lib/ref/ref_debayer.c:195-202 */
-KDEV float rs_ratio(float a, float b)
+KDEV float rs_ratio(float a, float b, bool kdiv)
 {
     a = a > 0.0f ? a : 0.0f;
     b = b > 0.0f ? b : 0.0f;
-    return (a - b) / (a + b + RS_EPS);
+    return rs_div(a - b, a + b + RS_EPS, kdiv);
 }
 
 struct RsState {
@@ -84,27 +91,27 @@ KDEV float rs_box(const RsRow (&g)[3])
 }
 
 template <int C>
-KDEV void rs_weights(RsState &s)
+KDEV void rs_weights(RsState &s, bool kdiv)
 {
     const float a0 = rs_box<C>(s.g0), a1 = rs_box<C>(s.g1);
     const float b0 = rs_box<C>(s.g2), b1 = rs_box<C>(s.g3);
     const float vsum = a0 * (1.0f / 9.0f), hsum = a1 * (1.0f / 9.0f);
     const float ps = b0 * (1.0f / 9.0f), qs = b1 * (1.0f / 9.0f);
 
-    s.wv[2].c[C] = hsum / (hsum + vsum + RS_EPS);
-    s.wp[1].c[C] = qs / (ps + qs + RS_EPS);
+    s.wv[2].c[C] = rs_div(hsum, hsum + vsum + RS_EPS, kdiv);
+    s.wp[1].c[C] = rs_div(qs, ps + qs + RS_EPS, kdiv);
 }
 
 /* Green of the red/blue column N of the centre row v[3]. */
 template <int N>
-KDEV float rs_green(const RsState &s)
+KDEV float rs_green(const RsState &s, bool kdiv)
 {
     const float l00 = s.lp[2].c[N], lu = s.lp[0].c[N], ld = s.lp[4].c[N];
     const float ll = rs_l2<N>(s.lp[2]), lr = rs_r2<N>(s.lp[2]);
-    const float gv = 0.5f * (s.v[2].c[N] * (1.0f + rs_ratio(l00, lu)) +
-                             s.v[4].c[N] * (1.0f + rs_ratio(l00, ld)));
-    const float gh = 0.5f * (rs_l1<N>(s.v[3]) * (1.0f + rs_ratio(l00, ll)) +
-                             rs_r1<N>(s.v[3]) * (1.0f + rs_ratio(l00, lr)));
+    const float gv = 0.5f * (s.v[2].c[N] * (1.0f + rs_ratio(l00, lu, kdiv)) +
+                             s.v[4].c[N] * (1.0f + rs_ratio(l00, ld, kdiv)));
+    const float gh = 0.5f * (rs_l1<N>(s.v[3]) * (1.0f + rs_ratio(l00, ll, kdiv)) +
+                             rs_r1<N>(s.v[3]) * (1.0f + rs_ratio(l00, lr, kdiv)));
     const float w = s.wv[2].c[N];
 
     return w * gv + (1.0f - w) * gh;
@@ -173,14 +180,14 @@ KDEV void rs_step(RsState &s, const KFrame &f, int vt, int ld0, bool out_row, bo
     rs_shift(s.wv);
     rs_shift(s.wp);
     if constexpr (L >= 3) {
-        rs_weights<0>(s);
-        rs_weights<1>(s);
+        rs_weights<0>(s, f.prep.kdiv != 0);
+        rs_weights<1>(s, f.prep.kdiv != 0);
     }
 
     /* green, row vt - 3: the red/blue site is column P there */
     rs_shift(s.G);
     if constexpr (L >= 3) {
-        s.G[3].c[P] = rs_green<P>(s);
+        s.G[3].c[P] = rs_green<P>(s, f.prep.kdiv != 0);
         s.G[3].c[1 - P] = s.v[3].c[1 - P];
     }
 

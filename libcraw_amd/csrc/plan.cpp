@@ -110,6 +110,33 @@ int craw_hl_levels(uint32_t w, uint32_t h, int lw[KHL_LEVELS], int lh[KHL_LEVELS
     return n;
 }
 
+/* Whether the debayer's quotients stay where kdiv_rn divides exactly
+ * (divisor in [2^-40, 2^40], dividend +0 or in [2^-80, 2^40]; the
+ * dividends are differences and sums of non-negative values, never -0).
+ * With every
+ * site's black level 0 or in [2^-16, 1], range in [2^-16, 2] and gain in
+ * [2^-4, 2^4], a prepped sample is 0 or of magnitude in [2^-45, 2^20]: raw
+ * / 65535 - black is 0 or at least 2^-40 (two floats of at least 2^-16,
+ * or raw 0), the range at most 2 and the gain at least 2^-4. Every such
+ * float is a multiple of 2^-68, and so is every sum and difference the
+ * debayer forms from them with power-of-two weights (a rounded multiple of
+ * 2^-68 above it in magnitude stays one); the low pass divides by 16 and
+ * the gradient means multiply by a ninth, so a nonzero dividend is at least
+ * 2^-76, while no value exceeds 2^26. A rebuilt R16F mosaic (the highlight
+ * stage) holds multiples of 2^-24 below 2^16, inside the same bounds. */
+static bool prep_kdiv_ok(const KPrep &p)
+{
+    for (int s = 0; s < 4; s++) {
+        const float b = p.black[s], r = p.range[s], g = p.wb[s];
+
+        if (!(b == 0.0f || (b >= 0x1p-16f && b <= 1.0f)))
+            return false;
+        if (!(r >= 0x1p-16f && r <= 2.0f) || !(g >= 0x1p-4f && g <= 0x1p4f))
+            return false;
+    }
+    return true;
+}
+
 int craw_frame_plan(CRAWContext *ctx, const CRAWStageConfig &cfg, const CRAWFrameBuffers &buf,
                     uint32_t w, uint32_t h, const CRAWImage *src, const CRAWImage *dst,
                     const CRAWFrameParams *fp, CRAWFramePlan *plan)
@@ -175,8 +202,11 @@ int craw_frame_plan(CRAWContext *ctx, const CRAWStageConfig &cfg, const CRAWFram
         }
         plan->f.prep.black[s] = fp->profile->black_level[s];
         plan->f.prep.range[s] = range;
+        plan->f.prep.rrange[s] = 1.0 / (double)range;
         plan->f.prep.wb[s] = frame.wb[chan[s]];
     }
+    plan->f.prep.fdiv = cfg.fp16 ? 1 : 0;
+    plan->f.prep.kdiv = prep_kdiv_ok(plan->f.prep) ? 1 : 0;
 
     /* Denoise: which filters, which state sets. */
     spatial = cfg.has_denoise && fp->denoise_strength > 0.0f;

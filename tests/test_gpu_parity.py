@@ -220,3 +220,24 @@ def test_full_path(gpu_available):
     fp = frame_params(prof, denoise_strength=1.0, denoise_chroma_strength=0.5,
                       noise_override=(2e-4, 4e-6))
     _check(run, bayer, fp, "full path", hist=True)
+
+
+@pytest.mark.parametrize("case", ["black_zero", "black_tiny", "gain_large"])
+def test_profiles_either_side_of_the_short_division(gpu_available, case):
+    """The streamed RCD divides through kdiv_rn only when the frame's black
+    levels, ranges and gains keep its quotients in that sequence's range
+    (black 0 or in [2^-16, 1], range in [2^-16, 2], gains in [2^-4, 2^4]);
+    otherwise through the full division. Both ways the frame equals the
+    oracle's: black 0 inside the range, a black level of 2^-20 and a red
+    gain of 24 outside it."""
+    w, h = 160, 96
+    run = Runner(w, h, debayer=capi.CRAW_DEBAYER_RCD, out_space=capi.CRAW_CS_ACES_AP1_LINEAR,
+                 flags=capi.CRAW_PIPELINE_FLAG_AWB_HISTOGRAM)
+    if case == "black_zero":
+        prof = scene.make_profile(black=0.0)
+    elif case == "black_tiny":
+        prof = scene.make_profile(black=2.0 ** -20)
+    else:
+        prof = scene.make_profile(wb=(24.0, 1.0, 1.6))
+    bayer = scene.make_frame(w, h, seed=23)
+    _check(run, bayer, frame_params(prof), case, hist=True)
