@@ -141,12 +141,19 @@ struct RcStage {
     static constexpr int W = RC_TW + 2 * A, H = RC_TH + 2 * A, N = W * H;
 };
 
+/* n / d, the float division's value: through kdiv_rn where the frame's
+ * constants bound the operands to its range (a uniform branch). */
+KDEV float rcd_div(float n, float d, bool kdiv)
+{
+    return kdiv ? kdiv_rn(n, d) : n / d;
+}
+
 /* lib/ref/ref_debayer.c:195-202 */
-KDEV float rcd_ratio(float a, float b)
+KDEV float rcd_ratio(float a, float b, bool kdiv)
 {
     a = a > 0.0f ? a : 0.0f;
     b = b > 0.0f ? b : 0.0f;
-    return (a - b) / (a + b + RCD_EPS);
+    return rcd_div(a - b, a + b + RCD_EPS, kdiv);
 }
 
 /* lib/ref/ref_debayer.c:204-395. Stages and the apron each
@@ -197,6 +204,7 @@ KDEV void dev_debayer_rcd(SmRcd &sm, const KFrame &f, int x0, int y0)
     const uint16_t *__restrict__ src = f.src;
     const int src_pitch = f.src_pitch, W = f.W, H = f.H;
     const KPrep &prep = f.prep;
+    const bool kdiv = prep.kdiv != 0;
     const int tid = threadIdx.x;
     const int xe = min(x0 + RC_TW, W), ye = min(y0 + RC_TH, H);
     const int tw = xe - x0, th = ye - y0;
@@ -252,7 +260,7 @@ KDEV void dev_debayer_rcd(SmRcd &sm, const KFrame &f, int x0, int y0)
         }
         {
             float vsum = a0 * (1.0f / 9.0f), hsum = a1 * (1.0f / 9.0f);
-            swv[ly * S2::W + lx] = hsum / (hsum + vsum + RCD_EPS);
+            swv[ly * S2::W + lx] = rcd_div(hsum, hsum + vsum + RCD_EPS, kdiv);
         }
         if (lx >= 1 && ly >= 1 && lx < S2::W - 1 && ly < S2::H - 1 &&
             (INTERIOR || (lx < tw + 3 && ly < th + 3))) {
@@ -268,7 +276,7 @@ KDEV void dev_debayer_rcd(SmRcd &sm, const KFrame &f, int x0, int y0)
             }
             {
                 float ps = b0 * (1.0f / 9.0f), qs = b1 * (1.0f / 9.0f);
-                swp[(ly - 1) * S1::W + (lx - 1)] = qs / (ps + qs + RCD_EPS);
+                swp[(ly - 1) * S1::W + (lx - 1)] = rcd_div(qs, ps + qs + RCD_EPS, kdiv);
             }
         }
     RC_STAGE_END
@@ -283,10 +291,10 @@ KDEV void dev_debayer_rcd(SmRcd &sm, const KFrame &f, int x0, int y0)
             gval = V(0, 0);
         } else {
             float l00 = L(0, 0), lu = L(0, -2), ld = L(0, 2), ll = L(-2, 0), lr = L(2, 0);
-            float gv = 0.5f * (V(0, -1) * (1.0f + rcd_ratio(l00, lu))
-                             + V(0, 1) * (1.0f + rcd_ratio(l00, ld)));
-            float gh = 0.5f * (V(-1, 0) * (1.0f + rcd_ratio(l00, ll))
-                             + V(1, 0) * (1.0f + rcd_ratio(l00, lr)));
+            float gv = 0.5f * (V(0, -1) * (1.0f + rcd_ratio(l00, lu, kdiv))
+                             + V(0, 1) * (1.0f + rcd_ratio(l00, ld, kdiv)));
+            float gh = 0.5f * (V(-1, 0) * (1.0f + rcd_ratio(l00, ll, kdiv))
+                             + V(1, 0) * (1.0f + rcd_ratio(l00, lr, kdiv)));
             float w = AT(swv, 2, mx, my);
             gval = w * gv + (1.0f - w) * gh;
         }
@@ -532,35 +540,36 @@ KDEV void dev_chroma_up_finish(SmUp &sm, const KFrame &f, int x0, int y0)
         }
         __syncthreads();
 
+        /* lib/ref/ref_chroma.c:128-140: pixel x's taps are the cells
+         * floor(f) and floor(f) + 1 of f = (x + 0.5) / 4 - 0.5, with weight
+         * frac(f) on the second. f is exact in float (x < 2^22), so floor(f)
+         * = (x - 2) >> 2 and frac(f) = (2 (x & 3) + 5) / 8 mod 1: 0.625,
+         * 0.875, 0.125, 0.375 for x mod 4 = 0 .. 3; likewise in y. A
+         * thread's pixels (tid + 256 k of the 64x16 tile, x0 and y0 multiples
+         * of 4) share x mod 4 and y mod 4, so its four weights, products of
+         * those fractions (exact, the values the reference's float products
+         * give), are formed once; the taps' staged cells are (px + 2) >> 2
+         * and (py + 2) >> 2 (the clamp is in the staged cells). */
+        static_assert(CU_NT % CU_TW == 0 && (CU_NT / CU_TW) % 4 == 0, "pixel phases per thread");
+        const int px = tid % CU_TW, py0 = tid / CU_TW;
+        const float tx = (float)((2 * (px & 3) + 5) & 7) * 0.125f;
+        const float ty = (float)((2 * (py0 & 3) + 5) & 7) * 0.125f;
+        const float w00 = (1.0f - tx) * (1.0f - ty), w10 = tx * (1.0f - ty);
+        const float w01 = (1.0f - tx) * ty, w11 = tx * ty;
+        const int lx0 = (px + 2) >> 2, lx1 = lx0 + 1;
+        const int x = x0 + px;
+
 #pragma unroll
         for (int k = 0; k < CU_TW * CU_TH / CU_NT; k++) {
-            int p = tid + k * CU_NT, px = p % CU_TW, py = p / CU_TW;
-            int x = x0 + px, y = y0 + py;
-            float fy, fly, ty, fx, flx, tx, w00, w10, w01, w11, r, g, b, d[2];
-            int ly0, ly1, lx0, lx1;
+            const int py = py0 + k * (CU_NT / CU_TW), y = y0 + py;
+            const int ly0 = (py + 2) >> 2, ly1 = ly0 + 1;
+            float r, g, b, d[2];
 
             if (x >= W || y >= H)
                 continue;
             r = kglobal(it.rgb)[(size_t)y * W + x];
             g = kglobal(it.rgb)[plane + (size_t)y * W + x];
             b = kglobal(it.rgb)[2 * plane + (size_t)y * W + x];
-
-            /* lib/ref/ref_chroma.c:128-140: the taps' weights and (clamped)
-             * cells; the clamp is in the staged cells */
-            fy = ((float)y + 0.5f) * 0.25f - 0.5f;
-            fly = floorf(fy);
-            ty = fy - fly;
-            ly0 = (int)fly - qby;
-            ly1 = ly0 + 1;
-            fx = ((float)x + 0.5f) * 0.25f - 0.5f;
-            flx = floorf(fx);
-            tx = fx - flx;
-            lx0 = (int)flx - qbx;
-            lx1 = lx0 + 1;
-            w00 = (1.0f - tx) * (1.0f - ty);
-            w10 = tx * (1.0f - ty);
-            w01 = (1.0f - tx) * ty;
-            w11 = tx * ty;
 
             d[0] = r - g;
             d[1] = b - g;
