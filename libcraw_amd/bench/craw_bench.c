@@ -8,7 +8,7 @@
  *              [--warmup K] [--debayer rcd|mhc|bilinear]
  *              [--space xyz|709|2020|p3|ap0|ap1] [--denoise S]
  *              [--chroma S] [--temporal A] [--false-color N] [--highlights]
- *              [--awb] [--fp16] [--half] [--cold] [--estimate-noise]
+ *              [--awb] [--fp16] [--fast] [--half] [--cold] [--estimate-noise]
  *              [--out FILE]
  *
  * The frame is a raw little-endian uint16 RGGB dump of W x H samples
@@ -119,7 +119,7 @@ static void usage(void)
     fprintf(stderr, "usage: craw_bench [--raw FILE] [--width W] [--height H] [--frames N] "
                     "[--warmup K] [--debayer rcd|mhc|bilinear] [--space xyz|709|2020|p3|ap0|ap1] "
                     "[--denoise S] [--chroma S] [--temporal A] [--false-color N] [--highlights] [--awb] "
-                    "[--fp16] [--half] "
+                    "[--fp16] [--fast] [--half] "
                     "[--cold] [--estimate-noise] [--out FILE]\n");
 }
 
@@ -136,6 +136,7 @@ int main(int argc, char **argv)
 {
     const char *raw_path = NULL, *out_path = NULL;
     int w = 5760, h = 3240, frames = 100, warmup = 50, cold = 0, awb = 0, fp16 = 0, half = 0;
+    int fast = 0;
     int estimate_noise = 0, false_color = 0, highlights = 0;
     float denoise = 1.0f, chroma = 0.5f, temporal = 0.0f;
     enum CRAWDebayer debayer = CRAW_DEBAYER_RCD;
@@ -173,6 +174,7 @@ int main(int argc, char **argv)
         else if (!strcmp(a, "--awb")) awb = 1;
         else if (!strcmp(a, "--highlights")) highlights = 1;
         else if (!strcmp(a, "--fp16")) fp16 = 1;
+        else if (!strcmp(a, "--fast")) fast = 1;
         else if (!strcmp(a, "--half")) half = 1;
         else if (!strcmp(a, "--cold")) cold = 1;
         else if (!strcmp(a, "--estimate-noise")) estimate_noise = 1;
@@ -293,7 +295,7 @@ int main(int argc, char **argv)
                (temporal > 0.0f ? CRAW_PIPELINE_FLAG_TEMPORAL_DENOISE : 0) |
                (false_color > 0 ? CRAW_PIPELINE_FLAG_FALSE_COLOR : 0) |
                (highlights ? CRAW_PIPELINE_FLAG_HIGHLIGHT_RECONSTRUCTION : 0) |
-               (fp16 ? CRAW_PIPELINE_FLAG_NLM_FP16 : 0);
+               (fp16 ? CRAW_PIPELINE_FLAG_NLM_FP16 : 0) | (fast ? CRAW_PIPELINE_FLAG_NLM_FAST : 0);
     if ((ret = craw_pipeline_create(ctx, &pp, &pipe)) < 0) {
         fprintf(stderr, "craw_pipeline_create: %d\n", ret);
         return 1;
