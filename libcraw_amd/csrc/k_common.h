@@ -76,14 +76,14 @@ struct KPrep {
 
 /* n / d correctly rounded, for d in [2^-40, 2^40] and n +0 or in
  * magnitude in [2^-80, 2^40], where neither operand nor the quotient is
- * near the ends of the float range (a -0 dividend gives +0, the division
- * -0): a reciprocal estimate refined once by
- * Newton's iteration, then the quotient corrected twice by its exact
- * residual (Markstein, IBM J. Res. Dev. 34(1), 1990). These are the steps
- * of the compiler's correctly rounded float division without its scaling
- * and fix-up instructions (v_div_scale, v_div_fmas, v_div_fixup), which
- * leave such operands unchanged; eight instructions against eleven.
- * Checked against the division on the GPU over operands across that range
+ * near the ends of the float range (a -0 dividend gives +0 where the
+ * division gives -0). A reciprocal estimate refined once by Newton's
+ * iteration, then the quotient corrected twice by its exact residual
+ * (Markstein, IBM J. Res. Dev. 34(1), 1990): the steps of the compiler's
+ * correctly rounded float division without its scaling and fix-up
+ * instructions (v_div_scale, v_div_fmas, v_div_fixup), which leave such
+ * operands unchanged; eight instructions against eleven. Checked against
+ * IEEE division on the GPU over operands across that range
  * (tests/test_gpu_kdiv.py). */
 KDEV float kdiv_rn(float n, float d)
 {
