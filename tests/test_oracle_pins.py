@@ -222,7 +222,7 @@ def pin_nlm_weight(mode):
     exp(-max(d2/9/h^2 - (vp + vq), 0)/vp). A noise-free step of D = 2 over a
     whole patch keeps exp(-2); with noise its expected distance is D^2 + 2,
     where the weight is exp(-D^2): 1.8% at two sigma, as the reference
-    states (DESIGN.md:1700-1706)."""
+    states (its DESIGN.md:1700-1706)."""
     rt = _tol(mode, 1e-14, 3e-7)
     for inv_h2 in (1.0, 1.0 / 0.49, 4.0):
         for d2 in (0.0, 3.0, 9.0, 17.9, 18.1, 27.0, 36.0, 60.0, 200.0):
