@@ -39,6 +39,22 @@ struct Staging {
     bool used;
 };
 
+/* A captured launch whose arguments are a frame's constants. */
+enum FrameNodeKind {
+    FN_UP,              /* (KFrame): chroma up + colour */
+    FN_HL,              /* (KHighlight): highlight field, coarse levels, fill */
+    FN_HL_PUSH,         /* (KHighlight, int level) */
+    FN_FC,              /* (KFrame, KInter): false colour */
+};
+
+struct FrameNode {
+    hipGraphNode_t node;
+    hipKernelNodeParams params;
+    uint32_t frame;
+    FrameNodeKind kind;
+    int level;
+};
+
 struct CRAWBatch {
     CRAWContext *ctx;
     CRAWBatchParams params;
@@ -75,12 +91,11 @@ struct CRAWBatch {
     std::vector<uint32_t> graph_key;
     hipGraph_t graph;
     hipGraphExec_t exec;
-    /* the graph's chroma-up launches, which take a frame's constants as
-     * arguments: each node, its captured launch shape and the frame it
-     * develops; their arguments are set again before every replay */
-    std::vector<hipGraphNode_t> cu_node;
-    std::vector<hipKernelNodeParams> cu_params;
-    std::vector<uint32_t> cu_frame;
+    /* the graph's launches that take a frame's constants as arguments
+     * (chroma up + colour, the highlight stage, false colour): each node,
+     * its captured launch shape, the frame it develops and what its
+     * arguments are; they are set again before every replay */
+    std::vector<FrameNode> fnode;
 
     CRAWProfiler prof;
 };
@@ -505,8 +520,8 @@ static hipError_t batch_launch(CRAWBatch *b, const BatchPlan &bp, const size_t *
         PE();
     }
     /* highlight reconstruction and false colour: plain launches a frame
-     * (such recordings are never captured into a graph), each step over
-     * all frames one stage */
+     * (in a graph, their arguments set again before each replay), each
+     * step over all frames one stage */
     {
         bool any = false;
 
@@ -603,22 +618,27 @@ rest:
     return e;
 }
 
-/* The chroma-up launches of a captured recording: every kernel node of
- * that kernel, matched to the frame whose constants (its one argument,
- * copied at capture) it took. */
-static hipError_t graph_chroma_nodes(hipGraph_t g, const BatchPlan &bp,
-                                     std::vector<hipGraphNode_t> &node,
-                                     std::vector<hipKernelNodeParams> &params,
-                                     std::vector<uint32_t> &frame)
+/* The launches of a captured recording that take a frame's constants as
+ * arguments: every kernel node of those kernels, matched to the frame
+ * whose constants it took (its arguments copied at capture), and checked
+ * against the number of such launches the recording makes, so that no
+ * node keeps a previous recording's constants. */
+static hipError_t graph_frame_nodes(hipGraph_t g, const BatchPlan &bp,
+                                    std::vector<FrameNode> &out)
 {
     const void *cu = craw_chroma_up_kernel();
+    const void *hl[4] = { craw_hl_kernel(0), craw_hl_kernel(1), craw_hl_kernel(2),
+                          craw_hl_kernel(3) };
     std::vector<hipGraphNode_t> nodes;
-    std::vector<bool> taken(bp.plans.size(), false);
     size_t n = 0, want = 0;
     hipError_t e;
 
-    for (const CRAWFramePlan &pl : bp.plans)
+    for (const CRAWFramePlan &pl : bp.plans) {
         want += has_up(pl) ? 1 : 0;
+        if (pl.hl)
+            want += 2 + (size_t)craw_hl_push_launches(pl.hl_k);
+        want += pl.fc_passes ? 1 : 0;
+    }
     if ((e = hipGraphGetNodes(g, NULL, &n)) != hipSuccess)
         return e;
     nodes.resize(n);
@@ -627,6 +647,8 @@ static hipError_t graph_chroma_nodes(hipGraph_t g, const BatchPlan &bp,
     for (hipGraphNode_t nd : nodes) {
         hipGraphNodeType t;
         hipKernelNodeParams kp;
+        FrameNode fnd;
+        size_t i;
 
         if ((e = hipGraphNodeGetType(nd, &t)) != hipSuccess)
             return e;
@@ -634,24 +656,45 @@ static hipError_t graph_chroma_nodes(hipGraph_t g, const BatchPlan &bp,
             continue;
         if ((e = hipGraphKernelNodeGetParams(nd, &kp)) != hipSuccess)
             return e;
-        if (kp.func != cu)
+        if (kp.func == cu)
+            fnd.kind = FN_UP;
+        else if (kp.func == hl[0] || kp.func == hl[1] || kp.func == hl[3])
+            fnd.kind = FN_HL;
+        else if (kp.func == hl[2])
+            fnd.kind = FN_HL_PUSH;
+        else if (craw_is_false_color_kernel(kp.func))
+            fnd.kind = FN_FC;
+        else
             continue;
-        if (!kp.kernelParams || !kp.kernelParams[0])
+        if (!kp.kernelParams || !kp.kernelParams[0] ||
+            ((fnd.kind == FN_HL_PUSH || fnd.kind == FN_FC) && !kp.kernelParams[1]))
             return hipErrorInvalidValue;
-        const KFrame *a = (const KFrame *)kp.kernelParams[0];
-        size_t i = 0;
-        while (i < bp.plans.size() &&
-               (taken[i] || !has_up(bp.plans[i]) ||
-                memcmp(a, &bp.plans[i].f, sizeof(KFrame)) != 0))
-            i++;
+        fnd.level = fnd.kind == FN_HL_PUSH ? *(const int *)kp.kernelParams[1] : 0;
+        for (i = 0; i < bp.plans.size(); i++) {
+            const CRAWFramePlan &pl = bp.plans[i];
+            const void *a = kp.kernelParams[0];
+
+            if (fnd.kind == FN_UP && has_up(pl) && !memcmp(a, &pl.f, sizeof(KFrame)))
+                break;
+            if ((fnd.kind == FN_HL || fnd.kind == FN_HL_PUSH) && pl.hl &&
+                !memcmp(a, &pl.hl_k, sizeof(KHighlight)))
+                break;
+            if (fnd.kind == FN_FC && pl.fc_passes && !memcmp(a, &pl.f, sizeof(KFrame)) &&
+                !memcmp(kp.kernelParams[1], &pl.fc_to, sizeof(KInter)))
+                break;
+        }
         if (i == bp.plans.size())
             return hipErrorInvalidValue;
-        taken[i] = true;
-        node.push_back(nd);
-        params.push_back(kp);
-        frame.push_back((uint32_t)i);
+        /* one node a frame for each kernel (a level for the push) */
+        for (const FrameNode &o : out)
+            if (o.frame == i && o.params.func == kp.func && o.level == fnd.level)
+                return hipErrorInvalidValue;
+        fnd.node = nd;
+        fnd.params = kp;
+        fnd.frame = (uint32_t)i;
+        out.push_back(fnd);
     }
-    return node.size() == want ? hipSuccess : hipErrorInvalidValue;
+    return out.size() == want ? hipSuccess : hipErrorInvalidValue;
 }
 
 static int batch_record(CRAWBatch *b, hipStream_t s, const CRAWBatchFrame *frames, uint32_t nb,
@@ -696,19 +739,11 @@ static int batch_record(CRAWBatch *b, hipStream_t s, const CRAWBatchFrame *frame
     stg->used = true;
     b->ring_pos = (b->ring_pos + 1) % RING;
 
-    /* a recording with a stage that takes a frame's constants as launch
-     * arguments before the chroma-up stage is launched directly */
-    for (uint32_t i = 0; i < nb && graphs; i++)
-        if (bp.plans[i].hl || bp.plans[i].fc_passes)
-            graphs = false;
     if (graphs) {
         if (!b->exec || bp.key != b->graph_key) {
             hipGraph_t g = NULL;
             hipGraphExec_t x = NULL;
-
-            std::vector<hipGraphNode_t> cu_node;
-            std::vector<hipKernelNodeParams> cu_params;
-            std::vector<uint32_t> cu_frame;
+            std::vector<FrameNode> fnode;
 
             e = hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed);
             if (e == hipSuccess) {
@@ -718,7 +753,7 @@ static int batch_record(CRAWBatch *b, hipStream_t s, const CRAWBatchFrame *frame
                     e = le;
             }
             if (e == hipSuccess)
-                e = graph_chroma_nodes(g, bp, cu_node, cu_params, cu_frame);
+                e = graph_frame_nodes(g, bp, fnode);
             if (e == hipSuccess)
                 e = hipGraphInstantiate(&x, g, NULL, NULL, 0);
             if (e != hipSuccess) {
@@ -735,19 +770,28 @@ static int batch_record(CRAWBatch *b, hipStream_t s, const CRAWBatchFrame *frame
             b->graph = g;
             b->exec = x;
             b->graph_key = bp.key;
-            b->cu_node.swap(cu_node);
-            b->cu_params.swap(cu_params);
-            b->cu_frame.swap(cu_frame);
+            b->fnode.swap(fnode);
         }
-        /* this recording's constants into the chroma-up launches */
-        for (size_t k = 0; k < b->cu_node.size() && e == hipSuccess; k++) {
-            hipKernelNodeParams kp = b->cu_params[k];
-            KFrame arg = bp.plans[b->cu_frame[k]].f;
-            void *args[1] = { &arg };
+        /* this recording's constants into the launches that take them */
+        for (size_t k = 0; k < b->fnode.size() && e == hipSuccess; k++) {
+            const FrameNode &fnd = b->fnode[k];
+            const CRAWFramePlan &pl = bp.plans[fnd.frame];
+            hipKernelNodeParams kp = fnd.params;
+            KFrame f = pl.f;
+            KHighlight h = pl.hl_k;
+            KInter to = pl.fc_to;
+            int level = fnd.level;
+            void *args[2];
 
+            switch (fnd.kind) {
+            case FN_UP: args[0] = &f; break;
+            case FN_HL: args[0] = &h; break;
+            case FN_HL_PUSH: args[0] = &h; args[1] = &level; break;
+            case FN_FC: args[0] = &f; args[1] = &to; break;
+            }
             kp.kernelParams = args;
             kp.extra = NULL;
-            e = hipGraphExecKernelNodeSetParams(b->exec, b->cu_node[k], &kp);
+            e = hipGraphExecKernelNodeSetParams(b->exec, fnd.node, &kp);
         }
         if (e != hipSuccess) {
             craw_log(b->ctx, CRAW_LOG_ERROR, "Graph update failed: %s\n", hipGetErrorString(e));

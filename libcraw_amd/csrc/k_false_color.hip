@@ -274,6 +274,22 @@ static void fc_launch(bool chroma, const KFrame &f, const KInter &to, hipStream_
         KLAUNCH((k_false_color<P, false>), grid, dim3(FC_NT), 0, s, f, to);
 }
 
+bool craw_is_false_color_kernel(const void *fn)
+{
+    static const void *const k[10] = {
+        (const void *)k_false_color<1, false>, (const void *)k_false_color<1, true>,
+        (const void *)k_false_color<2, false>, (const void *)k_false_color<2, true>,
+        (const void *)k_false_color<3, false>, (const void *)k_false_color<3, true>,
+        (const void *)k_false_color<4, false>, (const void *)k_false_color<4, true>,
+        (const void *)k_false_color<5, false>, (const void *)k_false_color<5, true>,
+    };
+
+    for (int i = 0; i < 10; i++)
+        if (fn == k[i])
+            return true;
+    return false;
+}
+
 hipError_t craw_launch_false_color(int passes, bool chroma, const KFrame &f, const KInter &to,
                                    hipStream_t s)
 {

@@ -461,6 +461,25 @@ hipError_t craw_launch_hl_push(const KHighlight &h, hipStream_t s)
     return hipSuccess;
 }
 
+/* Launches craw_launch_hl_push makes for `h`: the coarse levels' one
+ * workgroup when there are more than five levels, then a push a level from
+ * min(nl - 2, 3) down to 0. */
+int craw_hl_push_launches(const KHighlight &h)
+{
+    return (h.nl > 5 ? 1 : 0) + ((h.nl - 2 < 3 ? h.nl - 2 : 3) + 1);
+}
+
+const void *craw_hl_kernel(int which)
+{
+    switch (which) {
+    case 0: return (const void *)k_hl_field;
+    case 1: return (const void *)k_hl_coarse;
+    case 2: return (const void *)k_hl_push;
+    case 3: return (const void *)k_hl_fill;
+    default: return NULL;
+    }
+}
+
 hipError_t craw_launch_hl_fill(const KHighlight &h, hipStream_t s)
 {
     const dim3 grid((h.W + HL_FW - 1) / HL_FW, (h.H + HL_FH - 1) / HL_FH);

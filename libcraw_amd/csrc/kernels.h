@@ -409,6 +409,11 @@ hipError_t craw_launch_chroma_blur(const KFrame &f, hipStream_t s);
 hipError_t craw_launch_chroma_up_finish(const KFrame &f, hipStream_t s);
 /* The chroma-up kernel's host handle, as a graph's kernel nodes name it. */
 const void *craw_chroma_up_kernel(void);
+/* The same for the highlight stage's kernels (0 field, 1 coarse, 2 push,
+ * 3 fill) and the false colour kernels, whose arguments are also a frame's
+ * constants. */
+const void *craw_hl_kernel(int which);
+bool craw_is_false_color_kernel(const void *fn);
 /* RCD over the interior of craw_rcd_split(), into the chroma stage or to
  * the output (the tiled launcher does the ring). */
 hipError_t craw_launch_rcd_stream(bool to_inter, const KFrame &f, hipStream_t s);
@@ -441,6 +446,8 @@ struct KHighlight {
 hipError_t craw_launch_hl_field(const KHighlight &h, hipStream_t s);
 hipError_t craw_launch_hl_push(const KHighlight &h, hipStream_t s);
 hipError_t craw_launch_hl_fill(const KHighlight &h, hipStream_t s);
+/* How many launches craw_launch_hl_push makes for `h`. */
+int craw_hl_push_launches(const KHighlight &h);
 
 /* False colour suppression, `passes` 1..5, on f.inter.rgb: into the chroma
  * stage's inputs `to` (planes and quarter means) when chroma is set, else

@@ -196,27 +196,32 @@ def test_graph_replays_follow_new_buffers_and_params(gpu_available):
 def test_highlights_and_false_colour(gpu_available, graphs):
     """Frames of uneven sizes with and without highlight reconstruction and
     with 0-2 false colour passes, before and without the chroma stage, in
-    one recording (a graph is not used for such a recording even when
-    asked for): each frame as the plain form develops it."""
+    one recording: each frame as the plain form develops it. With graphs
+    the recording is captured once and replayed with other destination
+    buffers, mosaics and frame parameters each time (the highlight and
+    false colour launches take a frame's constants as arguments, set again
+    before each replay)."""
     flags = (capi.CRAW_PIPELINE_FLAG_CHROMA_DENOISE | capi.CRAW_PIPELINE_FLAG_AWB_HISTOGRAM |
              capi.CRAW_PIPELINE_FLAG_HIGHLIGHT_RECONSTRUCTION |
              capi.CRAW_PIPELINE_FLAG_FALSE_COLOR)
     prof = scene.make_profile()
     sizes = SIZES[:4]
-    fr = Frames(sizes)
-    mos = mosaics(sizes, 500)
-    for m in mos:
-        h, w = m.shape
-        m[h // 4:h // 2, w // 3:w // 2] = 4095           # blown
-    fps = [frame_params(prof, flags=capi.CRAW_FRAME_FLAG_HIGHLIGHT_RECONSTRUCTION if i % 2 == 0
-                        else 0, false_color_passes=i % 3,
-                        denoise_chroma_strength=0.5 if i != 1 else 0.0)
-           for i in range(len(sizes))]
     b = new_batch(sizes, flags, graphs=graphs)
     stream = hip.Stream()
-    frames = [batch_frame(*fr.images(i), fps[i]) for i in range(len(sizes))]
-    for rep in range(2):
+    for rep in range(3):
+        fr = Frames(sizes)
+        mos = mosaics(sizes, 500 + 10 * rep)
+        for m in mos:
+            h, w = m.shape
+            m[h // 4:h // 2, w // 3:w // 2 + 4 * rep] = 4095           # blown
+        fps = [frame_params(prof,
+                            flags=capi.CRAW_FRAME_FLAG_HIGHLIGHT_RECONSTRUCTION if i % 2 == 0
+                            else 0, false_color_passes=i % 3,
+                            denoise_chroma_strength=(0.5 + 0.1 * rep) if i != 1 else 0.0,
+                            exposure_stops=0.25 * rep)
+               for i in range(len(sizes))]
         fr.upload(mos)
+        frames = [batch_frame(*fr.images(i), fps[i]) for i in range(len(sizes))]
         assert b.record(stream, frames) == 0
         stream.synchronize()
         check_against_plain(b, fr, fps, mos, flags, capi.CRAW_DEBAYER_RCD, "rep %d" % rep)
