@@ -141,13 +141,22 @@ template <bool INTER, bool GAMUT, int P, int L = 5>
 KDEV void rs_step(RsState &s, const KFrame &f, int vt, int ld0, bool out_row, bool out_cols,
                   int X0)
 {
-    /* V row vt */
+    /* V row vt: the lane's two samples (ld0 is even) in one 32-bit load
+     * where the mosaic's rows are 4-byte aligned (a uniform branch) */
     rs_shift(s.v);
     {
         const uint16_t *row = f.src + (size_t)vt * f.src_pitch;
         const int sb = (vt & 1) * 2;
-        s.v[6].c[0] = kprep(row[ld0], sb, f.prep);
-        s.v[6].c[1] = kprep(row[ld0 + 1], sb + 1, f.prep);
+
+        if (((uintptr_t)f.src & 3u) == 0 && (f.src_pitch & 1) == 0) {
+            const uint32_t pair = *(const uint32_t *)(row + ld0);
+
+            s.v[6].c[0] = kprep(pair & 0xffffu, sb, f.prep);
+            s.v[6].c[1] = kprep(pair >> 16, sb + 1, f.prep);
+        } else {
+            s.v[6].c[0] = kprep(row[ld0], sb, f.prep);
+            s.v[6].c[1] = kprep(row[ld0 + 1], sb + 1, f.prep);
+        }
     }
 
     /* low pass, row vt - 1 */

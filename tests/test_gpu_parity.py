@@ -241,3 +241,37 @@ def test_profiles_either_side_of_the_short_division(gpu_available, case):
         prof = scene.make_profile(wb=(24.0, 1.0, 1.6))
     bayer = scene.make_frame(w, h, seed=23)
     _check(run, bayer, frame_params(prof), case, hist=True)
+
+
+@pytest.mark.parametrize("layout", ["odd_pitch", "offset_base", "wide_pitch"])
+def test_source_pitch_and_alignment(gpu_available, layout):
+    """The streamed RCD loads a lane's two samples as one 32-bit word where
+    the mosaic's rows are 4-byte aligned, and as two 16-bit loads where
+    they are not: a row pitch of an odd number of samples, a base address
+    2 bytes past a 4-byte boundary, and an aligned padded pitch all develop
+    the frame the oracle develops."""
+    from libcraw_amd import hip
+    from libcraw_amd.pipeline import image
+
+    w, h = 200, 112
+    pad, off = {"odd_pitch": (1, 0), "offset_base": (2, 2), "wide_pitch": (6, 0)}[layout]
+    run = Runner(w, h, debayer=capi.CRAW_DEBAYER_RCD, out_space=capi.CRAW_CS_ACES_AP1_LINEAR,
+                 flags=capi.CRAW_PIPELINE_FLAG_AWB_HISTOGRAM)
+    prof = scene.make_profile()
+    bayer = scene.make_frame(w, h, seed=29)
+    fp = frame_params(prof)
+    padded = np.zeros((h, w + pad), np.uint16)
+    padded[:, :w] = bayer
+    buf = hip.DeviceBuffer(padded.nbytes + 16)
+    raw = np.zeros(padded.nbytes + 16, np.uint8)
+    raw[off:off + padded.nbytes] = padded.view(np.uint8).ravel()
+    buf.upload(raw)
+    src = image(buf.value + off, w, h, capi.CRAW_FORMAT_R16_UNORM, pitch=(w + pad) * 2)
+    assert run.pipe.record(run.stream, src, image(run.dst, w, h, run.fmt), fp) == 0
+    run.stream.synchronize()
+    ref = oracle.f32d.process(bayer, run.pp, fp, want_hist=True)
+    assert ref["ret"] == 0
+    assert_bit_exact(run.output(), ref["out"], layout)
+    for name, a, b in zip(("count", "weight", "bright"), run.pipe.histogram(), ref["hist"]):
+        assert np.array_equal(a, b), "%s: histogram %s differs" % (layout, name)
+    buf.close()
