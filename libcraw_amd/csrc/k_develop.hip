@@ -50,19 +50,32 @@ KDEV void dev_debayer_simple(SmSimple &sm, const KFrame &f, int x0, int y0)
     const int rw = xe - x0 + 2 * BL_A, rh = ye - y0 + 2 * BL_A;
 
     /* compile-time strides; in an interior tile no position needs the
-     * mirror */
-    for (int i = tid; i < RW * RH; i += BL_NT) {
-        int lx = i % RW, ly = i / RW;
-        int mx = x0 - BL_A + lx, my = y0 - BL_A + ly, s;
+     * mirror, and where the mosaic's rows are 4-byte aligned a thread
+     * loads a pair of samples (x0 - BL_A is even) as one 32-bit word */
+    static_assert(RW % 2 == 0 && BL_A % 2 == 0, "sample pairs");
+    if (INTERIOR && ((uintptr_t)src & 3u) == 0 && (src_pitch & 1) == 0) {
+        for (int i = tid; i < RW / 2 * RH; i += BL_NT) {
+            const int lx = 2 * (i % (RW / 2)), ly = i / (RW / 2);
+            const int mx = x0 - BL_A + lx, my = y0 - BL_A + ly;
+            const uint32_t pair = *(const uint32_t *)(src + (size_t)my * src_pitch + mx);
 
-        if (!INTERIOR) {
-            if (lx >= rw || ly >= rh)
-                continue;
-            mx = kmirror(mx, W);
-            my = kmirror(my, H);
+            vs[ly][lx] = kprep(pair & 0xffffu, (my & 1) * 2, prep);
+            vs[ly][lx + 1] = kprep(pair >> 16, (my & 1) * 2 + 1, prep);
         }
-        s = (my & 1) * 2 + (mx & 1);
-        vs[ly][lx] = kprep(src[(size_t)my * src_pitch + mx], s, prep);
+    } else {
+        for (int i = tid; i < RW * RH; i += BL_NT) {
+            int lx = i % RW, ly = i / RW;
+            int mx = x0 - BL_A + lx, my = y0 - BL_A + ly, s;
+
+            if (!INTERIOR) {
+                if (lx >= rw || ly >= rh)
+                    continue;
+                mx = kmirror(mx, W);
+                my = kmirror(my, H);
+            }
+            s = (my & 1) * 2 + (mx & 1);
+            vs[ly][lx] = kprep(src[(size_t)my * src_pitch + mx], s, prep);
+        }
     }
     __syncthreads();
 

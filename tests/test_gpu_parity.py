@@ -243,19 +243,20 @@ def test_profiles_either_side_of_the_short_division(gpu_available, case):
     _check(run, bayer, frame_params(prof), case, hist=True)
 
 
+@pytest.mark.parametrize("method", METHODS)
 @pytest.mark.parametrize("layout", ["odd_pitch", "offset_base", "wide_pitch"])
-def test_source_pitch_and_alignment(gpu_available, layout):
-    """The streamed RCD loads a lane's two samples as one 32-bit word where
-    the mosaic's rows are 4-byte aligned, and as two 16-bit loads where
-    they are not: a row pitch of an odd number of samples, a base address
-    2 bytes past a 4-byte boundary, and an aligned padded pitch all develop
-    the frame the oracle develops."""
+def test_source_pitch_and_alignment(gpu_available, layout, method):
+    """The streamed RCD and the bilinear / MHC tiles load two samples as one
+    32-bit word where the mosaic's rows are 4-byte aligned, and as two
+    16-bit loads where they are not: a row pitch of an odd number of
+    samples, a base address 2 bytes past a 4-byte boundary, and an aligned
+    padded pitch all develop the frame the oracle develops."""
     from libcraw_amd import hip
     from libcraw_amd.pipeline import image
 
     w, h = 200, 112
     pad, off = {"odd_pitch": (1, 0), "offset_base": (2, 2), "wide_pitch": (6, 0)}[layout]
-    run = Runner(w, h, debayer=capi.CRAW_DEBAYER_RCD, out_space=capi.CRAW_CS_ACES_AP1_LINEAR,
+    run = Runner(w, h, debayer=method, out_space=capi.CRAW_CS_ACES_AP1_LINEAR,
                  flags=capi.CRAW_PIPELINE_FLAG_AWB_HISTOGRAM)
     prof = scene.make_profile()
     bayer = scene.make_frame(w, h, seed=29)
