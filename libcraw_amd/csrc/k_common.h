@@ -151,13 +151,36 @@ struct KColor {
     float scale;
     float out[3][3];
     float thr[3], scl[3], pwr, inv_pwr;
+    float thr_lo[3];     /* kgamut's screen: kgamut_screen_bound(thr[i]) */
     uint32_t flags;
 };
+
+/* A bound below a gamut threshold for kgamut's screen: thr (1 - 2^-20),
+ * or -inf (the screen never passes) for a threshold that is not a
+ * positive float in [2^-100, 2^100]. Rounding the double product to
+ * float moves it by at most 2^-24 relative, which the margin covers. */
+__host__ __device__ static inline float kgamut_screen_bound(float thr)
+{
+    if (!(thr >= 0x1p-100f && thr <= 0x1p100f))
+        return -__builtin_inff();
+    return (float)((double)thr * (1.0 - 0x1p-20));
+}
 
 /* lib/gamut.c:66-92: ACES 1.3 RGC of one AP1 triple. Each channel's
  * distance comes from the input triple; the compression of a channel past
  * its threshold is one loop body (not unrolled), so its two powf
- * evaluations appear once in the code. */
+ * evaluations appear once in the code.
+ *
+ * Most triples have every distance below its threshold and leave the
+ * compression unchanged. A screen finds them without the three correctly
+ * rounded divisions: with y the hardware reciprocal of mag (1 ulp) and
+ * q = (ach - x) y rounded, q is within 2^-22 (relative) of the quotient
+ * (ach - x) / mag for mag in [2^-64, 2^64] (y a normal float; a q that
+ * underflows is within 2^-149 of it). q < thr_lo = thr (1 - 2^-20) then
+ * puts the quotient below thr (1 - 2^-21), under the midpoint between thr
+ * and its predecessor (at least thr (1 - 2^-24)), so its correct rounding,
+ * the distance d, is below thr and the channel is left as it is. A NaN or
+ * infinite operand fails the screen and takes the exact path. */
 KDEV void kgamut(const KColor &c, float &r, float &g, float &b)
 {
     const float ach = r > (g > b ? g : b) ? r : (g > b ? g : b);
@@ -172,6 +195,13 @@ KDEV void kgamut(const KColor &c, float &r, float &g, float &b)
     if (ach == 0.0f)
         return;
     mag = fabsf(ach);
+    {
+        const float y = __builtin_amdgcn_rcpf(mag);
+
+        if (mag >= 0x1p-64f && mag <= 0x1p64f && (ach - r) * y < c.thr_lo[0] &&
+            (ach - g) * y < c.thr_lo[1] && (ach - b) * y < c.thr_lo[2])
+            return;
+    }
     d0 = (ach - r) / mag;
     d1 = (ach - g) / mag;
     d2 = (ach - b) / mag;

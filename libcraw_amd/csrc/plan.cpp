@@ -356,6 +356,8 @@ int craw_frame_plan(CRAWContext *ctx, const CRAWStageConfig &cfg, const CRAWFram
     craw_gamut_params(&gp);
     memcpy(plan->f.col.thr, gp.thr, sizeof(gp.thr));
     memcpy(plan->f.col.scl, gp.scl, sizeof(gp.scl));
+    for (int i = 0; i < 3; i++)
+        plan->f.col.thr_lo[i] = kgamut_screen_bound(gp.thr[i]);
     plan->f.col.pwr = gp.pwr;
     plan->f.col.inv_pwr = gp.inv_pwr;
     plan->f.col.flags = (fp->flags & CRAW_FRAME_FLAG_CLAMP_NEGATIVE) ? KCOLOR_CLAMP_NEGATIVE : 0u;
