@@ -610,14 +610,19 @@ KDEV void dev_chroma_up_finish(SmUp &sm, const KFrame &f, int x0, int y0)
 /* ------------------------------------------------------ AWB histogram */
 
 /* The histogram of one chunk of a frame's block samples, privatised in an
- * LDS hash of bins: each lane merges the runs of equal bins among its 32
- * consecutive samples (neighbouring blocks mostly share a bin), inserts
+ * LDS hash of bins: each lane merges the runs of equal bins among its 8
+ * consecutive samples (neighbouring blocks often share a bin), inserts
  * each run into the hash with LDS atomics, and the workgroup then adds
  * every occupied entry to the frame's histogram with three global
  * atomics. A run whose bin finds no slot within AWB_PROBES probes goes to
  * the global histogram directly, so the stage terminates whatever the
- * data. Integer sums: the result does not depend on the grouping. */
-#define AWB_NT 256
+ * data. Integer sums: the result does not depend on the grouping.
+ * A chunk of 8192 samples (a 5.8K frame has 36) is shared by 1024
+ * threads: a lane's chain of hash inserts, each waiting for its
+ * compare-and-swap, is what bounds a chunk's time (0.020 ms a 5.8K frame
+ * with 256 threads of 32 samples, 0.014 with 1024 of 8; chunks of 4096 or
+ * 16384 samples at 1024 threads: 0.016 and 0.020). */
+#define AWB_NT 1024
 #define AWB_PER (KG_AWB_CHUNK / AWB_NT)
 #define AWB_HASH 4096
 #define AWB_PROBES 32
