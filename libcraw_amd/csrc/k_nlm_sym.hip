@@ -206,6 +206,26 @@ KDEV float xs_quot(float a, double r)
     return (float)((double)a * r);
 }
 
+/* The workgroup's plane constants, selected by the plane index (the same
+ * in every lane), held in scalar registers: in vector registers the
+ * precision form's history band spilled them, and reloaded them in its
+ * row loop. */
+KDEV float xs_uniform(float v)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, v)));
+}
+
+/* Sample i of a mosaic row, through a buffer resource on the row (in
+ * scalar registers), so that a lane holds only its 32-bit offset rather
+ * than a 64-bit pointer (which the same band spilled). */
+KDEV uint32_t xs_row_load(const uint16_t *row, int i)
+{
+    const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(
+        (void *)row, (short)0, 0x7fffffff, 0x00020000);
+
+    return __builtin_amdgcn_raw_buffer_load_b16(r, 2 * i, 0, 0);
+}
+
 /* A lane's running state down its column. The squared differences of
  * three rows live in three arrays whose roles rotate (the row loop is
  * unrolled by three, so the rotation costs no moves); the sums of the four
@@ -371,7 +391,8 @@ struct XsBand {
                 /* highlight guard and the 16-bit re-encode
                  * (lib/ref/ref_denoise.c:298-323) */
                 u = xs_igat(p, L.a[0] / L.ws[0]);
-                raw0 = (float)f.dn_src[(size_t)(2 * y + py) * p.src_pitch + 2 * x + px] *
+                raw0 = (float)xs_row_load(f.dn_src + (size_t)(2 * y + py) * p.src_pitch,
+                                          2 * x + px) *
                        (1.0f / 65535.0f);
                 u0 = (raw0 - black) * inv_range;
                 gr = (u0 - p.guard_lo) * p.guard_inv;
@@ -619,7 +640,8 @@ KDEV void dev_denoise_sym(SmXs<MODE, TALL, F16> &sm, double *tab, const KFrame &
     const int x = x0 - 4 + XS_OUTW * wave + lane;
     XsBand<MODE, F16, TALL> band = { sm, f, lane, lc, x, y0, px, py,
                                lane >= 4 && lane < 4 + XS_OUTW && x < w2,
-                               black, range, inv_range, p.inv_h2, tab };
+                               xs_uniform(black), xs_uniform(range), xs_uniform(inv_range),
+                               p.inv_h2, tab };
     XsLane L;
 
     if constexpr (F16) {
