@@ -414,9 +414,10 @@ const void *craw_chroma_up_kernel(void);
  * constants. */
 const void *craw_hl_kernel(int which);
 bool craw_is_false_color_kernel(const void *fn);
-/* RCD over the interior of craw_rcd_split(), into the chroma stage or to
- * the output (the tiled launcher does the ring). */
-hipError_t craw_launch_rcd_stream(bool to_inter, const KFrame &f, hipStream_t s);
+/* RCD over the frame, into the chroma stage or to the output: the
+ * interior of craw_rcd_split() streamed and the ring of tiles around it,
+ * in one launch. */
+hipError_t craw_launch_rcd(bool to_inter, const KFrame &f, hipStream_t s);
 hipError_t craw_launch_batch_rcd_stream(bool to_inter, bool gamut, const KFrame *frames,
                                         const uint32_t *tiles, uint32_t ntiles, int grid,
                                         hipStream_t s);
