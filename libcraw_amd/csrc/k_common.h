@@ -104,14 +104,18 @@ KDEV float kdiv_rn(float n, float d)
  * within 2^-52 (relative) of x / range, and a quotient of two floats lies
  * at least 2^-49 from every midpoint of the float grid, so its rounding is
  * the correctly rounded float quotient, the float division's value (three
- * instructions against eleven). The precision form divides in float. */
+ * instructions against eleven). The precision form divides in float,
+ * through kdiv_rn where the frame's constants keep x and the range in its
+ * range (kdiv: x is 0 or at least 2^-40 in magnitude, the range in
+ * [2^-16, 2]). */
 KDEV float kprep(uint32_t raw, int s, const KPrep &p)
 {
     if (p.half)
         return __half2float(__ushort_as_half((unsigned short)raw));
     float r = (float)raw * (1.0f / 65535.0f);
     const float x = r - ksel4(s, p.black[0], p.black[1], p.black[2], p.black[3]);
-    float t = p.fdiv ? x / ksel4(s, p.range[0], p.range[1], p.range[2], p.range[3])
+    const float range = ksel4(s, p.range[0], p.range[1], p.range[2], p.range[3]);
+    float t = p.fdiv ? (p.kdiv ? kdiv_rn(x, range) : x / range)
                      : (float)((double)x * ksel4d(s, p.rrange[0], p.rrange[1], p.rrange[2],
                                                   p.rrange[3]));
     t *= ksel4(s, p.wb[0], p.wb[1], p.wb[2], p.wb[3]);

@@ -123,7 +123,10 @@ int craw_hl_levels(uint32_t w, uint32_t h, int lw[KHL_LEVELS], int lh[KHL_LEVELS
  * 2^-68 above it in magnitude stays one); the low pass divides by 16 and
  * the gradient means multiply by a ninth, so a nonzero dividend is at least
  * 2^-76, while no value exceeds 2^26. A rebuilt R16F mosaic (the highlight
- * stage) holds multiples of 2^-24 below 2^16, inside the same bounds. */
+ * stage) holds multiples of 2^-24 below 2^16, inside the same bounds. The
+ * precision form's prep divides raw / 65535 - black (0, never -0, or at
+ * least 2^-40 in magnitude, at most 2) by the range through kdiv_rn under
+ * the same condition. */
 static bool prep_kdiv_ok(const KPrep &p)
 {
     for (int s = 0; s < 4; s++) {
