@@ -321,7 +321,7 @@ KDEV void dev_rcd_stream(const KFrame &f, int strip)
  * of them, and are dispatched first; the 762 ring tiles then run in the
  * slots beside them instead of in a launch of their own before them (0.120
  * -> 0.105 ms). The ring's LDS is allocated to every workgroup: 32 KB,
- * five a CU, as many as the strips' registers allow. Four waves a SIMD for
+ * which fits five a CU, more than the strips' registers allow. Four waves a SIMD for
  * the fused colour tail without the compression (128 VGPRs; the
  * compiler's own choice is 130, three waves). */
 static_assert(RS_NT == RC_NT, "one workgroup size for strips and ring tiles");
