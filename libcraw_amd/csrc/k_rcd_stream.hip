@@ -31,6 +31,10 @@
 #define RS_NW 4
 #define RS_NT (64 * RS_NW)
 #define RS_EPS 1e-5f
+/* V rows a lane loads ahead of the step that takes them in: a wave's walk
+ * down its strip would otherwise wait on each row's load (0.106 -> 0.096
+ * ms at 5760x3240; 2 rows ahead 0.098, 8 rows 0.096) */
+#define RS_PF 4
 
 static_assert(KG_RS_OUT == 116 && KG_RS_OUT_F == 112 && KG_RS_R % 8 == 0, "strip geometry");
 
@@ -58,6 +62,7 @@ struct RsState {
     RsRow G[4];                 /* rows vt - 6 .. vt - 3 */
     RsRow C[3];                 /* rows vt - 6 .. vt - 4 */
     float acc[3];               /* 4x4 block sums */
+    uint32_t raw[RS_PF];        /* raw sample pairs of V rows vt .. vt + RS_PF - 1 */
 };
 
 /* Gradients of column C at the centre row v[4] (rows v[2] .. v[6]). */
@@ -138,26 +143,35 @@ KDEV float rs_colour(const RsState &s)
  * output reaches): 0 the V row, 1 and the low pass, 2 and the gradients,
  * 3 and the weights and green, 4 and the colour estimate, 5 and the
  * output. */
+/* The lane's two samples of row y (ld0 even) as one 32-bit word: one load
+ * where the mosaic's rows are 4-byte aligned (a uniform branch). */
+KDEV uint32_t rs_load_pair(const KFrame &f, int y, int ld0)
+{
+    const uint16_t *row = f.src + (size_t)y * f.src_pitch;
+
+    if (((uintptr_t)f.src & 3u) == 0 && (f.src_pitch & 1) == 0)
+        return *(const uint32_t *)(row + ld0);
+    return (uint32_t)row[ld0] | (uint32_t)row[ld0 + 1] << 16;
+}
+
 template <bool INTER, bool GAMUT, int P, int L = 5>
 KDEV void rs_step(RsState &s, const KFrame &f, int vt, int ld0, bool out_row, bool out_cols,
                   int X0)
 {
-    /* V row vt: the lane's two samples (ld0 is even) in one 32-bit load
-     * where the mosaic's rows are 4-byte aligned (a uniform branch) */
+    /* V row vt from the pair loaded RS_PF steps before; row vt + RS_PF
+     * loaded now (clamped to the frame: past its last row the values are
+     * never used) */
     rs_shift(s.v);
     {
-        const uint16_t *row = f.src + (size_t)vt * f.src_pitch;
+        const uint32_t pair = s.raw[0];
         const int sb = (vt & 1) * 2;
 
-        if (((uintptr_t)f.src & 3u) == 0 && (f.src_pitch & 1) == 0) {
-            const uint32_t pair = *(const uint32_t *)(row + ld0);
-
-            s.v[6].c[0] = kprep(pair & 0xffffu, sb, f.prep);
-            s.v[6].c[1] = kprep(pair >> 16, sb + 1, f.prep);
-        } else {
-            s.v[6].c[0] = kprep(row[ld0], sb, f.prep);
-            s.v[6].c[1] = kprep(row[ld0 + 1], sb + 1, f.prep);
-        }
+#pragma unroll
+        for (int k = 0; k < RS_PF - 1; k++)
+            s.raw[k] = s.raw[k + 1];
+        s.raw[RS_PF - 1] = rs_load_pair(f, vt + RS_PF < f.H ? vt + RS_PF : f.H - 1, ld0);
+        s.v[6].c[0] = kprep(pair & 0xffffu, sb, f.prep);
+        s.v[6].c[1] = kprep(pair >> 16, sb + 1, f.prep);
     }
 
     /* low pass, row vt - 1 */
@@ -289,6 +303,9 @@ KDEV void dev_rcd_stream(const KFrame &f, int strip)
             s.G[k].c[0] = s.G[k].c[1] = 0.0f;
         s.wp[0].c[0] = s.wp[0].c[1] = s.wp[1].c[0] = s.wp[1].c[1] = 0.0f;
         s.acc[0] = s.acc[1] = s.acc[2] = 0.0f;
+#pragma unroll
+        for (int k = 0; k < RS_PF; k++)
+            s.raw[k] = rs_load_pair(f, ys - 5 + k, ld0);
 
         /* ys is even and nsteps is even: steps alternate parity 0, 1. The
          * first ten steps fill the stages: V rows from ys - 5, the low pass
