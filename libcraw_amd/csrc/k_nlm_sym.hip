@@ -235,8 +235,8 @@ KDEV uint32_t xs_row_load(const uint16_t *row, int i)
 /* Band rows ahead whose raw samples (the guard's input at the output) a
  * lane loads before it needs them: each output row otherwise waited on its
  * load (the exact NLM 0.586 -> 0.567 ms at 5760x3240, the precision form
- * 0.288 -> 0.272; not in the precision form's history band, whose
- * registers it would spill). */
+ * 0.288 -> 0.272). The precision form's history band loads one row ahead
+ * (three would spill its registers): 0.635 -> 0.612 ms a frame with MHC. */
 #define XS_RQ 3
 
 struct XsLane {
@@ -256,7 +256,8 @@ struct XsBand {
     float black, range, inv_range, inv_h2;
     const double *tab;                  /* the exp table in LDS */
     static constexpr int TABN = XsG<MODE, TALL, F16>::TABN;
-    static constexpr bool RQ = !(F16 && HIST);  /* raw samples loaded ahead */
+    static constexpr bool RQ = true;            /* raw samples loaded ahead, */
+    static constexpr int RQN = (F16 && HIST) ? 1 : XS_RQ;   /* RQN rows */
 
     /* The raw mosaic sample at this lane's column of band row rr, the
      * guard's input, clamped into the plane (a clamped position's value
@@ -432,9 +433,9 @@ struct XsBand {
         }
         if constexpr (RQ) {
 #pragma unroll
-            for (int k = 0; k < XS_RQ - 1; k++)
+            for (int k = 0; k < RQN - 1; k++)
                 L.rq[k] = L.rq[k + 1];
-            L.rq[XS_RQ - 1] = raw_at(r + XS_RQ);
+            L.rq[RQN - 1] = raw_at(r + RQN);
         }
 #pragma unroll
         for (int k = 0; k < 3; k++) {
@@ -676,7 +677,7 @@ KDEV void dev_denoise_sym(SmXs<MODE, TALL, F16> &sm, double *tab, const KFrame &
 
     if constexpr (XsBand<MODE, F16, TALL>::RQ) {
 #pragma unroll
-        for (int k = 0; k < XS_RQ; k++)
+        for (int k = 0; k < XsBand<MODE, F16, TALL>::RQN; k++)
             L.rq[k] = band.raw_at(1 + k);
     }
     if constexpr (F16) {
